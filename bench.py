@@ -1,0 +1,198 @@
+"""Benchmark of the whole-graph compute path on one MI355X.
+
+Workload (R-MAT scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, self
+loops and duplicates removed, u32 ids, no vertex permutation; synthetic):
+a "step" is one plus_times SpMV with f32 weights uniform in (0, 1] and f32
+accumulation over the forward CSR, inputs resident in HBM. The JSON line also
+carries the other measured legs (min_plus over u32, or_and over bool).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 22]
+
+Prints one JSON line. The CPU baseline is the oracle (a C++ restatement of
+warren-core's row-parallel pull, oracle/) timed on this host's cores over a
+bounded number of the same SpMVs.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class HipTimer:
+    """HIP events recorded on the stream the kernels run on."""
+
+    def __init__(self, stream):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.stream = C.c_void_p(stream)
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        for e in (self.a, self.b):
+            assert self.hip.hipEventCreate(C.byref(e)) == 0
+
+    def start(self):
+        assert self.hip.hipEventRecord(self.a, self.stream) == 0
+
+    def stop_ms(self):
+        assert self.hip.hipEventRecord(self.b, self.stream) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), self.a, self.b) == 0
+        return ms.value
+
+
+def spmv_bytes(n, m, b_idx, b_val, b_x, b_y, b_off=8):
+    """Compulsory traffic of one SpMV: offsets, targets and values streamed
+    once, x gathered once per node, y written once."""
+    return (n + 1) * b_off + m * (b_idx + b_val) + n * b_x + n * b_y
+
+
+def timed(timer, fn, steps):
+    timer.start()
+    for _ in range(steps):
+        fn()
+    return timer.stop_ms() / steps
+
+
+def cpu_baseline(off, tgt, wf, x, budget_s=15.0):
+    """The oracle's row-parallel pull (ROWS_PER_TASK 4096, f32 fold in
+    ascending column order) on every host core, repeated within a budget."""
+    import oracle
+
+    oracle.build()
+    threads = min(os.cpu_count() or 1, 16)
+    g = oracle.Csr(len(off) - 1, off, tgt, wf)
+    y = np.zeros_like(x)
+    oracle.spmv(g, x, "plus_times", "f32", y=y, threads=threads)  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.spmv(g, x, "plus_times", "f32", y=y, threads=threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or reps >= 50:
+            break
+    return {"value": round(len(tgt) / (el / reps) / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} plus_times f32 SpMVs over the full scale-22 graph, "
+                      f"{el / reps * 1e3:.1f} ms each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding plan is written down, not built); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import pg_warren_amd as W
+
+    t0 = time.time()
+    off, tgt = W.rmat(args.scale, 16, seed=1)
+    n, m = len(off) - 1, len(tgt)
+    wf = W.edge_weights(off, tgt, "uniform01", seed=1)
+    wi = W.edge_weights(off, tgt, "int", seed=2, p0=1, p1=255)
+    log(f"rmat scale {args.scale}: n={n} m={m} ({time.time() - t0:.1f}s)")
+    rng = np.random.default_rng(5)
+    xf = ((rng.integers(0, 2**24, n) + 1) / 2.0**24).astype(np.float32)
+    xu = rng.integers(0, 2**24, n).astype(np.uint32)
+    xu[rng.random(n) < 0.1] = 2**32 - 1
+    xb = (rng.random(n) < 0.5).astype(np.uint8)
+
+    ctx = W.Context(0)
+    gf = W.Graph(ctx, n, off, tgt, wf)
+    gi = W.Graph(ctx, n, off, tgt, wi)
+    log(f"uploaded ({time.time() - t0:.1f}s)")
+    vx = {"f32": W.Vector(ctx, xf), "u32": W.Vector(ctx, xu), "bool": W.Vector(ctx, xb)}
+    vy = {k: W.Vector(ctx, np.zeros_like(v.numpy())) for k, v in vx.items()}
+    timer = HipTimer(ctx.stream)
+    lib, P = ctx.lib, W._lib.Product
+
+    def launcher(g, sr, dt):
+        prod = W.Product(sr).c()
+        gh, xh, yh = g.h, vx[dt].h, vy[dt].h
+
+        def run():
+            code = lib.wg_spmv(ctx.h, gh, C.byref(prod), xh, yh)
+            if code:
+                ctx.check(code)
+        return run
+
+    ctx.set_blocking(False)
+    legs = {}
+    for name, g, sr, dt, bval, bx in [("plus_times_f32", gf, "plus_times", "f32", 4, 4),
+                                      ("min_plus_u32", gi, "min_plus", "u32", 4, 4),
+                                      ("or_and_bool", gf, "lor_land", "bool", 0, 1)]:
+        run = launcher(g, sr, dt)
+        for _ in range(args.warmup):
+            run()
+        ctx.synchronize()
+        ms = timed(timer, run, args.steps)
+        ctx.synchronize()
+        byt = spmv_bytes(n, m, 4, bval, bx, bx)
+        legs[name] = {"ms": round(ms, 4), "gteps": round(m / ms / 1e6, 2),
+                      "gbs": round(byt / ms / 1e6, 1), "bytes": byt}
+        log(f"{name}: {ms * 1e3:.1f} us  {m / ms / 1e6:.1f} GTEPS  {byt / ms / 1e6:.0f} GB/s")
+
+    # Achievable stream bandwidth (a plain copy kernel), for context.
+    nbytes = 1 << 30
+    a = W.Vector(ctx, dtype="u32", id_space=nbytes // 4)
+    b = W.Vector(ctx, dtype="u32", id_space=nbytes // 4)
+    cp = lambda: lib.wg_device_copy(ctx.h, b.device_ptr, a.device_ptr, nbytes)
+    cp()
+    ctx.synchronize()
+    copy_ms = timed(timer, cp, 10)
+    copy_gbs = 2 * nbytes / copy_ms / 1e6
+    log(f"copy kernel: {copy_gbs:.0f} GB/s")
+    ctx.set_blocking(True)
+
+    head = legs["plus_times_f32"]
+    result = {
+        "metric": "GTEPS & HBM GB/s per SpMV/BFS; batched PPR vs plain, same GPU, 1e-9 L1",
+        "value": head["gteps"],
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale} (n={n}, m={m}), plus_times SpMV, "
+                               "f32 weights in (0,1], f32 accumulation",
+                   "graph": "rmat", "scale": args.scale, "edge_factor": 16, "parallelism": "none"},
+        "roofline": {"bound": "hbm", "achieved": head["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "spmv_blocks<float, plus, times>",
+                     "algorithmic_bytes": head["bytes"]},
+        "legs": legs,
+        "copy_kernel_gbs": round(copy_gbs, 1),
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(off, tgt, wf, xf)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * warren_gpu.h -- C ABI of the MI355X (gfx950) implementation of warren-core's
+ * whole-graph compute path: device CSR, semiring SpMV/SpMM, PageRank, BFS and
+ * Lanczos. Plain pointers and sizes only; every entry point returns a status
+ * (0 = ok) whose numbering mirrors warren-core's error variants, and the
+ * message of the last failure is read with wg_last_error().
+ *
+ * Reference interfaces (paths relative to crates/warren-core/src/ of
+ * ExaDev/pg_warren) are cited at each declaration. The ABI is not thread-safe
+ * per context (warren-core's callers drive a kernel from one thread,
+ * compute/workers.rs:1-16); each context owns one HIP stream.
+ */
+#ifndef WARREN_GPU_H
+#define WARREN_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: ComputeError / TraversalError / ArithmeticError
+ * (compute/error.rs:10-210, traversal/error.rs:10-62) ---------------------- */
+enum wg_status {
+  WG_OK = 0,
+  WG_CANCELLED = 1,            /* ComputeError::Cancelled */
+  WG_DTYPE_MISMATCH = 2,       /* ComputeError::DtypeMismatch */
+  WG_ID_SPACE_MISMATCH = 3,    /* ComputeError::IdSpaceMismatch */
+  WG_WIDTH_MISMATCH = 4,       /* ComputeError::WidthMismatch */
+  WG_MEMORY_LIMIT = 5,         /* ComputeError::MemoryLimit */
+  WG_OUT_OF_MEMORY = 6,        /* ComputeError::OutOfMemory */
+  WG_DIVISION_BY_ZERO = 7,     /* ArithmeticError::DivisionByZero */
+  WG_FLOAT_OVERFLOW = 8,       /* ArithmeticError::FloatOverflow */
+  WG_FLOAT_UNDERFLOW = 9,      /* ArithmeticError::FloatUnderflow */
+  WG_BIGINT_OUT_OF_RANGE = 10, /* ArithmeticError::BigintOutOfRange */
+  WG_INVALID_ARGUMENT = 11,    /* UnknownSemiring / UnknownBinaryOp / ZeroWidth */
+  WG_NODE_OUTSIDE_ID_SPACE = 12, /* TraversalError::NodeOutsideIdSpace */
+  WG_NODE_FILTER_ID_SPACE = 13,  /* TraversalError::NodeFilterIdSpace */
+  WG_INVALID_CSR = 14,         /* GraphError from Csr::from_parts (graph/csr/mod.rs:78-137) */
+  WG_UNSUPPORTED = 15,         /* a dtype / semiring pair with no meaning */
+  WG_DEVICE_ERROR = 16         /* a HIP runtime failure */
+};
+
+/* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
+ * [0, 2^32-2] with 2^32-1 standing for i64::MAX (+infinity) ------------------ */
+enum wg_dtype { WG_F32 = 0, WG_F64 = 1, WG_I64 = 2, WG_BOOL = 3, WG_U32 = 4 };
+
+/* ---- semirings in Semiring::ALL order (compute/semiring.rs:149-163) ------ */
+enum wg_semiring {
+  WG_PLUS_TIMES = 0, WG_MIN_PLUS, WG_MAX_PLUS, WG_MAX_TIMES, WG_MIN_TIMES,
+  WG_LOR_LAND, WG_ANY_PAIR, WG_PLUS_PAIR, WG_PLUS_FIRST, WG_PLUS_SECOND,
+  WG_MIN_FIRST, WG_MIN_SECOND, WG_MAX_SECOND
+};
+
+/* ---- binary operators in BinaryOp::ALL order (compute/semiring.rs:42-53) - */
+enum wg_binary_op {
+  WG_OP_NONE = -1, WG_FIRST = 0, WG_SECOND, WG_PAIR, WG_PLUS, WG_TIMES, WG_MIN,
+  WG_MAX, WG_LOR, WG_LAND, WG_ANY
+};
+
+/* ---- traversal direction (graph::Direction, traversal/adjacency.rs:20-60) */
+enum wg_direction { WG_OUT = 0, WG_IN = 1 };
+
+typedef struct wg_context wg_context;
+typedef struct wg_graph wg_graph;
+typedef struct wg_vector wg_vector;
+
+/* ---- context: Workers + ComputeMemory + Interrupts ------------------------
+ * Replaces Workers::new (compute/workers.rs:65-80), ComputeMemory::new
+ * (compute/memory.rs:31-60) and the Interrupts/Cancellation pair
+ * (interrupts.rs:22-70). memory_limit = 0: no limit. */
+int wg_context_create(int device, uint64_t memory_limit, wg_context** out);
+/* Every graph and vector of the context must be destroyed first. */
+void wg_context_destroy(wg_context* ctx);
+/* Copies the last failure's message; returns its status. */
+int wg_last_error(const wg_context* ctx, char* buf, size_t len);
+/* Cancellation::cancel (interrupts.rs:40-70): the next check returns
+ * WG_CANCELLED; wg_cancel_reset clears it. */
+void wg_cancel(wg_context* ctx);
+void wg_cancel_reset(wg_context* ctx);
+/* ComputeMemory::in_use (compute/memory.rs:97-120). */
+uint64_t wg_memory_in_use(const wg_context* ctx);
+/* Waits for the context's stream and reports a deferred kernel error. */
+int wg_synchronize(wg_context* ctx);
+/* 1 (default): every call waits and reports its own errors. 0: calls whose
+ * element type cannot fail (float products) return without waiting, for
+ * back-to-back launches; errors surface at wg_synchronize. */
+void wg_set_blocking(wg_context* ctx, int blocking);
+/* The context's HIP stream (a hipStream_t), for timing with HIP events. */
+void* wg_stream(wg_context* ctx);
+
+/* ---- graphs: GraphCsr with a forward and a reverse CSR ---------------------
+ * Replaces Csr::from_parts (graph/csr/mod.rs:56-137) for the merged
+ * all-relations CSR (graph/csr/graph_csr.rs:15-153) and its reverse, built by
+ * transpose (graph/csr/transpose.rs:16-44). offsets: n+1 u64, targets:
+ * offsets[n] u32, each row strictly ascending; weights: offsets[n] f64 or
+ * NULL (unweighted: every weight reads as 1). Host pointers; copied. */
+int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* offsets,
+                    const uint32_t* targets, const double* weights, wg_graph** out);
+void wg_graph_destroy(wg_graph* g);
+/* n, nnz and whether the graph carries weights. */
+int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted);
+/* The reverse CSR as built on the device (host buffers of n+1, nnz, nnz;
+ * weights may be NULL). */
+int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* offsets,
+                          uint32_t* targets, double* weights);
+
+/* ---- vectors: AnyVector (compute/vector.rs:9-50), row-major id_space x
+ * width, resident in device memory and charged to the context's budget --- */
+int wg_vector_create(wg_context* ctx, int dtype, uint64_t id_space, uint64_t width,
+                     const void* host_values, wg_vector** out);
+void wg_vector_destroy(wg_vector* v);
+int wg_vector_write(wg_context* ctx, wg_vector* v, const void* host_values);
+int wg_vector_read(wg_context* ctx, const wg_vector* v, void* host_values);
+int wg_vector_info(const wg_vector* v, int* dtype, uint64_t* id_space, uint64_t* width);
+void* wg_vector_device_ptr(wg_vector* v);
+
+/* ---- semiring products: Product (compute/product/mod.rs:33-56) ----------- */
+typedef struct {
+  int32_t semiring;        /* wg_semiring */
+  int32_t transpose;       /* y = A^T (x) x: pull over the reverse CSR */
+  int32_t weighted;        /* -1: if the graph has weights (Product::weighted None) */
+  int32_t mask_complement;
+  const wg_vector* mask;   /* width-1 value mask of any dtype, or NULL */
+  int32_t replace;
+  int32_t accumulate;      /* wg_binary_op, or WG_OP_NONE */
+} wg_product;
+
+/* spmv (compute/product/mod.rs:77-92): width-1 x and y of one dtype. */
+int wg_spmv(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+            wg_vector* y);
+/* spmm (compute/product/mod.rs:97-110): x and y of the same width. */
+int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+            wg_vector* y);
+
+/* ---- PageRank: PageRank / pagerank (compute/pagerank/mod.rs:27-151) ----- */
+typedef struct {
+  double damping;     /* probability of following an edge (0.85) */
+  double tol;         /* stop once the L1 change < live nodes * tol */
+  int32_t has_tol;    /* 0: PageRank::tol None, run max_iter iterations */
+  uint32_t max_iter;
+  int32_t weighted;   /* -1: if the graph has weights */
+  int32_t pad;
+} wg_pagerank_opts;
+
+/* NetworkX-style power iteration in f64 over the reverse CSR. tombstones: a
+ * bool vector over the graph (non-zero = tombstoned) or NULL; seed: a width-1
+ * numeric vector (personalisation seed / sum(seed) over live rows) or NULL
+ * (uniform). ranks: an f64 width-1 vector over the graph, written.
+ * Errors as the reference: division by zero (no live node, seed summing to
+ * zero), float overflow/underflow in a step, bool seed (dtype mismatch). */
+int wg_pagerank(wg_context* ctx, const wg_graph* g, const wg_vector* tombstones,
+                const wg_pagerank_opts* opts, const wg_vector* seed, wg_vector* ranks,
+                uint32_t* iterations);
+
+/* ---- measurement ---------------------------------------------------------
+ * No counterpart in the reference: a streaming copy kernel (16-byte loads and
+ * stores, grid-stride) of `bytes` from src to dst, both device buffers of
+ * at least that size, on the context stream. Calibrates the achievable HBM
+ * bandwidth and the profiler's FETCH_SIZE / WRITE_SIZE counters. */
+int wg_device_copy(wg_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WARREN_GPU_H */

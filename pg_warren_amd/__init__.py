@@ -1,0 +1,298 @@
+"""pg_warren_amd: warren-core's whole-graph compute path on one MI355X.
+
+The Python host layer over libwarren_gpu.so, mirroring warren-core's API
+(crates/warren-core/src/): ``Graph`` is the device ``GraphCsr`` (forward +
+reverse CSR, graph/csr/graph_csr.rs:15-153), ``Vector`` an ``AnyVector``
+(compute/vector.rs:9-50), ``Product`` the product descriptor
+(compute/product/mod.rs:33-56) and ``spmv`` / ``spmm`` the products
+(compute/product/mod.rs:77-110). Every call runs the HIP kernels; errors are
+raised as :class:`WarrenError` carrying warren-core's error kind.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import _lib
+
+STATUS_NAMES = {
+    0: "ok", 1: "cancelled", 2: "dtype_mismatch", 3: "id_space_mismatch",
+    4: "width_mismatch", 5: "memory_limit", 6: "out_of_memory",
+    7: "division_by_zero", 8: "float_overflow", 9: "float_underflow",
+    10: "bigint_out_of_range", 11: "invalid_argument",
+    12: "node_outside_id_space", 13: "node_filter_id_space", 14: "invalid_csr",
+    15: "unsupported", 16: "device_error",
+}
+DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
+NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
+             "u32": np.uint32}
+# Semiring::ALL order (compute/semiring.rs:149-163).
+SEMIRINGS = ["plus_times", "min_plus", "max_plus", "max_times", "min_times", "lor_land",
+             "any_pair", "plus_pair", "plus_first", "plus_second", "min_first", "min_second",
+             "max_second"]
+# BinaryOp::ALL order (compute/semiring.rs:42-53).
+BINARY_OPS = ["first", "second", "pair", "plus", "times", "min", "max", "lor", "land", "any"]
+
+
+class WarrenError(RuntimeError):
+    """A failed call: ``name`` is warren-core's error kind."""
+
+    def __init__(self, code, message=""):
+        self.code = code
+        self.name = STATUS_NAMES.get(code, str(code))
+        super().__init__(f"{self.name}: {message}" if message else self.name)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def semiring_index(name):
+    """Semiring::from_str (compute/semiring.rs:219-233): canonical names and
+    the ``or_and`` alias."""
+    canonical = "lor_land" if name == "or_and" else name
+    if canonical not in SEMIRINGS:
+        raise WarrenError(11, f"unknown semiring {name!r}")
+    return SEMIRINGS.index(canonical)
+
+
+def _dtype_of(arr):
+    for k, v in NP_DTYPES.items():
+        if np.dtype(v) == arr.dtype:
+            return k
+    if arr.dtype == np.bool_:
+        return "bool"
+    raise WarrenError(11, f"unsupported element type {arr.dtype}")
+
+
+class Context:
+    """Device, stream, memory budget and cancellation (Workers +
+    ComputeMemory + Interrupts)."""
+
+    def __init__(self, device=0, memory_limit=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        code = self.lib.wg_context_create(device, memory_limit, C.byref(h))
+        if code:
+            raise WarrenError(code, "context creation")
+        self.h = h
+        # Vectors and graphs hold device memory of this context: they are
+        # released before it is (the C ABI requires children to go first).
+        self._children = weakref.WeakSet()
+
+    def check(self, code):
+        if code:
+            buf = C.create_string_buffer(512)
+            self.lib.wg_last_error(self.h, buf, 512)
+            raise WarrenError(code, buf.value.decode(errors="replace"))
+
+    def cancel(self):
+        self.lib.wg_cancel(self.h)
+
+    def cancel_reset(self):
+        self.lib.wg_cancel_reset(self.h)
+
+    def memory_in_use(self):
+        return self.lib.wg_memory_in_use(self.h)
+
+    def synchronize(self):
+        self.check(self.lib.wg_synchronize(self.h))
+
+    def set_blocking(self, blocking):
+        self.lib.wg_set_blocking(self.h, int(blocking))
+
+    @property
+    def stream(self):
+        return self.lib.wg_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for child in list(self._children):
+                child.close()
+            self.lib.wg_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Vector:
+    """A device AnyVector: row-major id_space x width of one dtype."""
+
+    def __init__(self, ctx, values=None, *, dtype=None, id_space=None, width=1):
+        self.ctx = ctx
+        if values is not None:
+            values = np.asarray(values)
+            if dtype is None:
+                dtype = _dtype_of(values)
+            values = np.ascontiguousarray(values, NP_DTYPES[dtype])
+            id_space = values.shape[0]
+            width = 1 if values.ndim == 1 else values.shape[1]
+        self.dtype, self.id_space, self.width = dtype, int(id_space), int(width)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wg_vector_create(ctx.h, DTYPES[dtype], self.id_space, self.width,
+                                           _p(values), C.byref(h)))
+        self.h = h
+        ctx._children.add(self)
+
+    def numpy(self):
+        shape = (self.id_space,) if self.width == 1 else (self.id_space, self.width)
+        out = np.empty(shape, NP_DTYPES[self.dtype])
+        self.ctx.check(self.ctx.lib.wg_vector_read(self.ctx.h, self.h, _p(out)))
+        return out
+
+    def write(self, values):
+        values = np.ascontiguousarray(values, NP_DTYPES[self.dtype])
+        self.ctx.check(self.ctx.lib.wg_vector_write(self.ctx.h, self.h, _p(values)))
+
+    @property
+    def device_ptr(self):
+        return self.ctx.lib.wg_vector_device_ptr(self.h)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h is not None:
+            self.ctx.lib.wg_vector_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A device GraphCsr: the merged all-relations CSR and its reverse."""
+
+    def __init__(self, ctx, n, offsets, targets, weights=None):
+        self.ctx = ctx
+        self.n = int(n)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        targets = np.ascontiguousarray(targets, np.uint32)
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, np.float64)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wg_graph_upload(ctx.h, self.n, _p(offsets), _p(targets), _p(weights),
+                                          C.byref(h)))
+        self.h = h
+        ctx._children.add(self)
+        self.nnz = int(offsets[-1])
+        self.weighted = weights is not None
+
+    def reverse(self):
+        off = np.zeros(self.n + 1, np.uint64)
+        tgt = np.zeros(max(self.nnz, 1), np.uint32)
+        w = np.zeros(max(self.nnz, 1), np.float64) if self.weighted else None
+        self.ctx.check(self.ctx.lib.wg_graph_read_reverse(self.ctx.h, self.h, _p(off), _p(tgt),
+                                                          _p(w)))
+        return off, tgt[:self.nnz], None if w is None else w[:self.nnz]
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h is not None:
+            self.ctx.lib.wg_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Product:
+    """What to multiply and how to write the result (Product,
+    compute/product/mod.rs:33-56). weighted=None: if the graph has weights."""
+
+    def __init__(self, semiring="plus_times", *, transpose=False, weighted=None, mask=None,
+                 mask_complement=False, replace=False, accumulate=None):
+        self.semiring = semiring
+        self.transpose = transpose
+        self.weighted = weighted
+        self.mask = mask
+        self.mask_complement = mask_complement
+        self.replace = replace
+        self.accumulate = accumulate
+
+    def c(self):
+        acc = -1
+        if self.accumulate is not None:
+            if self.accumulate not in BINARY_OPS:
+                raise WarrenError(11, f"unknown binary op {self.accumulate!r}")
+            acc = BINARY_OPS.index(self.accumulate)
+        return _lib.Product(semiring_index(self.semiring), int(self.transpose),
+                            -1 if self.weighted is None else int(self.weighted),
+                            int(self.mask_complement),
+                            None if self.mask is None else self.mask.h.value,
+                            int(self.replace), acc)
+
+
+def spmv(ctx, graph, product, x, y):
+    """y = A (x) x, or A^T (x) x (compute/product/mod.rs:77-92)."""
+    p = product.c()
+    ctx.check(ctx.lib.wg_spmv(ctx.h, graph.h, C.byref(p), x.h, y.h))
+    return y
+
+
+def spmm(ctx, graph, product, x, y):
+    """Y = A (x) X for row-major X of any width (compute/product/mod.rs:97-110)."""
+    p = product.c()
+    ctx.check(ctx.lib.wg_spmm(ctx.h, graph.h, C.byref(p), x.h, y.h))
+    return y
+
+
+def pagerank(ctx, graph, *, damping=0.85, tol=1e-6, max_iter=100, seed=None, tombstones=None,
+             weighted=None, out=None):
+    """PageRank (compute/pagerank/mod.rs:96-151): NetworkX's power
+    iteration in f64, stopping once an iteration's L1 change falls below live
+    nodes x tol (tol None: never). seed / tombstones: device Vectors or
+    array-likes (tombstones as bool). Returns (ranks Vector, iterations)."""
+    if seed is not None and not isinstance(seed, Vector):
+        seed = Vector(ctx, np.asarray(seed))
+    if tombstones is not None and not isinstance(tombstones, Vector):
+        tombstones = Vector(ctx, np.asarray(tombstones, dtype=np.uint8), dtype="bool")
+    if out is None:
+        out = Vector(ctx, dtype="f64", id_space=graph.n)
+    opts = _lib.PagerankOpts(damping, 0.0 if tol is None else tol, int(tol is not None),
+                             max_iter, -1 if weighted is None else int(weighted), 0)
+    it = C.c_uint32()
+    ctx.check(ctx.lib.wg_pagerank(ctx.h, graph.h, None if tombstones is None else tombstones.h,
+                                  C.byref(opts), None if seed is None else seed.h, out.h,
+                                  C.byref(it)))
+    return out, it.value
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetrize=False, threads=0):
+    """A synthetic R-MAT graph's CSR (offsets u64, targets u32): self loops and
+    duplicate edges removed, ids not permuted."""
+    lib = _lib.load()
+    n = 1 << scale
+    cap = (edge_factor << scale) * (2 if symmetrize else 1)
+    off = np.zeros(n + 1, np.uint64)
+    tgt = np.zeros(cap, np.uint32)
+    nnz = np.zeros(1, np.uint64)
+    code = lib.wg_rmat_csr(scale, edge_factor, a, b, c, seed, int(symmetrize), _p(off), _p(tgt),
+                           cap, _p(nnz), threads)
+    if code:
+        raise WarrenError(code, "rmat")
+    return off, tgt[:int(nnz[0])].copy()
+
+
+WEIGHT_KINDS = {"uniform01": 0, "int": 1, "lognormal": 2}
+
+
+def edge_weights(offsets, targets, kind="uniform01", seed=7, p0=0.0, p1=1.0, threads=0):
+    """Per-edge weights as a function of (seed, source, target)."""
+    lib = _lib.load()
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    targets = np.ascontiguousarray(targets, np.uint32)
+    w = np.zeros(len(targets), np.float64)
+    code = lib.wg_edge_weights(len(offsets) - 1, _p(offsets), _p(targets), seed,
+                               WEIGHT_KINDS[kind], p0, p1, _p(w), threads)
+    if code:
+        raise WarrenError(code, "edge_weights")
+    return w

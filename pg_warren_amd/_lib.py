@@ -1,0 +1,81 @@
+"""ctypes binding of libwarren_gpu.so (the C ABI in include/warren_gpu.h).
+
+The library is built in-tree by pg_warren_amd/build.py. Loading fails loudly
+when it is missing: there is no CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwarren_gpu.so")
+
+P = C.c_void_p
+U64 = C.c_uint64
+U32 = C.c_uint32
+I32 = C.c_int
+I64 = C.c_int64
+D = C.c_double
+
+_LIB = None
+
+
+class Product(C.Structure):
+    """wg_product: compute/product/mod.rs:33-56."""
+    _fields_ = [("semiring", C.c_int32), ("transpose", C.c_int32), ("weighted", C.c_int32),
+                ("mask_complement", C.c_int32), ("mask", P), ("replace", C.c_int32),
+                ("accumulate", C.c_int32)]
+
+
+class PagerankOpts(C.Structure):
+    """wg_pagerank_opts: compute/pagerank/mod.rs:27-47."""
+    _fields_ = [("damping", C.c_double), ("tol", C.c_double), ("has_tol", C.c_int32),
+                ("max_iter", C.c_uint32), ("weighted", C.c_int32), ("pad", C.c_int32)]
+
+
+# (name, restype, argtypes) of every exported entry point.
+SIGNATURES = [
+    ("wg_context_create", I32, [I32, U64, P]),
+    ("wg_context_destroy", None, [P]),
+    ("wg_last_error", I32, [P, C.c_char_p, C.c_size_t]),
+    ("wg_cancel", None, [P]),
+    ("wg_cancel_reset", None, [P]),
+    ("wg_memory_in_use", U64, [P]),
+    ("wg_synchronize", I32, [P]),
+    ("wg_set_blocking", None, [P, I32]),
+    ("wg_stream", P, [P]),
+    ("wg_graph_upload", I32, [P, U64, P, P, P, P]),
+    ("wg_graph_destroy", None, [P]),
+    ("wg_graph_info", I32, [P, P, P, P]),
+    ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
+    ("wg_vector_create", I32, [P, I32, U64, U64, P, P]),
+    ("wg_vector_destroy", None, [P]),
+    ("wg_vector_write", I32, [P, P, P]),
+    ("wg_vector_read", I32, [P, P, P]),
+    ("wg_vector_info", I32, [P, P, P, P]),
+    ("wg_vector_device_ptr", P, [P]),
+    ("wg_spmv", I32, [P, P, P, P, P]),
+    ("wg_spmm", I32, [P, P, P, P, P]),
+    ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
+    ("wg_device_copy", I32, [P, P, P, U64]),
+    ("wg_rmat_csr", I32, [U32, U32, D, D, D, U64, I32, P, P, U64, P, I32]),
+    ("wg_edge_weights", I32, [U64, P, P, U64, I32, D, D, P, I32]),
+]
+
+
+def load():
+    """Loads libwarren_gpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python pg_warren_amd/build.py` "
+                "(or __graft_entry__.build()); pg_warren_amd has no CPU fallback")
+        lib = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB

@@ -1,0 +1,516 @@
+// Context, memory budget, cancellation, device vectors and the device CSR
+// pair of libwarren_gpu.
+#include <algorithm>
+#include <cstdio>
+#include <new>
+
+#include "semiring.hpp"
+#include "wg_internal.hpp"
+
+namespace wg {
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case WG_F32: return 4;
+    case WG_F64: return 8;
+    case WG_I64: return 8;
+    case WG_BOOL: return 1;
+    case WG_U32: return 4;
+  }
+  return 0;
+}
+
+int fail(wg_context* ctx, int status, const std::string& msg) {
+  if (ctx) {
+    ctx->last_status = status;
+    ctx->last_msg = msg;
+  }
+  return status;
+}
+
+int ok(wg_context* ctx) {
+  if (ctx) {
+    ctx->last_status = WG_OK;
+    ctx->last_msg.clear();
+  }
+  return WG_OK;
+}
+
+int hip_fail(wg_context* ctx, hipError_t e, const char* what) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  return fail(ctx, e == hipErrorOutOfMemory ? WG_OUT_OF_MEMORY : WG_DEVICE_ERROR, buf);
+}
+
+int dalloc(wg_context* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  uint64_t used = ctx->in_use.load();
+  if (ctx->limit && used + bytes > ctx->limit) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "memory limit: %llu bytes in use, %zu more asked, limit %llu",
+             (unsigned long long)used, bytes, (unsigned long long)ctx->limit);
+    return fail(ctx, WG_MEMORY_LIMIT, buf);
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    char buf[160];
+    snprintf(buf, sizeof buf, "out of device memory allocating %zu bytes", bytes);
+    return fail(ctx, WG_OUT_OF_MEMORY, buf);
+  }
+  ctx->in_use += bytes;
+  return WG_OK;
+}
+
+void dfree(wg_context* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 8;
+  (void)hipFree(p);
+  ctx->in_use -= bytes;
+}
+
+int ensure_scratch(wg_context* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return WG_OK;
+  if (ctx->scratch) {
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, ctx->scratch, ctx->scratch_bytes);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 1 << 20);
+  int s = dalloc(ctx, &ctx->scratch, want);
+  if (s) return s;
+  ctx->scratch_bytes = want;
+  return WG_OK;
+}
+
+__global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+int clear_error(wg_context* ctx) {
+  set_u64<<<1, 1, 0, ctx->stream>>>(ctx->d_err, ~0ull);
+  WG_HIP(ctx, hipGetLastError());
+  return WG_OK;
+}
+
+static const char* status_text(int s) {
+  switch (s) {
+    case WG_FLOAT_OVERFLOW: return "value out of range: overflow";
+    case WG_FLOAT_UNDERFLOW: return "value out of range: underflow";
+    case WG_BIGINT_OUT_OF_RANGE: return "bigint out of range";
+    case WG_DIVISION_BY_ZERO: return "division by zero";
+  }
+  return "kernel error";
+}
+
+int collect_error(wg_context* ctx, const char* what) {
+  WG_HIP(ctx, hipMemcpyAsync(ctx->h_err, ctx->d_err, 8, hipMemcpyDeviceToHost, ctx->stream));
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long w = *ctx->h_err;
+  if (w != ~0ull) {
+    int status = (int)(w & 0xff);
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s (at index %llu)", what, status_text(status), w >> 8);
+    return fail(ctx, status, buf);
+  }
+  return ok(ctx);
+}
+
+bool is_cancelled(wg_context* ctx) { return ctx->cancelled.load() != 0; }
+
+template <class E>
+__global__ void cast_weights(const double* __restrict__ w, E* __restrict__ out, uint64_t m,
+                             unsigned long long* err) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < m; i += stride) {
+    uint32_t e = 0;
+    E v = weight_cast<E>(w[i], e);
+    if (e) raise_error(err, i, e);
+    out[i] = v;
+  }
+}
+
+int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out) {
+  *out = nullptr;
+  if (!csr.weights) return WG_OK;
+  if (dtype == WG_F64) {
+    *out = csr.weights;
+    return WG_OK;
+  }
+  if (csr.wstatus[dtype] >= 0) {
+    *out = csr.wview[dtype];
+    if (csr.wstatus[dtype] != WG_OK)
+      return fail(ctx, csr.wstatus[dtype], std::string("edge weight cast: ") +
+                                               status_text(csr.wstatus[dtype]));
+    return WG_OK;
+  }
+  size_t es = dtype_size(dtype);
+  void* p = nullptr;
+  int s = dalloc(ctx, &p, csr.nnz * es);
+  if (s) return s;
+  if ((s = clear_error(ctx))) return s;
+  uint64_t m = csr.nnz;
+  unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 65536);
+  if (grid == 0) grid = 1;
+  switch (dtype) {
+    case WG_F32: cast_weights<float><<<grid, 256, 0, ctx->stream>>>(csr.weights, (float*)p, m, ctx->d_err); break;
+    case WG_I64: cast_weights<int64_t><<<grid, 256, 0, ctx->stream>>>(csr.weights, (int64_t*)p, m, ctx->d_err); break;
+    case WG_BOOL: cast_weights<uint8_t><<<grid, 256, 0, ctx->stream>>>(csr.weights, (uint8_t*)p, m, ctx->d_err); break;
+    case WG_U32: cast_weights<uint32_t><<<grid, 256, 0, ctx->stream>>>(csr.weights, (uint32_t*)p, m, ctx->d_err); break;
+  }
+  WG_HIP(ctx, hipGetLastError());
+  s = collect_error(ctx, "edge weight cast");
+  csr.wview[dtype] = p;
+  csr.wstatus[dtype] = s;
+  *out = p;
+  return s;
+}
+
+// Host partition of a CSR's rows into SpMV blocks.
+static void partition_rows(const std::vector<uint64_t>& off, std::vector<Block>& blocks,
+                           uint32_t& nlong, uint32_t& npartials, uint64_t& maxdeg) {
+  uint64_t n = off.size() - 1;
+  blocks.clear();
+  nlong = npartials = 0;
+  maxdeg = 0;
+  uint64_t r = 0;
+  while (r < n) {
+    uint64_t deg = off[r + 1] - off[r];
+    maxdeg = std::max(maxdeg, deg);
+    if (deg > kBlockNnz) {
+      uint32_t np = (uint32_t)((deg + kBlockNnz - 1) / kBlockNnz);
+      for (uint32_t p = 0; p < np; ++p)
+        blocks.push_back(Block{(uint32_t)r, 0, p, np, nlong, npartials, 0, 0});
+      ++nlong;
+      npartials += np;
+      ++r;
+      continue;
+    }
+    uint64_t r0 = r, nnz = 0;
+    while (r < n && r - r0 < kBlockRows) {
+      uint64_t d = off[r + 1] - off[r];
+      if (d > kBlockNnz || nnz + d > kBlockNnz) break;
+      maxdeg = std::max(maxdeg, d);
+      nnz += d;
+      ++r;
+    }
+    blocks.push_back(Block{(uint32_t)r0, (uint32_t)(r - r0), 0, 1, 0, 0, 0, 0});
+  }
+}
+
+static int upload_csr(wg_context* ctx, DevCsr& c, uint64_t n, const uint64_t* off,
+                      const uint32_t* tgt, const double* w) {
+  c.n = n;
+  c.nnz = off[n];
+  c.h_offsets.assign(off, off + n + 1);
+  int s;
+  if ((s = dalloc(ctx, (void**)&c.offsets, (n + 1) * 8))) return s;
+  if ((s = dalloc(ctx, (void**)&c.targets, c.nnz * 4))) return s;
+  WG_HIP(ctx, hipMemcpyAsync(c.offsets, off, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (c.nnz)
+    WG_HIP(ctx, hipMemcpyAsync(c.targets, tgt, c.nnz * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (w) {
+    if ((s = dalloc(ctx, (void**)&c.weights, c.nnz * 8))) return s;
+    if (c.nnz)
+      WG_HIP(ctx, hipMemcpyAsync(c.weights, w, c.nnz * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  std::vector<Block> blocks;
+  partition_rows(c.h_offsets, blocks, c.nlong, c.npartials, c.max_degree);
+  c.nblocks = (uint32_t)blocks.size();
+  if ((s = dalloc(ctx, (void**)&c.blocks, blocks.size() * sizeof(Block)))) return s;
+  if (!blocks.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.blocks, blocks.data(), blocks.size() * sizeof(Block),
+                               hipMemcpyHostToDevice, ctx->stream));
+  if ((s = dalloc(ctx, (void**)&c.tickets, std::max<uint32_t>(c.nlong, 1) * 4))) return s;
+  WG_HIP(ctx, hipMemsetAsync(c.tickets, 0, std::max<uint32_t>(c.nlong, 1) * 4, ctx->stream));
+  // The copies read pageable host memory the caller may free on return.
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return WG_OK;
+}
+
+static void free_csr(wg_context* ctx, DevCsr& c) {
+  dfree(ctx, c.offsets, (c.n + 1) * 8);
+  dfree(ctx, c.targets, c.nnz * 4);
+  if (c.weights) dfree(ctx, c.weights, c.nnz * 8);
+  dfree(ctx, c.blocks, c.nblocks * sizeof(Block));
+  dfree(ctx, c.tickets, std::max<uint32_t>(c.nlong, 1) * 4);
+  for (int d = 0; d < 5; ++d)
+    if (c.wview[d]) dfree(ctx, c.wview[d], c.nnz * dtype_size(d));
+  if (c.spmm_ready) {
+    dfree(ctx, c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8);
+    dfree(ctx, c.spmm_ticket_of_row, c.n * 4);
+    dfree(ctx, c.spmm_pbase, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
+    dfree(ctx, c.spmm_tickets, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
+  }
+  c = DevCsr();
+}
+
+int prepare_spmm(wg_context* ctx, DevCsr& c) {
+  if (c.spmm_ready) return WG_OK;
+  std::vector<uint32_t> extra, ticket_of_row(c.n, ~0u), pbase;
+  uint32_t np_total = 0;
+  for (uint64_t r = 0; r < c.n; ++r) {
+    uint64_t deg = c.h_offsets[r + 1] - c.h_offsets[r];
+    if (deg <= kSpmmPiece) continue;
+    uint32_t np = (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece);
+    ticket_of_row[r] = (uint32_t)pbase.size();
+    pbase.push_back(np_total);
+    np_total += np;
+    for (uint32_t p = 1; p < np; ++p) {
+      extra.push_back((uint32_t)r);
+      extra.push_back(p);
+    }
+  }
+  c.spmm_nextra = (uint32_t)(extra.size() / 2);
+  c.spmm_nlong = (uint32_t)pbase.size();
+  c.spmm_npartials = np_total;
+  int s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_ticket_of_row, c.n * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_pbase, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_tickets, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
+  if (!extra.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.spmm_extra, extra.data(), extra.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (c.n)
+    WG_HIP(ctx, hipMemcpyAsync(c.spmm_ticket_of_row, ticket_of_row.data(), c.n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (!pbase.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.spmm_pbase, pbase.data(), pbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  WG_HIP(ctx, hipMemsetAsync(c.spmm_tickets, 0, std::max<uint32_t>(c.spmm_nlong, 1) * 4, ctx->stream));
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  c.spmm_ready = true;
+  return WG_OK;
+}
+
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return WG_DEVICE_ERROR;
+  if (device < 0 || device >= count) return WG_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return WG_DEVICE_ERROR;
+  wg_context* ctx = new (std::nothrow) wg_context();
+  if (!ctx) return WG_OUT_OF_MEMORY;
+  ctx->device = device;
+  ctx->limit = memory_limit;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_err, 8) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return WG_DEVICE_ERROR;
+  }
+  *out = ctx;
+  return WG_OK;
+}
+
+void wg_context_destroy(wg_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int wg_last_error(const wg_context* ctx, char* buf, size_t len) {
+  if (!ctx) return WG_INVALID_ARGUMENT;
+  if (buf && len) {
+    size_t k = std::min(len - 1, ctx->last_msg.size());
+    memcpy(buf, ctx->last_msg.data(), k);
+    buf[k] = 0;
+  }
+  return ctx->last_status;
+}
+
+void wg_cancel(wg_context* ctx) {
+  if (ctx) ctx->cancelled = 1;
+}
+void wg_cancel_reset(wg_context* ctx) {
+  if (ctx) ctx->cancelled = 0;
+}
+uint64_t wg_memory_in_use(const wg_context* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int wg_synchronize(wg_context* ctx) {
+  if (!ctx) return WG_INVALID_ARGUMENT;
+  return collect_error(ctx, "deferred kernel");
+}
+
+void wg_set_blocking(wg_context* ctx, int blocking) {
+  if (ctx) ctx->blocking = blocking;
+}
+
+void* wg_stream(wg_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int wg_vector_create(wg_context* ctx, int dtype, uint64_t id_space, uint64_t width,
+                     const void* host, wg_vector** out) {
+  *out = nullptr;
+  if (dtype < 0 || dtype > 4) return fail(ctx, WG_INVALID_ARGUMENT, "unknown dtype");
+  if (width == 0) return fail(ctx, WG_INVALID_ARGUMENT, "zero width");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  wg_vector* v = new (std::nothrow) wg_vector();
+  if (!v) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  v->ctx = ctx;
+  v->dtype = dtype;
+  v->id_space = id_space;
+  v->width = width;
+  v->bytes = id_space * width * dtype_size(dtype);
+  int s = dalloc(ctx, &v->data, v->bytes);
+  if (s) {
+    delete v;
+    return s;
+  }
+  hipError_t e = host ? hipMemcpy(v->data, host, v->bytes, hipMemcpyHostToDevice)
+                      : hipMemsetAsync(v->data, 0, v->bytes ? v->bytes : 8, ctx->stream);
+  if (e == hipSuccess && !host) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    dfree(ctx, v->data, v->bytes);
+    delete v;
+    return hip_fail(ctx, e, "vector create");
+  }
+  *out = v;
+  return ok(ctx);
+}
+
+void wg_vector_destroy(wg_vector* v) {
+  if (!v) return;
+  (void)hipStreamSynchronize(v->ctx->stream);
+  dfree(v->ctx, v->data, v->bytes);
+  delete v;
+}
+
+int wg_vector_write(wg_context* ctx, wg_vector* v, const void* host) {
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (v->bytes) WG_HIP(ctx, hipMemcpy(v->data, host, v->bytes, hipMemcpyHostToDevice));
+  return ok(ctx);
+}
+
+int wg_vector_read(wg_context* ctx, const wg_vector* v, void* host) {
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (v->bytes) WG_HIP(ctx, hipMemcpy(host, v->data, v->bytes, hipMemcpyDeviceToHost));
+  return ok(ctx);
+}
+
+int wg_vector_info(const wg_vector* v, int* dtype, uint64_t* id_space, uint64_t* width) {
+  if (!v) return WG_INVALID_ARGUMENT;
+  if (dtype) *dtype = v->dtype;
+  if (id_space) *id_space = v->id_space;
+  if (width) *width = v->width;
+  return WG_OK;
+}
+
+void* wg_vector_device_ptr(wg_vector* v) { return v ? v->data : nullptr; }
+
+// Csr::from_parts checks (graph/csr/mod.rs:78-137).
+static int validate(wg_context* ctx, uint64_t n, const uint64_t* off, const uint32_t* tgt) {
+  if (n >= (1ull << 32)) return fail(ctx, WG_INVALID_CSR, "id space exceeds u32 targets");
+  if (off[0] != 0) return fail(ctx, WG_INVALID_CSR, "first offset is not zero");
+  for (uint64_t r = 0; r < n; ++r)
+    if (off[r + 1] < off[r]) return fail(ctx, WG_INVALID_CSR, "offsets not monotone at row " + std::to_string(r));
+  for (uint64_t r = 0; r < n; ++r)
+    for (uint64_t p = off[r]; p < off[r + 1]; ++p) {
+      if (tgt[p] >= n) return fail(ctx, WG_INVALID_CSR, "target outside id space in row " + std::to_string(r));
+      if (p > off[r] && tgt[p - 1] >= tgt[p])
+        return fail(ctx, WG_INVALID_CSR, "row " + std::to_string(r) + " not sorted");
+    }
+  return WG_OK;
+}
+
+int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint32_t* tgt,
+                    const double* w, wg_graph** out) {
+  *out = nullptr;
+  if (!off) return fail(ctx, WG_INVALID_CSR, "no offsets");
+  int s = validate(ctx, n, off, tgt);
+  if (s) return s;
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  wg_graph* g = new (std::nothrow) wg_graph();
+  if (!g) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  g->ctx = ctx;
+  g->n = n;
+  g->nnz = off[n];
+  g->weighted = w != nullptr;
+  s = upload_csr(ctx, g->fwd, n, off, tgt, w);
+  if (!s) {
+    // The reverse CSR by counting-sort transpose, scanning sources in order
+    // so every reverse row ascends (graph/csr/transpose.rs:24-44).
+    uint64_t m = off[n];
+    std::vector<uint64_t> ro(n + 2, 0);
+    std::vector<uint32_t> rt(m);
+    std::vector<double> rw(w ? m : 0);
+    for (uint64_t p = 0; p < m; ++p) ro[tgt[p] + 2]++;
+    for (uint64_t r = 1; r < n + 2; ++r) ro[r] += ro[r - 1];
+    for (uint64_t src = 0; src < n; ++src)
+      for (uint64_t p = off[src]; p < off[src + 1]; ++p) {
+        uint64_t pos = ro[tgt[p] + 1]++;
+        rt[pos] = (uint32_t)src;
+        if (w) rw[pos] = w[p];
+      }
+    ro.pop_back();
+    s = upload_csr(ctx, g->rev, n, ro.data(), rt.data(), w ? rw.data() : nullptr);
+  }
+  if (s) {
+    free_csr(ctx, g->fwd);
+    free_csr(ctx, g->rev);
+    delete g;
+    return s;
+  }
+  *out = g;
+  return ok(ctx);
+}
+
+void wg_graph_destroy(wg_graph* g) {
+  if (!g) return;
+  (void)hipStreamSynchronize(g->ctx->stream);
+  free_csr(g->ctx, g->fwd);
+  free_csr(g->ctx, g->rev);
+  delete g;
+}
+
+int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted) {
+  if (!g) return WG_INVALID_ARGUMENT;
+  if (n) *n = g->n;
+  if (nnz) *nnz = g->nnz;
+  if (weighted) *weighted = g->weighted;
+  return WG_OK;
+}
+
+int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* off, uint32_t* tgt,
+                          double* w) {
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  WG_HIP(ctx, hipMemcpy(off, g->rev.offsets, (g->n + 1) * 8, hipMemcpyDeviceToHost));
+  if (g->nnz) {
+    WG_HIP(ctx, hipMemcpy(tgt, g->rev.targets, g->nnz * 4, hipMemcpyDeviceToHost));
+    if (w && g->rev.weights)
+      WG_HIP(ctx, hipMemcpy(w, g->rev.weights, g->nnz * 8, hipMemcpyDeviceToHost));
+  }
+  return ok(ctx);
+}
+
+}  // extern "C"
+
+namespace wg {
+__global__ __launch_bounds__(256) void copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                              uint64_t n16) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace wg
+
+extern "C" int wg_device_copy(wg_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes & 15)) return wg::fail(ctx, WG_INVALID_ARGUMENT, "copy size must be a multiple of 16");
+  uint64_t n16 = bytes / 16;
+  unsigned grid = (unsigned)std::min<uint64_t>((n16 + 255) / 256, 256 * 32);
+  if (grid == 0) return WG_OK;
+  wg::copy16<<<grid, 256, 0, ctx->stream>>>((uint4*)dst, (const uint4*)src, n16);
+  WG_HIP(ctx, hipGetLastError());
+  if (ctx->blocking) WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return wg::ok(ctx);
+}

@@ -1,0 +1,456 @@
+// PageRank by power iteration on gfx950 (compute/pagerank/mod.rs:96-287).
+//
+// One fused kernel per iteration over the reverse CSR's row blocks:
+//   gather   y_i = sum_j w_ji * msg_j       (f64, the SpMV block scheme)
+//   advance  x_i' = d (y_i + dangling p_i) + (1 - d) p_i, SQL float8 checks
+//   spread   msg_i' = x_i' / out_i, or 0 and x_i' joins the next dangling sum
+// Block partials of the L1 change and of the next dangling rank are summed
+// in a fixed order by the last block to finish, which also takes the stop
+// decision (L1 change < live nodes * tol, in SQL order) into a device flag;
+// later launches of the batch see the flag and return at once, so the host
+// enqueues iterations in batches and reads the flag once per batch.
+#include <algorithm>
+#include <cmath>
+
+#include "semiring.hpp"
+#include "wg_internal.hpp"
+
+namespace wg {
+
+struct PrState {
+  double dangling[2];  // dangling rank read by the iteration of each parity
+  double delta;
+  uint32_t stop;
+  uint32_t iterations;
+  uint32_t arrivals;
+  uint32_t pad;
+};
+
+struct PrArgs {
+  const Block* blocks;
+  const uint64_t* off;
+  const uint32_t* src;
+  const double* w;
+  const double* out;    // weighted out-degree (PlusFirst over the forward CSR)
+  const double* p;      // personalisation, or null: uniform p_uniform
+  double p_uniform;
+  const uint8_t* tomb;  // tombstoned rows, or null
+  double damping, teleport, bound;
+  int has_tol;
+  uint32_t nblocks, nlong;
+  double* x;
+  double* part;        // 2 * (nblocks + nlong): change, then dangling
+  double* piece;       // split rows' gather partials
+  uint32_t* tickets;
+  PrState* st;
+  unsigned long long* err;
+};
+
+__device__ __forceinline__ bool pr_halted(const PrArgs& a) {
+  return __hip_atomic_load(&a.st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+         __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ~0ull;
+}
+
+// advance + spread of one row; returns its L1 change and adds its dangling
+// rank to `dang`.
+__device__ __forceinline__ double pr_row(const PrArgs& a, uint64_t row, double y, double dangling,
+                                         double* msg_out, double& dang, uint32_t& err) {
+  double xo = a.x[row], next = 0.0;
+  if (!(a.tomb && a.tomb[row])) {
+    const double p = a.p ? a.p[row] : a.p_uniform;
+    double flowed = sqlf<ADD_>(y, sqlf<MUL_>(dangling, p, err), err);
+    next = sqlf<ADD_>(sqlf<MUL_>(a.damping, flowed, err), sqlf<MUL_>(a.teleport, p, err), err);
+  }
+  a.x[row] = next;
+  const double o = a.out[row];
+  if (sql_lt(0.0, o)) {
+    msg_out[row] = sqlf<DIV_>(next, o, err);
+  } else {
+    dang += next;
+    msg_out[row] = 0.0;
+  }
+  return fabs(next - xo);
+}
+
+// The last block of a launch: fixed-order sums of the partials, the stop
+// decision, and the arrival counter reset.
+__device__ void pr_finish(const PrArgs& a, int parity) {
+  __shared__ double sc[256], sd[256];
+  const uint32_t tid = threadIdx.x, total = a.nblocks + a.nlong;
+  double c = 0.0, d = 0.0;
+  for (uint32_t i = tid; i < total; i += 256) {
+    c += __hip_atomic_load(&a.part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    d += __hip_atomic_load(&a.part[total + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  sc[tid] = c;
+  sd[tid] = d;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      sc[tid] += sc[tid + s];
+      sd[tid] += sd[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.st->delta = sc[0];
+    a.st->dangling[parity ^ 1] = sd[0];
+    a.st->iterations += 1;
+    if (a.has_tol && sql_lt(sc[0], a.bound)) a.st->stop = 1;
+    a.st->arrivals = 0;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const uint32_t tid = threadIdx.x;
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (tid == 0)
+    for (int k = 0; k < 4; ++k) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+template <bool HW>
+__global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const double* __restrict__ msg_in,
+                                                            double* __restrict__ msg_out, int parity) {
+  __shared__ double prod[kBlockNnz];
+  __shared__ uint32_t rs[kBlockRows + 1];
+  __shared__ double wsum[8];
+  __shared__ int last;
+  if (pr_halted(a)) return;
+  const double dangling = a.st->dangling[parity];
+  const Block b = a.blocks[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  uint32_t e_err = 0;
+  uint64_t e0, e1;
+  if (b.nrows) {
+    e0 = a.off[b.row0];
+    e1 = a.off[b.row0 + b.nrows];
+  } else {
+    const uint64_t rb = a.off[b.row0], re = a.off[b.row0 + 1];
+    e0 = rb + (uint64_t)b.piece * kBlockNnz;
+    e1 = e0 + kBlockNnz < re ? e0 + kBlockNnz : re;
+  }
+  const uint32_t cnt = (uint32_t)(e1 - e0);
+  {
+    uint32_t col[kBlockNnz / kBlockThreads];
+    double wv[kBlockNnz / kBlockThreads];
+#pragma unroll
+    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+      const uint32_t i = tid + k * kBlockThreads;
+      if (i < cnt) {
+        col[k] = a.src[e0 + i];
+        if (HW) wv[k] = a.w[e0 + i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+      const uint32_t i = tid + k * kBlockThreads;
+      if (i < cnt) prod[i] = HW ? wv[k] * msg_in[col[k]] : msg_in[col[k]];
+    }
+  }
+  if (b.nrows)
+    for (uint32_t k = tid; k <= b.nrows; k += kBlockThreads) rs[k] = (uint32_t)(a.off[b.row0 + k] - e0);
+  __syncthreads();
+
+  double change = 0.0, dang = 0.0;
+  if (b.nrows) {
+    const uint32_t nr = b.nrows;
+    uint32_t T = 1;
+    if (nr < kBlockThreads) {
+      uint32_t q = kBlockThreads / nr;
+      T = 1u << (31 - __builtin_clz(q));
+      if (T > 64) T = 64;
+    }
+    const uint32_t groups = kBlockThreads / T, g = tid / T, l = tid % T;
+    for (uint32_t r = g; r < nr; r += groups) {
+      const uint32_t s0 = rs[r], s1 = rs[r + 1];
+      const uint32_t len = s1 - s0, chunk = (len + T - 1) / T;
+      uint32_t s = s0 + l * chunk, t = s + chunk < s1 ? s + chunk : s1;
+      double acc = 0.0;
+      for (uint32_t i = s; i < t; ++i) acc += prod[i];
+      for (uint32_t d = 1; d < T; d <<= 1) {
+        double o = __shfl_down(acc, d, T);
+        if ((l & (2 * d - 1)) == 0) acc += o;
+      }
+      if (l == 0) change += pr_row(a, (uint64_t)b.row0 + r, acc, dangling, msg_out, dang, e_err);
+    }
+    double c = block_sum(change, wsum);
+    double dd = block_sum(dang, wsum + 4);
+    if (tid == 0) {
+      a.part[blockIdx.x] = c;
+      a.part[a.nblocks + a.nlong + blockIdx.x] = dd;
+    }
+  } else {
+    const uint32_t chunk = (cnt + kBlockThreads - 1) / kBlockThreads;
+    uint32_t s = tid * chunk, t = s + chunk < cnt ? s + chunk : cnt;
+    double acc = 0.0;
+    for (uint32_t i = s; i < t; ++i) acc += prod[i];
+    acc = block_sum(acc, wsum);
+    if (tid == 0) {
+      a.part[blockIdx.x] = 0.0;
+      a.part[a.nblocks + a.nlong + blockIdx.x] = 0.0;
+      a.piece[b.pbase + b.piece] = acc;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      uint32_t prev = atomicAdd(&a.tickets[b.ticket], 1u);
+      if (prev == b.npieces - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double y = __hip_atomic_load(&a.piece[b.pbase], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t p = 1; p < b.npieces; ++p)
+          y += __hip_atomic_load(&a.piece[b.pbase + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.tickets[b.ticket] = 0;
+        double dl = 0.0;
+        double ch = pr_row(a, b.row0, y, dangling, msg_out, dl, e_err);
+        // A split row's sums have a slot of their own, so the fixed-order
+        // total does not depend on which piece finished last.
+        a.part[a.nblocks + b.ticket] = ch;
+        a.part[2 * a.nblocks + a.nlong + b.ticket] = dl;
+      }
+    }
+  }
+  if (e_err) raise_error(a.err, b.row0, e_err);
+  if (tid == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    uint32_t prev = atomicAdd(&a.st->arrivals, 1u);
+    last = prev == gridDim.x - 1;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  if (last) pr_finish(a, parity);
+}
+
+// msg = x / out (or 0, the rank joining the dangling sum) for the first
+// iteration, and the live-row count / seed sums of the set-up.
+__global__ __launch_bounds__(256) void pr_spread0(const double* __restrict__ x,
+                                                  const double* __restrict__ out,
+                                                  double* __restrict__ msg, uint64_t n,
+                                                  double* __restrict__ part,
+                                                  unsigned long long* err) {
+  __shared__ double wsum[4];
+  uint32_t e_err = 0;
+  double dang = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double o = out[i];
+    if (sql_lt(0.0, o)) {
+      msg[i] = sqlf<DIV_>(x[i], o, e_err);
+    } else {
+      dang += x[i];
+      msg[i] = 0.0;
+    }
+    if (e_err) {
+      raise_error(err, i, e_err);
+      e_err = 0;
+    }
+  }
+  double t = block_sum(dang, wsum);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Fills x with `start` at live rows and 0 at tombstoned ones, and counts the
+// live rows.
+__global__ __launch_bounds__(256) void pr_init(double* __restrict__ x, const uint8_t* __restrict__ tomb,
+                                               uint64_t n, double start, unsigned long long* live) {
+  uint64_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const bool dead = tomb && tomb[i];
+    x[i] = dead ? 0.0 : start;
+    c += !dead;
+  }
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(live, (unsigned long long)c);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void pr_seed_sum(const S* __restrict__ seed, const uint8_t* __restrict__ tomb,
+                                                   uint64_t n, double* __restrict__ part) {
+  __shared__ double wsum[4];
+  double s = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    if (!(tomb && tomb[i])) s += (double)seed[i];
+  double t = block_sum(s, wsum);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void pr_shares(const S* __restrict__ seed, const uint8_t* __restrict__ tomb,
+                                                 uint64_t n, double total, double* __restrict__ p,
+                                                 unsigned long long* err) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    uint32_t e = 0;
+    p[i] = (tomb && tomb[i]) ? 0.0 : sqlf<MUL_>((double)seed[i], total, e);
+    if (e) raise_error(err, i, e);
+  }
+}
+
+__global__ void pr_set_state(PrState* st, double dangling0) {
+  st->dangling[0] = dangling0;
+  st->dangling[1] = 0.0;
+  st->delta = 0.0;
+  st->stop = 0;
+  st->iterations = 0;
+  st->arrivals = 0;
+}
+
+constexpr unsigned kReduceGrid = 1024;
+
+// Device buffers of one PageRank run, charged to the context.
+struct PrBuffers {
+  wg_context* ctx;
+  std::vector<std::pair<void*, size_t>> held;
+  explicit PrBuffers(wg_context* c) : ctx(c) {}
+  template <class T>
+  int get(T** p, size_t count) {
+    void* q = nullptr;
+    int s = dalloc(ctx, &q, count * sizeof(T));
+    if (s) return s;
+    held.push_back({q, count * sizeof(T)});
+    *p = (T*)q;
+    return WG_OK;
+  }
+  ~PrBuffers() {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& h : held) dfree(ctx, h.first, h.second);
+  }
+};
+
+int launch_out_weights(wg_context* ctx, wg_graph* g, bool weighted, const double* x, double* out);
+
+static double host_sum(const std::vector<double>& v) {
+  double s = 0.0;
+  for (double d : v) s += d;
+  return s;
+}
+
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector* tombstones,
+                           const wg_pagerank_opts* o, const wg_vector* seed, wg_vector* result,
+                           uint32_t* iterations) {
+  if (!ctx || !gc || !o || !result) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  wg_graph* g = const_cast<wg_graph*>(gc);
+  const uint64_t n = g->n;
+  if (result->dtype != WG_F64 || result->width != 1 || result->id_space != n)
+    return fail(ctx, WG_INVALID_ARGUMENT, "the ranks vector must be f64, width 1, over the graph");
+  if (tombstones && (tombstones->dtype != WG_BOOL || tombstones->id_space != n || tombstones->width != 1))
+    return fail(ctx, WG_INVALID_ARGUMENT, "tombstones must be a bool vector over the graph");
+  if (iterations) *iterations = 0;
+  if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const uint8_t* tomb = tombstones ? (const uint8_t*)tombstones->data : nullptr;
+  double* x = (double*)result->data;
+  PrBuffers buf(ctx);
+  int s;
+  unsigned long long* live_d;
+  if ((s = buf.get(&live_d, 1))) return s;
+  WG_HIP(ctx, hipMemsetAsync(live_d, 0, 8, st));
+  if ((s = clear_error(ctx))) return s;
+  // x = 1 / live at live rows, the live count first (mod.rs:105-114).
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, kReduceGrid));
+  pr_init<<<grid, 256, 0, st>>>(x, tomb, n, 0.0, live_d);
+  unsigned long long live = 0;
+  WG_HIP(ctx, hipMemcpyAsync(&live, live_d, 8, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  uint32_t herr = 0;
+  const double nodes = (double)live;
+  const double start = sqlf<DIV_>(1.0, nodes, herr);
+  if (herr) return fail(ctx, (int)herr, "pagerank: 1 / live nodes: division by zero");
+  pr_init<<<grid, 256, 0, st>>>(x, tomb, n, start, live_d);
+  // out = PlusFirst(A) . x over the forward CSR (mod.rs:115-117).
+  const bool weighted = o->weighted < 0 ? g->weighted : o->weighted != 0;
+  double *out, *msg0, *msg1, *p = nullptr, *part, *piece;
+  if ((s = buf.get(&out, std::max<uint64_t>(n, 1)))) return s;
+  if ((s = launch_out_weights(ctx, g, weighted, x, out))) return s;
+  // Personalisation (mod.rs:118-121, 155-199).
+  if (seed) {
+    if (seed->dtype == WG_BOOL)
+      return fail(ctx, WG_DTYPE_MISMATCH, "dtype mismatch: expected f64, found bool");
+    if (seed->width != 1) {
+      if (o->max_iter != 0)
+        return fail(ctx, WG_WIDTH_MISMATCH, "width mismatch: expected " + std::to_string(seed->width) + ", found 1");
+    } else {
+      if (seed->id_space != n) return fail(ctx, WG_ID_SPACE_MISMATCH, "seed id space mismatch");
+      if ((s = buf.get(&part, kReduceGrid))) return s;
+      switch (seed->dtype) {
+        case WG_F64: pr_seed_sum<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, part); break;
+        case WG_F32: pr_seed_sum<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, part); break;
+        case WG_I64: pr_seed_sum<int64_t><<<grid, 256, 0, st>>>((const int64_t*)seed->data, tomb, n, part); break;
+        case WG_U32: pr_seed_sum<uint32_t><<<grid, 256, 0, st>>>((const uint32_t*)seed->data, tomb, n, part); break;
+      }
+      std::vector<double> hp(grid);
+      WG_HIP(ctx, hipMemcpyAsync(hp.data(), part, grid * 8, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      const double total = sqlf<DIV_>(1.0, host_sum(hp), herr);
+      if (herr) return fail(ctx, (int)herr, "pagerank: 1 / sum(seed): division by zero");
+      if ((s = buf.get(&p, std::max<uint64_t>(n, 1)))) return s;
+      switch (seed->dtype) {
+        case WG_F64: pr_shares<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, total, p, ctx->d_err); break;
+        case WG_F32: pr_shares<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, total, p, ctx->d_err); break;
+        case WG_I64: pr_shares<int64_t><<<grid, 256, 0, st>>>((const int64_t*)seed->data, tomb, n, total, p, ctx->d_err); break;
+        case WG_U32: pr_shares<uint32_t><<<grid, 256, 0, st>>>((const uint32_t*)seed->data, tomb, n, total, p, ctx->d_err); break;
+      }
+      if ((s = collect_error(ctx, "pagerank seed shares"))) return s;
+    }
+  }
+  const double teleport = sqlf<SUB_>(1.0, o->damping, herr);
+  if (herr) return fail(ctx, (int)herr, "pagerank: 1 - damping");
+  uint32_t bound_err = 0;
+  const double bound = o->has_tol ? sqlf<MUL_>(nodes, o->tol, bound_err) : 0.0;
+  if (o->max_iter == 0) {
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    return ok(ctx);
+  }
+  // First messages and dangling rank.
+  DevCsr& rev = g->rev;
+  if ((s = buf.get(&msg0, std::max<uint64_t>(n, 1)))) return s;
+  if ((s = buf.get(&msg1, std::max<uint64_t>(n, 1)))) return s;
+  const uint32_t nparts = 2 * (rev.nblocks + rev.nlong);
+  if ((s = buf.get(&part, std::max<uint32_t>(std::max<uint32_t>(nparts, kReduceGrid), 1)))) return s;
+  if ((s = buf.get(&piece, std::max<uint32_t>(rev.npartials, 1)))) return s;
+  PrState* state;
+  if ((s = buf.get(&state, 1))) return s;
+  pr_spread0<<<grid, 256, 0, st>>>(x, out, msg0, n, part, ctx->d_err);
+  {
+    std::vector<double> hp(grid);
+    WG_HIP(ctx, hipMemcpyAsync(hp.data(), part, grid * 8, hipMemcpyDeviceToHost, st));
+    if ((s = collect_error(ctx, "pagerank"))) return s;
+    pr_set_state<<<1, 1, 0, st>>>(state, host_sum(hp));
+  }
+  const double* wv = nullptr;
+  if (weighted && rev.weights) wv = rev.weights;
+  PrArgs a{rev.blocks, rev.offsets, rev.targets, wv, out, p, start, tomb, o->damping, teleport,
+           bound, o->has_tol && !bound_err, rev.nblocks, rev.nlong, x, part, piece, rev.tickets,
+           state, ctx->d_err};
+  // With a tolerance whose bound fails, the reference raises that error
+  // after the first iteration (mod.rs:144-147).
+  const uint32_t max_iter = bound_err ? 1 : o->max_iter;
+  uint32_t launched = 0;
+  PrState hs{};
+  const uint32_t batch = 8;
+  while (launched < max_iter) {
+    if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+    const uint32_t k = std::min(batch, max_iter - launched);
+    for (uint32_t i = 0; i < k; ++i, ++launched) {
+      const int parity = launched & 1;
+      const double* mi = parity ? msg1 : msg0;
+      double* mo = parity ? msg0 : msg1;
+      if (rev.nblocks == 0) break;
+      if (wv)
+        pr_iterate<true><<<rev.nblocks, kBlockThreads, 0, st>>>(a, mi, mo, parity);
+      else
+        pr_iterate<false><<<rev.nblocks, kBlockThreads, 0, st>>>(a, mi, mo, parity);
+    }
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, st));
+    if ((s = collect_error(ctx, "pagerank"))) return s;
+    if (hs.stop || rev.nblocks == 0) break;
+  }
+  if (iterations) *iterations = rev.nblocks ? hs.iterations : max_iter;
+  if (bound_err) return fail(ctx, (int)bound_err, "pagerank: live nodes * tol out of range");
+  return ok(ctx);
+}

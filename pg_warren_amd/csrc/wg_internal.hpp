@@ -1,0 +1,122 @@
+// Host-side internals of libwarren_gpu: the context (stream, memory budget,
+// cancellation, error word), device vectors and the device CSR pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/warren_gpu.h"
+
+namespace wg {
+
+// A row block of the SpMV partition: either up to kBlockRows whole rows
+// whose entries total at most kBlockNnz (nrows > 0), or one piece of a row
+// too long for a block (nrows == 0, piece < npieces). Rows split into
+// several pieces combine their partials in piece order through `ticket`.
+struct Block {
+  uint32_t row0;
+  uint32_t nrows;
+  uint32_t piece;
+  uint32_t npieces;
+  uint32_t ticket;  // long rows: index of the row's arrival counter
+  uint32_t pbase;   // long rows: first partial slot of the row
+  uint32_t pad0, pad1;
+};
+
+constexpr uint32_t kBlockThreads = 256;
+constexpr uint32_t kBlockNnz = 2048;   // entries staged in LDS per block
+constexpr uint32_t kBlockRows = 512;   // rows a short block may hold
+
+// A CSR resident in device memory, with its SpMV partition and the edge
+// weights cast once per element type (SemiringElement::from_weight,
+// compute/semiring.rs:243-246); a cast that fails keeps its status.
+struct DevCsr {
+  uint64_t n = 0, nnz = 0;
+  uint64_t* offsets = nullptr;   // n + 1
+  uint32_t* targets = nullptr;   // nnz
+  double* weights = nullptr;     // nnz or null
+  std::vector<uint64_t> h_offsets;  // host copy of the offsets (partitioning)
+  Block* blocks = nullptr;
+  uint32_t nblocks = 0;
+  uint32_t nlong = 0;        // rows split into pieces
+  uint32_t npartials = 0;    // pieces of split rows
+  uint32_t* tickets = nullptr;  // nlong counters, zero between launches
+  uint64_t max_degree = 0;
+  void* wview[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int wstatus[5] = {-1, -1, -1, -1, -1};  // -1: not cast yet
+  // SpMM: long rows split into pieces of at most kSpmmPiece entries.
+  uint32_t* spmm_extra = nullptr;  // (row, piece) pairs beyond piece 0
+  uint32_t spmm_nextra = 0;
+  uint32_t* spmm_ticket_of_row = nullptr;  // per row: ticket index or ~0
+  uint32_t* spmm_pbase = nullptr;          // per ticket: first partial slot
+  uint32_t spmm_nlong = 0, spmm_npartials = 0;
+  uint32_t* spmm_tickets = nullptr;
+  bool spmm_ready = false;
+};
+
+constexpr uint32_t kSpmmPiece = 2048;
+
+}  // namespace wg
+
+struct wg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t limit = 0;
+  std::atomic<uint64_t> in_use{0};
+  std::atomic<int> cancelled{0};
+  int blocking = 1;
+  int last_status = 0;
+  std::string last_msg;
+  unsigned long long* d_err = nullptr;  // (index << 8 | status), ~0 when clear
+  unsigned long long* h_err = nullptr;  // pinned mirror
+  void* scratch = nullptr;              // partial results of split rows
+  size_t scratch_bytes = 0;
+};
+
+struct wg_vector {
+  wg_context* ctx = nullptr;
+  int dtype = 0;
+  uint64_t id_space = 0, width = 0;
+  void* data = nullptr;
+  size_t bytes = 0;
+};
+
+struct wg_graph {
+  wg_context* ctx = nullptr;
+  uint64_t n = 0, nnz = 0;
+  bool weighted = false;
+  wg::DevCsr fwd, rev;
+};
+
+namespace wg {
+
+size_t dtype_size(int dtype);
+int fail(wg_context* ctx, int status, const std::string& msg);
+int ok(wg_context* ctx);
+int hip_fail(wg_context* ctx, hipError_t e, const char* what);
+// Device memory charged to the context's budget before it is allocated
+// (ComputeMemory::charge, compute/memory.rs:97).
+int dalloc(wg_context* ctx, void** p, size_t bytes);
+void dfree(wg_context* ctx, void* p, size_t bytes);
+int ensure_scratch(wg_context* ctx, size_t bytes);
+// Clears the device error word (enqueued on the context stream).
+int clear_error(wg_context* ctx);
+// Waits for the stream and turns a raised error word into a status.
+int collect_error(wg_context* ctx, const char* what);
+bool is_cancelled(wg_context* ctx);
+// The weights of `csr` cast to `dtype` (cached); returns the cast status.
+int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out);
+int prepare_spmm(wg_context* ctx, DevCsr& csr);
+
+#define WG_HIP(ctx, expr)                                   \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return ::wg::hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+}  // namespace wg

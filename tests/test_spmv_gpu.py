@@ -1,0 +1,292 @@
+"""Parity of the HIP semiring products with the oracle.
+
+Bit-exact for every monoid except float plus, whose rounding depends on the
+grouping of the sum: there the check is exactness on integer-valued floats
+(every partial sum representable) and, for random floats, a rounding bound
+|gpu - oracle| <= 2 * len * eps * sum|terms| per row.
+"""
+import numpy as np
+import pytest
+
+import pg_warren_amd as W
+
+pytestmark = pytest.mark.gpu
+
+I64MAX = 2**63 - 1
+I64MIN = -(2**63)
+U32INF = 2**32 - 1
+EPS = {"f32": np.finfo(np.float32).eps, "f64": np.finfo(np.float64).eps}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = W.Context(0)
+    yield c
+    _graphs.clear()
+    c.close()
+
+
+_graphs = {}
+
+
+def gpu_graph(ctx, csr):
+    key = id(csr)
+    if key not in _graphs:
+        _graphs[key] = (csr, W.Graph(ctx, csr.n, csr.offsets, csr.targets, csr.weights))
+    return _graphs[key][1]
+
+
+def run_gpu(ctx, csr, x, semiring, dtype, *, transpose=False, weighted=None, mask=None,
+            mask_complement=False, replace=False, accumulate=None, y0=None, spmm=False):
+    g = gpu_graph(ctx, csr)
+    xv = W.Vector(ctx, x, dtype=dtype)
+    yv = W.Vector(ctx, np.zeros_like(x) if y0 is None else y0, dtype=dtype)
+    mv = None if mask is None else W.Vector(ctx, mask)
+    p = W.Product(semiring, transpose=transpose, weighted=weighted, mask=mv,
+                  mask_complement=mask_complement, replace=replace, accumulate=accumulate)
+    (W.spmm if spmm else W.spmv)(ctx, g, p, xv, yv)
+    return yv.numpy()
+
+
+def run_oracle(oracle, csr, x, semiring, dtype, *, transpose=False, weighted=None, mask=None,
+               mask_complement=False, replace=False, accumulate=None, y0=None, width=1):
+    w = csr.weights is not None if weighted is None else weighted
+    y = np.zeros_like(x) if y0 is None else y0.copy()
+    return oracle.spmv(csr, x.reshape(-1), semiring, dtype, rev=csr.transpose() if transpose else None,
+                       transpose=transpose, weighted=w, mask=mask, mask_complement=mask_complement,
+                       replace=replace, accumulate=accumulate, y=y.reshape(-1), width=width)
+
+
+def example(oracle):
+    src, dst, w = [0, 0, 1, 2], [1, 2, 2, 0], [12.0, 3.0, 4.0, 5.0]
+    return oracle.csr_from_edges(4, src, dst, w)
+
+
+X4 = np.array([1.0, 2.0, 3.0, 4.0])
+
+
+def test_known_answers(ctx, oracle):
+    """compute/product/tests.rs:51-106 through the device."""
+    g = example(oracle)
+    assert list(run_gpu(ctx, g, X4, "plus_times", "f64")) == [33.0, 12.0, 5.0, 0.0]
+    assert list(run_gpu(ctx, g, X4, "plus_times", "f64", transpose=True)) == [15.0, 12.0, 11.0, 0.0]
+    assert list(run_gpu(ctx, g, X4, "plus_times", "f64", weighted=False)) == [5.0, 3.0, 1.0, 0.0]
+    assert list(run_gpu(ctx, g, X4, "min_plus", "f64")) == [6.0, 7.0, 6.0, float("inf")]
+    mask = np.array([1, 0, -7, 0], np.int64)
+    y100 = np.full(4, 100.0)
+
+    def w(c, r, a):
+        return list(run_gpu(ctx, g, X4, "plus_times", "f64", mask=mask, mask_complement=c,
+                            replace=r, accumulate=a, y0=y100))
+    assert w(False, False, None) == [33.0, 100.0, 5.0, 100.0]
+    assert w(False, True, None) == [33.0, 0.0, 5.0, 0.0]
+    assert w(False, False, "plus") == [133.0, 100.0, 105.0, 100.0]
+    assert w(True, False, None) == [100.0, 12.0, 100.0, 0.0]
+    assert w(True, True, "plus") == [0.0, 112.0, 0.0, 100.0]
+    x2 = np.array([[1.0, 10.0], [2.0, 20.0], [3.0, 30.0], [4.0, 40.0]])
+    y = run_gpu(ctx, g, x2, "plus_times", "f64", transpose=True, spmm=True)
+    assert list(y.reshape(-1)) == [15.0, 150.0, 12.0, 120.0, 11.0, 110.0, 0.0, 0.0]
+
+
+def test_i64_min_plus_supersteps(ctx, oracle):
+    """compute/product/tests.rs:160-185."""
+    g = oracle.csr_from_edges(3, [0, 2], [1, 1], [1.0, 1.0])
+    d = np.array([0, I64MAX, I64MAX], np.int64)
+    for _ in range(2):
+        d = run_gpu(ctx, g, d, "min_plus", "i64", transpose=True, accumulate="min", y0=d)
+        assert list(d) == [0, 1, I64MAX]
+    y = run_gpu(ctx, g, np.array([5, I64MIN, I64MIN], np.int64), "max_plus", "i64",
+                transpose=True, accumulate="max", y0=np.full(3, I64MIN, np.int64))
+    assert list(y) == [I64MIN, 6, I64MIN]
+
+
+def test_errors(ctx, oracle):
+    g = example(oracle)
+    with pytest.raises(W.WarrenError) as e:
+        run_gpu(ctx, g, np.full(4, I64MAX, np.int64), "plus_times", "i64")
+    assert e.value.name == "bigint_out_of_range"
+    with pytest.raises(W.WarrenError) as e:
+        run_gpu(ctx, g, np.full(4, 2**32 - 2, np.uint32), "plus_times", "u32")
+    assert e.value.name == "bigint_out_of_range"
+    big = oracle.csr_from_edges(2, [0], [1], [1e300])
+    with pytest.raises(W.WarrenError) as e:
+        run_gpu(ctx, big, np.ones(2, np.float32), "plus_times", "f32")
+    assert e.value.name == "float_overflow"
+    # (x) second never reads the weight, so the bad weight is not cast.
+    assert list(run_gpu(ctx, big, np.ones(2, np.float32), "plus_second", "f32")) == [1.0, 0.0]
+    gg = gpu_graph(ctx, g)
+    x = W.Vector(ctx, np.ones(4))
+    for y, name in [(W.Vector(ctx, np.zeros(4, np.int64)), "dtype_mismatch"),
+                    (W.Vector(ctx, np.zeros(3)), "id_space_mismatch"),
+                    (W.Vector(ctx, np.zeros((4, 2))), "width_mismatch")]:
+        with pytest.raises(W.WarrenError) as e:
+            W.spmv(ctx, gg, W.Product(), x, y)
+        assert e.value.name == name
+    with pytest.raises(W.WarrenError) as e:
+        W.spmv(ctx, gg, W.Product("plus_min"), x, W.Vector(ctx, np.zeros(4)))
+    assert e.value.name == "invalid_argument"
+    with pytest.raises(W.WarrenError) as e:
+        run_gpu(ctx, g, np.ones(4, np.uint32), "max_plus", "u32")
+    assert e.value.name == "unsupported"
+    ctx.cancel()
+    with pytest.raises(W.WarrenError) as e:
+        W.spmv(ctx, gg, W.Product(), x, W.Vector(ctx, np.zeros(4)))
+    assert e.value.name == "cancelled"
+    ctx.cancel_reset()
+
+
+def hub_graph(oracle, seed, n=5000, m=40000):
+    """Random edges plus a hub whose row and column exceed one block (2048
+    entries), so long rows split into pieces both ways."""
+    rng = np.random.default_rng(seed)
+    src = np.concatenate([rng.integers(0, n, m), np.zeros(n - 1, np.int64),
+                          np.arange(1, n)])
+    dst = np.concatenate([rng.integers(0, n, m), np.arange(1, n), np.full(n - 1, 7)])
+    w = rng.integers(1, 4, len(src)).astype(np.float64)
+    return oracle.csr_from_edges(n, src, dst, w)
+
+
+def values(dtype, n, rng, add):
+    if dtype in ("f32", "f64"):
+        v = rng.integers(0, 5, n).astype(np.float64)
+        if add in ("min", "max"):
+            v[rng.random(n) < 0.05] = np.nan
+            v[rng.random(n) < 0.05] = -0.0
+        return v.astype(W.NP_DTYPES[dtype])
+    if dtype == "i64":
+        v = rng.integers(-3, 6, n).astype(np.int64)
+        if add == "min":
+            v[rng.random(n) < 0.1] = I64MAX
+        if add == "max":
+            v[rng.random(n) < 0.1] = I64MIN
+        return v
+    if dtype == "u32":
+        v = rng.integers(0, 6, n).astype(np.uint32)
+        if add == "min":
+            v[rng.random(n) < 0.1] = U32INF
+        return v
+    return (rng.random(n) < 0.5).astype(np.uint8)
+
+
+ADD_OF = {"plus_times": "plus", "min_plus": "min", "max_plus": "max", "max_times": "max",
+          "min_times": "min", "lor_land": "lor", "any_pair": "any", "plus_pair": "plus",
+          "plus_first": "plus", "plus_second": "plus", "min_first": "min", "min_second": "min",
+          "max_second": "max"}
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool", "u32"])
+def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype):
+    """Integer-valued inputs: every partial sum is exact, so even float plus
+    must agree to the bit; NaN and -0 exercise the SQL min/max order."""
+    g = hub_graph(oracle, 1)
+    rng = np.random.default_rng(2)
+    for sr in W.SEMIRINGS:
+        add = ADD_OF[sr]
+        if dtype == "u32" and add == "max":
+            continue
+        x = values(dtype, g.n, rng, add)
+        for transpose in (False, True):
+            for weighted in (None, False):
+                try:
+                    want = run_oracle(oracle, g, x, sr, dtype, transpose=transpose,
+                                      weighted=weighted)
+                except oracle.OracleError as e:
+                    with pytest.raises(W.WarrenError) as ge:
+                        run_gpu(ctx, g, x, sr, dtype, transpose=transpose, weighted=weighted)
+                    assert ge.value.name == e.name, (sr, dtype, transpose)
+                    continue
+                got = run_gpu(ctx, g, x, sr, dtype, transpose=transpose, weighted=weighted)
+                assert got.tobytes() == want.tobytes(), (sr, dtype, transpose, weighted)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool"])
+def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype):
+    g = hub_graph(oracle, 3)
+    rng = np.random.default_rng(4)
+    for sr in ["plus_times", "min_plus", "lor_land", "max_second"]:
+        x = values(dtype, g.n, rng, ADD_OF[sr])
+        y0 = values(dtype, g.n, rng, ADD_OF[sr])
+        mask = rng.integers(0, 2, g.n).astype(np.float32)
+        for complement in (False, True):
+            for replace in (False, True):
+                for acc in (None, "plus", "min", "second", "lor"):
+                    kw = dict(mask=mask, mask_complement=complement, replace=replace,
+                              accumulate=acc, y0=y0, transpose=bool(replace))
+                    try:
+                        want = run_oracle(oracle, g, x, sr, dtype, **kw)
+                    except oracle.OracleError as e:
+                        with pytest.raises(W.WarrenError) as ge:
+                            run_gpu(ctx, g, x, sr, dtype, **kw)
+                        assert ge.value.name == e.name
+                        continue
+                    got = run_gpu(ctx, g, x, sr, dtype, **kw)
+                    assert got.tobytes() == want.tobytes(), (sr, dtype, complement, replace, acc)
+
+
+def test_float_plus_within_rounding_bound(ctx, oracle):
+    g = hub_graph(oracle, 5)
+    rng = np.random.default_rng(6)
+    rev = g.transpose()
+    for dtype in ("f32", "f64"):
+        x = (rng.random(g.n) * 2 - 1).astype(W.NP_DTYPES[dtype])
+        for transpose in (False, True):
+            got = run_gpu(ctx, g, x, "plus_times", dtype, transpose=transpose).astype(np.float64)
+            want = run_oracle(oracle, g, x, "plus_times", dtype, transpose=transpose).astype(np.float64)
+            m = rev if transpose else g
+            absx = np.abs(x.astype(np.float64))
+            abssum = oracle.spmv(m, absx, "plus_times", "f64")
+            deg = np.diff(m.offsets.astype(np.int64))
+            bound = 2 * deg * EPS[dtype] * abssum
+            assert np.all(np.abs(got - want) <= bound + 1e-300)
+
+
+@pytest.mark.parametrize("width", [2, 5, 64, 70])
+def test_spmm_matches_the_oracle(ctx, oracle, width):
+    g = hub_graph(oracle, 7, n=3000, m=20000)
+    rng = np.random.default_rng(width)
+    for sr, dtype in [("plus_times", "f64"), ("min_plus", "i64"), ("lor_land", "bool"),
+                      ("plus_times", "f32"), ("min_plus", "u32")]:
+        x = values(dtype, g.n * width, rng, ADD_OF[sr]).reshape(g.n, width)
+        for transpose in (False, True):
+            want = run_oracle(oracle, g, x, sr, dtype, transpose=transpose, width=width)
+            got = run_gpu(ctx, g, x, sr, dtype, transpose=transpose, spmm=True)
+            assert got.tobytes() == want.tobytes(), (sr, dtype, width, transpose)
+
+
+def test_rmat_scale18_three_semirings(ctx, oracle):
+    """The benchmark's three products on R-MAT scale 18: min_plus over u32
+    and or_and over bool bit-exact, plus_times f32 within its bound."""
+    off, tgt = W.rmat(18, 16, seed=11)
+    wf = W.edge_weights(off, tgt, "uniform01", seed=1)
+    wi = W.edge_weights(off, tgt, "int", seed=2, p0=1, p1=255)
+    n = len(off) - 1
+    rng = np.random.default_rng(12)
+    gf = oracle.Csr(n, off, tgt, wf)
+    gi = oracle.Csr(n, off, tgt, wi)
+    xf = ((rng.integers(0, 2**24, n) + 1) / 2.0**24).astype(np.float32)
+    got = run_gpu(ctx, gf, xf, "plus_times", "f32").astype(np.float64)
+    want = run_oracle(oracle, gf, xf, "plus_times", "f32").astype(np.float64)
+    exact = oracle.spmv(gf, xf.astype(np.float64), "plus_times", "f64")
+    deg = np.diff(off.astype(np.int64))
+    assert np.all(np.abs(got - want) <= 2 * deg * EPS["f32"] * exact + 1e-30)
+    xu = rng.integers(0, 2**24, n).astype(np.uint32)
+    xu[rng.random(n) < 0.1] = U32INF
+    got = run_gpu(ctx, gi, xu, "min_plus", "u32")
+    want = run_oracle(oracle, gi, xu, "min_plus", "u32")
+    assert got.tobytes() == want.tobytes()
+    # The same through the reference's i64 type: u32 is its exact image.
+    xi = np.where(xu == U32INF, I64MAX, xu.astype(np.int64))
+    wi64 = run_oracle(oracle, gi, xi, "min_plus", "i64")
+    assert np.array_equal(np.where(wi64 == I64MAX, U32INF, wi64).astype(np.uint32), got)
+    xb = (rng.random(n) < 0.5).astype(np.uint8)
+    got = run_gpu(ctx, gf, xb, "lor_land", "bool")
+    want = run_oracle(oracle, gf, xb, "lor_land", "bool")
+    assert got.tobytes() == want.tobytes()
+
+
+def test_repeated_launches_are_bit_identical(ctx, oracle):
+    g = hub_graph(oracle, 9)
+    x = np.random.default_rng(1).random(g.n).astype(np.float32)
+    first = run_gpu(ctx, g, x, "plus_times", "f32", transpose=True)
+    for _ in range(3):
+        assert run_gpu(ctx, g, x, "plus_times", "f32", transpose=True).tobytes() == first.tobytes()
