@@ -90,6 +90,35 @@ def cpu_baseline(off, tgt, wf, x, budget_s=15.0):
                       f"{el / reps * 1e3:.1f} ms each"}
 
 
+def oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs, bfs_depths, pr_gpu, pr_iters):
+    """Each measured result against the oracle on the same inputs: the f32
+    product's largest relative error against the f64 product, mismatches of
+    the bit-exact legs, and PageRank's L1 distance at the same iteration
+    count (part of the CPU-baseline leg: the oracle runs on the host)."""
+    import oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    n = len(off) - 1
+    gf = oracle.Csr(n, off, tgt, wf)
+    gi = oracle.Csr(n, off, tgt, wi)
+    exact = oracle.spmv(gf, xf.astype(np.float64), "plus_times", "f64", threads=threads)
+    got = outputs["f32"].astype(np.float64)
+    rel = np.abs(got - exact) / np.maximum(np.abs(exact), 1e-300)
+    mp = oracle.spmv(gi, xu, "min_plus", "u32", threads=threads)
+    oa = oracle.spmv(gf, xb, "lor_land", "bool", threads=threads)
+    bfs_mismatch = 0
+    for s, d in bfs_depths:
+        bfs_mismatch += int(np.count_nonzero(oracle.bfs_depths(gf, [s])[0] != d))
+    pr_ref, it_ref = oracle.pagerank(oracle.Csr(n, off, tgt), tol=1e-10, max_iter=100,
+                                     threads=threads)
+    return {"plus_times_f32_max_rel_err_vs_f64": float(rel[np.abs(exact) > 0].max()),
+            "min_plus_u32_mismatches": int(np.count_nonzero(mp != outputs["u32"])),
+            "or_and_bool_mismatches": int(np.count_nonzero(oa != outputs["bool"])),
+            "bfs_depth_mismatches_2_sources": bfs_mismatch,
+            "pagerank_l1_vs_f64": float(np.abs(pr_ref - pr_gpu).sum()),
+            "pagerank_iterations_oracle_vs_gpu": [it_ref, pr_iters]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +193,54 @@ def main():
     copy_gbs = 2 * nbytes / copy_ms / 1e6
     log(f"copy kernel: {copy_gbs:.0f} GB/s")
     ctx.set_blocking(True)
+    for name, g, sr, dt in [("plus_times_f32", gf, "plus_times", "f32"),
+                            ("min_plus_u32", gi, "min_plus", "u32"),
+                            ("or_and_bool", gf, "lor_land", "bool")]:
+        legs[name]["tuned_variant"] = W.spmv_variant(g, sr, dt)
+    outputs = {k: vy[k].numpy() for k in vy}
+
+    # BFS: full-depth direction-optimising search from 64 sources with an
+    # out-edge; TEPS as Graph500 counts them (out-degree summed over the
+    # nodes reached) over the wall time of the call.
+    deg = np.diff(off.astype(np.int64))
+    cand = np.flatnonzero(deg > 0)
+    sources = np.random.default_rng(64).choice(cand, 64, replace=False)
+    depth = W.Vector(ctx, dtype="u32", id_space=n)
+    via = W.Vector(ctx, dtype="u32", id_space=n)
+    W.bfs_levels(ctx, gf, [int(sources[0])], depth=depth, via=via)  # warm
+    teps, times, levels, pulled = [], [], [], []
+    bfs_depths = []
+    for s in sources:
+        t1 = time.perf_counter()
+        _, _, st = W.bfs_levels(ctx, gf, [int(s)], depth=depth, via=via)
+        el = time.perf_counter() - t1
+        d = depth.numpy()
+        reached = d != 0xFFFFFFFF
+        teps.append(deg[reached].sum() / el)
+        times.append(el)
+        pulled.append(st["levels_pulled"])
+        levels.append(int(d[reached].max()))
+        if len(bfs_depths) < 2:
+            bfs_depths.append((int(s), d))
+    hm = len(teps) / sum(1.0 / t for t in teps)
+    legs["bfs_64_sources"] = {"gteps_harmonic_mean": round(hm / 1e9, 3),
+                              "ms_mean": round(1e3 * float(np.mean(times)), 3),
+                              "levels_mean": float(np.mean(levels)),
+                              "levels_pulled_mean": float(np.mean(pulled))}
+    log(f"bfs: {hm / 1e9:.2f} GTEPS (harmonic mean), {1e3 * np.mean(times):.2f} ms/source")
+
+    # PageRank: global, damping 0.85, tol 1e-10 (stop at L1 change < n tol),
+    # at most 100 iterations, f64, over the unweighted structure.
+    W.pagerank(ctx, gf, tol=1e-10, max_iter=2, weighted=False)  # warm
+    t1 = time.perf_counter()
+    ranks, iters = W.pagerank(ctx, gf, tol=1e-10, max_iter=100, weighted=False)
+    el = time.perf_counter() - t1
+    pr_bytes = (n + 1) * 8 + m * 4 + n * 40
+    legs["pagerank"] = {"iterations": iters, "ms_total": round(el * 1e3, 3),
+                        "ms_per_iteration": round(el * 1e3 / max(iters, 1), 4),
+                        "gbs_per_iteration": round(pr_bytes * iters / el / 1e9, 1)}
+    log(f"pagerank: {iters} iterations, {el * 1e3:.1f} ms")
+    pr_gpu = ranks.numpy()
 
     head = legs["plus_times_f32"]
     result = {
@@ -191,6 +268,9 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(off, tgt, wf, xf)
+        result["errors_vs_oracle"] = oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs,
+                                                   bfs_depths, pr_gpu, iters)
+        log(f"errors vs oracle: {result['errors_vs_oracle']}")
     print(json.dumps(result), flush=True)
 
 

@@ -134,6 +134,17 @@ int wg_spmv(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_ve
 int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
             wg_vector* y);
 
+/* ---- SpMV autotuner: no counterpart in the reference (whose launch shape is
+ * fixed at 4096-row tasks, compute/product/kernel.rs:15). By default (-1) the
+ * first product over a graph orientation of at least 2^22 entries, for each
+ * dtype and semiring, times the launch variants and keeps the fastest:
+ * bit 0 = the LDS-free direct kernel (only where the monoid's result does not
+ * depend on grouping), bit 1 = non-temporal row streams, bit 2 = XCD-contiguous
+ * block order. wg_set_spmv_variant forces one variant (0..7) for the context;
+ * wg_spmv_variant reports the choice made (-1: not tuned yet). */
+void wg_set_spmv_variant(wg_context* ctx, int variant);
+int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted);
+
 /* ---- PageRank: PageRank / pagerank (compute/pagerank/mod.rs:27-151) ----- */
 typedef struct {
   double damping;     /* probability of following an edge (0.85) */
@@ -153,6 +164,53 @@ typedef struct {
 int wg_pagerank(wg_context* ctx, const wg_graph* g, const wg_vector* tombstones,
                 const wg_pagerank_opts* opts, const wg_vector* seed, wg_vector* ranks,
                 uint32_t* iterations);
+
+/* ---- traversal: Visit / TraversalStats (traversal/bfs.rs:15-23,
+ * traversal/stats.rs:12-20) ------------------------------------------------ */
+typedef struct {
+  uint64_t node;
+  uint64_t via;    /* UINT64_MAX for a start (Visit::via None) */
+  uint32_t depth;
+  uint32_t pad;
+} wg_visit;
+
+typedef struct {
+  uint64_t nodes_visited;
+  uint64_t edges_traversed; /* row entries scanned; pushed levels scan every
+                               frontier row, pulled levels stop at the first
+                               frontier hit, as the reference counts them */
+  uint32_t truncated;
+  uint32_t levels_pulled;
+} wg_traversal_stats;
+
+/* The device-resident search: depth and via (u32 vectors over the graph,
+ * 0xffffffff where unreached) of every node reached from starts along
+ * direction (WG_OUT: the forward CSR, WG_IN: the reverse), levels pushed or,
+ * when pulling != 0 and Beamer's switch (alpha 1, beta 24, traversal/
+ * level.rs:19-29,184-211) says so, pulled. via is the smallest frontier
+ * predecessor, as the reference reports it in both directions. filter: n host
+ * bytes (non-zero admits) or NULL; max_depth < 0: unbounded. Starts must lie
+ * in the id space (NodeOutsideIdSpace) and repeats count once. */
+int wg_bfs_levels(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
+                  uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int pulling,
+                  wg_vector* depth, wg_vector* via, wg_traversal_stats* stats);
+
+/* bfs (traversal/bfs.rs:34-45): every node reachable from starts, once, in
+ * BFS order -- by depth; the starts in the order given; within a depth by
+ * (via, node), the order the reference's push reaches them. filter prunes
+ * admission (starts are always admitted); max_results < 0: unbounded, else
+ * the output stops there and stats->truncated says whether a visit was cut.
+ * out must hold n visits. */
+int wg_bfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
+           uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
+           wg_visit* out, uint64_t* count, wg_traversal_stats* stats);
+
+/* reachable_set (traversal/reachable.rs:48-90): the direction-optimising
+ * search's nodes by depth, pushed levels in the order reached, pulled
+ * levels ascending by node. out must hold n ids. */
+int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
+                     uint64_t nstarts, int64_t max_depth, uint64_t* out, uint64_t* count,
+                     wg_traversal_stats* stats);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference: a streaming copy kernel (16-byte loads and

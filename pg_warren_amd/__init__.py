@@ -266,6 +266,81 @@ def pagerank(ctx, graph, *, damping=0.85, tol=1e-6, max_iter=100, seed=None, tom
     return out, it.value
 
 
+DIRECTIONS = {"out": 0, "in": 1}
+
+
+def _stats(st):
+    return {"nodes_visited": st.nodes_visited, "edges_traversed": st.edges_traversed,
+            "truncated": bool(st.truncated), "levels_pulled": st.levels_pulled}
+
+
+def _starts(starts):
+    return np.ascontiguousarray(np.atleast_1d(np.asarray(starts, dtype=np.uint64)))
+
+
+def _filter(node_filter):
+    return None if node_filter is None else np.ascontiguousarray(node_filter, dtype=np.uint8)
+
+
+def bfs(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
+        max_results=None):
+    """bfs (traversal/bfs.rs:34-45): [(node, depth, via or None)] in BFS order
+    and the traversal stats."""
+    st = _starts(starts)
+    flt = _filter(node_filter)
+    out = (_lib.Visit * max(graph.n, 1))()
+    cnt = C.c_uint64()
+    stats = _lib.TraversalStats()
+    ctx.check(ctx.lib.wg_bfs(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st), _p(flt),
+                             -1 if max_depth is None else max_depth,
+                             -1 if max_results is None else max_results, C.addressof(out),
+                             C.byref(cnt), C.byref(stats)))
+    arr = np.ctypeslib.as_array(out)[:cnt.value]
+    visits = [(int(v["node"]), int(v["depth"]), None if v["via"] == 2**64 - 1 else int(v["via"]))
+              for v in arr]
+    return visits, _stats(stats)
+
+
+def reachable_set(ctx, graph, starts, *, direction="out", max_depth=None):
+    """reachable_set (traversal/reachable.rs:48-90): node ids in BFS order."""
+    st = _starts(starts)
+    out = np.zeros(max(graph.n, 1), np.uint64)
+    cnt = C.c_uint64()
+    stats = _lib.TraversalStats()
+    ctx.check(ctx.lib.wg_reachable_set(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+                                       -1 if max_depth is None else max_depth, _p(out),
+                                       C.byref(cnt), C.byref(stats)))
+    return out[:cnt.value].copy(), _stats(stats)
+
+
+def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
+               pulling=True, depth=None, via=None):
+    """The device-resident search: (depth, via) u32 Vectors (0xffffffff where
+    unreached) and the traversal stats."""
+    st = _starts(starts)
+    flt = _filter(node_filter)
+    if depth is None:
+        depth = Vector(ctx, dtype="u32", id_space=graph.n)
+    if via is None:
+        via = Vector(ctx, dtype="u32", id_space=graph.n)
+    stats = _lib.TraversalStats()
+    ctx.check(ctx.lib.wg_bfs_levels(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+                                    _p(flt), -1 if max_depth is None else max_depth,
+                                    int(pulling), depth.h, via.h, C.byref(stats)))
+    return depth, via, _stats(stats)
+
+
+def set_spmv_variant(ctx, variant):
+    """Forces an SpMV launch variant (0..7), or -1 to autotune."""
+    ctx.lib.wg_set_spmv_variant(ctx.h, variant)
+
+
+def spmv_variant(graph, semiring, dtype, *, transpose=False, weighted=True):
+    """The autotuner's choice for this graph orientation, or -1."""
+    return graph.ctx.lib.wg_spmv_variant(graph.h, int(transpose), DTYPES[dtype],
+                                         semiring_index(semiring), int(weighted))
+
+
 def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetrize=False, threads=0):
     """A synthetic R-MAT graph's CSR (offsets u64, targets u32): self loops and
     duplicate edges removed, ids not permuted."""

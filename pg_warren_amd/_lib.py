@@ -34,6 +34,18 @@ class PagerankOpts(C.Structure):
                 ("max_iter", C.c_uint32), ("weighted", C.c_int32), ("pad", C.c_int32)]
 
 
+class Visit(C.Structure):
+    """wg_visit: traversal/bfs.rs:15-23."""
+    _fields_ = [("node", C.c_uint64), ("via", C.c_uint64), ("depth", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class TraversalStats(C.Structure):
+    """wg_traversal_stats: traversal/stats.rs:12-20."""
+    _fields_ = [("nodes_visited", C.c_uint64), ("edges_traversed", C.c_uint64),
+                ("truncated", C.c_uint32), ("levels_pulled", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("wg_context_create", I32, [I32, U64, P]),
@@ -57,7 +69,12 @@ SIGNATURES = [
     ("wg_vector_device_ptr", P, [P]),
     ("wg_spmv", I32, [P, P, P, P, P]),
     ("wg_spmm", I32, [P, P, P, P, P]),
+    ("wg_set_spmv_variant", None, [P, I32]),
+    ("wg_spmv_variant", I32, [P, I32, I32, I32, I32]),
     ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
+    ("wg_bfs_levels", I32, [P, P, I32, P, U64, P, I64, I32, P, P, P]),
+    ("wg_bfs", I32, [P, P, I32, P, U64, P, I64, I64, P, P, P]),
+    ("wg_reachable_set", I32, [P, P, I32, P, U64, I64, P, P, P]),
     ("wg_device_copy", I32, [P, P, P, U64]),
     ("wg_rmat_csr", I32, [U32, U32, D, D, D, U64, I32, P, P, U64, P, I32]),
     ("wg_edge_weights", I32, [U64, P, P, U64, I32, D, D, P, I32]),

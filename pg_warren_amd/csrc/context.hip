@@ -223,8 +223,19 @@ static int upload_csr(wg_context* ctx, DevCsr& c, uint64_t n, const uint64_t* of
   if (!blocks.empty())
     WG_HIP(ctx, hipMemcpyAsync(c.blocks, blocks.data(), blocks.size() * sizeof(Block),
                                hipMemcpyHostToDevice, ctx->stream));
-  if ((s = dalloc(ctx, (void**)&c.tickets, std::max<uint32_t>(c.nlong, 1) * 4))) return s;
-  WG_HIP(ctx, hipMemsetAsync(c.tickets, 0, std::max<uint32_t>(c.nlong, 1) * 4, ctx->stream));
+  // The split rows and their partial slots, in block order.
+  std::vector<uint32_t> lrows, lbase;
+  for (const Block& b : blocks)
+    if (b.nrows == 0 && b.piece == 0) {
+      lrows.push_back(b.row0);
+      lbase.push_back(b.pbase);
+    }
+  lbase.push_back(c.npartials);
+  if ((s = dalloc(ctx, (void**)&c.long_rows, std::max<size_t>(lrows.size(), 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.long_pbase, lbase.size() * 4))) return s;
+  if (!lrows.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.long_rows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  WG_HIP(ctx, hipMemcpyAsync(c.long_pbase, lbase.data(), lbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   // The copies read pageable host memory the caller may free on return.
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return WG_OK;
@@ -235,21 +246,22 @@ static void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.targets, c.nnz * 4);
   if (c.weights) dfree(ctx, c.weights, c.nnz * 8);
   dfree(ctx, c.blocks, c.nblocks * sizeof(Block));
-  dfree(ctx, c.tickets, std::max<uint32_t>(c.nlong, 1) * 4);
+  dfree(ctx, c.long_rows, std::max<uint32_t>(c.nlong, 1) * 4);
+  dfree(ctx, c.long_pbase, (c.nlong + 1) * 4);
   for (int d = 0; d < 5; ++d)
     if (c.wview[d]) dfree(ctx, c.wview[d], c.nnz * dtype_size(d));
   if (c.spmm_ready) {
     dfree(ctx, c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8);
-    dfree(ctx, c.spmm_ticket_of_row, c.n * 4);
-    dfree(ctx, c.spmm_pbase, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
-    dfree(ctx, c.spmm_tickets, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
+    dfree(ctx, c.spmm_ticket_of_row, std::max<uint64_t>(c.n, 1) * 4);
+    dfree(ctx, c.spmm_pbase, (c.spmm_nlong + 1) * 4);
+    dfree(ctx, c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
   }
   c = DevCsr();
 }
 
 int prepare_spmm(wg_context* ctx, DevCsr& c) {
   if (c.spmm_ready) return WG_OK;
-  std::vector<uint32_t> extra, ticket_of_row(c.n, ~0u), pbase;
+  std::vector<uint32_t> extra, ticket_of_row(c.n, ~0u), pbase, lrows;
   uint32_t np_total = 0;
   for (uint64_t r = 0; r < c.n; ++r) {
     uint64_t deg = c.h_offsets[r + 1] - c.h_offsets[r];
@@ -257,27 +269,29 @@ int prepare_spmm(wg_context* ctx, DevCsr& c) {
     uint32_t np = (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece);
     ticket_of_row[r] = (uint32_t)pbase.size();
     pbase.push_back(np_total);
+    lrows.push_back((uint32_t)r);
     np_total += np;
     for (uint32_t p = 1; p < np; ++p) {
       extra.push_back((uint32_t)r);
       extra.push_back(p);
     }
   }
+  pbase.push_back(np_total);
   c.spmm_nextra = (uint32_t)(extra.size() / 2);
-  c.spmm_nlong = (uint32_t)pbase.size();
+  c.spmm_nlong = (uint32_t)lrows.size();
   c.spmm_npartials = np_total;
   int s;
   if ((s = dalloc(ctx, (void**)&c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_ticket_of_row, c.n * 4))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_pbase, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_tickets, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_ticket_of_row, std::max<uint64_t>(c.n, 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_pbase, (c.spmm_nlong + 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
   if (!extra.empty())
     WG_HIP(ctx, hipMemcpyAsync(c.spmm_extra, extra.data(), extra.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   if (c.n)
     WG_HIP(ctx, hipMemcpyAsync(c.spmm_ticket_of_row, ticket_of_row.data(), c.n * 4, hipMemcpyHostToDevice, ctx->stream));
-  if (!pbase.empty())
-    WG_HIP(ctx, hipMemcpyAsync(c.spmm_pbase, pbase.data(), pbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  WG_HIP(ctx, hipMemsetAsync(c.spmm_tickets, 0, std::max<uint32_t>(c.spmm_nlong, 1) * 4, ctx->stream));
+  WG_HIP(ctx, hipMemcpyAsync(c.spmm_pbase, pbase.data(), pbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (!lrows.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.spmm_long_rows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   c.spmm_ready = true;
   return WG_OK;
@@ -470,6 +484,7 @@ void wg_graph_destroy(wg_graph* g) {
   (void)hipStreamSynchronize(g->ctx->stream);
   free_csr(g->ctx, g->fwd);
   free_csr(g->ctx, g->rev);
+  for (auto& w : g->workspace) dfree(g->ctx, w.first, w.second);
   delete g;
 }
 

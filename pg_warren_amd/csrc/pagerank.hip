@@ -41,7 +41,6 @@ struct PrArgs {
   double* x;
   double* part;        // 2 * (nblocks + nlong): change, then dangling
   double* piece;       // split rows' gather partials
-  uint32_t* tickets;
   PrState* st;
   unsigned long long* err;
 };
@@ -72,15 +71,17 @@ __device__ __forceinline__ double pr_row(const PrArgs& a, uint64_t row, double y
   return fabs(next - xo);
 }
 
-// The last block of a launch: fixed-order sums of the partials, the stop
-// decision, and the arrival counter reset.
-__device__ void pr_finish(const PrArgs& a, int parity) {
+// One workgroup after each iteration: fixed-order sums of the partials
+// (the kernel boundary makes every block's stores visible, with no
+// cross-workgroup fence), and the stop decision.
+__global__ __launch_bounds__(256) void pr_finish(PrArgs a, int parity) {
   __shared__ double sc[256], sd[256];
+  if (pr_halted(a)) return;
   const uint32_t tid = threadIdx.x, total = a.nblocks + a.nlong;
   double c = 0.0, d = 0.0;
   for (uint32_t i = tid; i < total; i += 256) {
-    c += __hip_atomic_load(&a.part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    d += __hip_atomic_load(&a.part[total + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c += a.part[i];
+    d += a.part[total + i];
   }
   sc[tid] = c;
   sd[tid] = d;
@@ -97,8 +98,25 @@ __device__ void pr_finish(const PrArgs& a, int parity) {
     a.st->dangling[parity ^ 1] = sd[0];
     a.st->iterations += 1;
     if (a.has_tol && sql_lt(sc[0], a.bound)) a.st->stop = 1;
-    a.st->arrivals = 0;
   }
+}
+
+// The split rows of an iteration: each folds its pieces' gather partials in
+// order, then advances; its L1 change and dangling rank take a slot of
+// their own, so the fixed-order total does not depend on scheduling.
+__global__ __launch_bounds__(256) void pr_long(PrArgs a, const uint32_t* __restrict__ long_rows,
+                                               const uint32_t* __restrict__ long_pbase,
+                                               double* __restrict__ msg_out, int parity) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.nlong || pr_halted(a)) return;
+  uint32_t e_err = 0;
+  double y = a.piece[long_pbase[i]];
+  for (uint32_t p = long_pbase[i] + 1; p < long_pbase[i + 1]; ++p) y += a.piece[p];
+  double dl = 0.0;
+  const double ch = pr_row(a, long_rows[i], y, a.st->dangling[parity], msg_out, dl, e_err);
+  a.part[a.nblocks + i] = ch;
+  a.part[2 * a.nblocks + a.nlong + i] = dl;
+  if (e_err) raise_error(a.err, long_rows[i], e_err);
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -119,7 +137,6 @@ __global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const doub
   __shared__ double prod[kBlockNnz];
   __shared__ uint32_t rs[kBlockRows + 1];
   __shared__ double wsum[8];
-  __shared__ int last;
   if (pr_halted(a)) return;
   const double dangling = a.st->dangling[parity];
   const Block b = a.blocks[blockIdx.x];
@@ -190,36 +207,14 @@ __global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const doub
     double acc = 0.0;
     for (uint32_t i = s; i < t; ++i) acc += prod[i];
     acc = block_sum(acc, wsum);
+    // A piece of a split row: its gather partial waits for pr_long.
     if (tid == 0) {
       a.part[blockIdx.x] = 0.0;
       a.part[a.nblocks + a.nlong + blockIdx.x] = 0.0;
       a.piece[b.pbase + b.piece] = acc;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      uint32_t prev = atomicAdd(&a.tickets[b.ticket], 1u);
-      if (prev == b.npieces - 1) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        double y = __hip_atomic_load(&a.piece[b.pbase], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (uint32_t p = 1; p < b.npieces; ++p)
-          y += __hip_atomic_load(&a.piece[b.pbase + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a.tickets[b.ticket] = 0;
-        double dl = 0.0;
-        double ch = pr_row(a, b.row0, y, dangling, msg_out, dl, e_err);
-        // A split row's sums have a slot of their own, so the fixed-order
-        // total does not depend on which piece finished last.
-        a.part[a.nblocks + b.ticket] = ch;
-        a.part[2 * a.nblocks + a.nlong + b.ticket] = dl;
-      }
     }
   }
   if (e_err) raise_error(a.err, b.row0, e_err);
-  if (tid == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    uint32_t prev = atomicAdd(&a.st->arrivals, 1u);
-    last = prev == gridDim.x - 1;
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-  if (last) pr_finish(a, parity);
 }
 
 // msg = x / out (or 0, the rank joining the dangling sum) for the first
@@ -424,8 +419,8 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
   const double* wv = nullptr;
   if (weighted && rev.weights) wv = rev.weights;
   PrArgs a{rev.blocks, rev.offsets, rev.targets, wv, out, p, start, tomb, o->damping, teleport,
-           bound, o->has_tol && !bound_err, rev.nblocks, rev.nlong, x, part, piece, rev.tickets,
-           state, ctx->d_err};
+           bound, o->has_tol && !bound_err, rev.nblocks, rev.nlong, x, part, piece, state,
+           ctx->d_err};
   // With a tolerance whose bound fails, the reference raises that error
   // after the first iteration (mod.rs:144-147).
   const uint32_t max_iter = bound_err ? 1 : o->max_iter;
@@ -444,6 +439,9 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
         pr_iterate<true><<<rev.nblocks, kBlockThreads, 0, st>>>(a, mi, mo, parity);
       else
         pr_iterate<false><<<rev.nblocks, kBlockThreads, 0, st>>>(a, mi, mo, parity);
+      if (rev.nlong)
+        pr_long<<<(rev.nlong + 255) / 256, 256, 0, st>>>(a, rev.long_rows, rev.long_pbase, mo, parity);
+      pr_finish<<<1, 256, 0, st>>>(a, parity);
     }
     WG_HIP(ctx, hipGetLastError());
     WG_HIP(ctx, hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, st));

@@ -46,12 +46,40 @@ __device__ __forceinline__ T load_coherent(const T* p) {
 template <int MUL>
 constexpr bool reads_x() { return MUL != FIRST && MUL != PAIR; }
 
+// Graphs smaller than this use the default launch without tuning.
+constexpr uint64_t kTuneMinNnz = 1ull << 22;
+
+// Launch-shape choices the autotuner makes per graph orientation and
+// semiring (bit flags).
+enum : int {
+  kVarDirect = 1,  // spmv_direct instead of the LDS-staged spmv_blocks
+  kVarNT = 2,      // non-temporal loads of the target / weight streams
+  kVarXcd = 4,     // XCD-contiguous block order
+};
+
+// A streamed load, non-temporal when nt (the row streams are read once and
+// should not evict the gathered x from L2 and the infinity cache).
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+// Workgroups are dispatched to the eight XCDs in turn; with xcd set, XCD k
+// takes the k-th contiguous run of blocks (a bijection for any count), so
+// neighbouring rows share one L2.
+__device__ __forceinline__ uint32_t block_index(bool xcd) {
+  if (!xcd) return blockIdx.x;
+  const uint32_t nb = gridDim.x, q = nb / 8, r = nb % 8;
+  const uint32_t k = blockIdx.x % 8, j = blockIdx.x / 8;
+  return k * q + (k < r ? k : r) + j;
+}
+
 template <class E, int ADD, int MUL, bool HW>
 __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
     const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
     E* __restrict__ y, WriteArgs wa, typename Fold<E, ADD>::A* __restrict__ partials,
-    uint32_t* __restrict__ tickets, unsigned long long* __restrict__ err) {
+    unsigned long long* __restrict__ err, int variant) {
   using F = Fold<E, ADD>;
   using A = typename F::A;
   __shared__ E prod[kBlockNnz];
@@ -59,7 +87,8 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
   __shared__ A wpart[kBlockThreads / 64];
   __shared__ int wpres[kBlockThreads / 64];
 
-  const Block b = blocks[blockIdx.x];
+  const bool nt = variant & kVarNT;
+  const Block b = blocks[block_index(variant & kVarXcd)];
   const uint32_t tid = threadIdx.x;
   uint32_t e_err = 0;
   uint64_t e0, e1;
@@ -81,8 +110,8 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
     for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
       const uint32_t i = tid + k * kBlockThreads;
       if (i < cnt) {
-        if constexpr (reads_x<MUL>()) col[k] = tgt[e0 + i];
-        if constexpr (HW) wv[k] = w[e0 + i];
+        if constexpr (reads_x<MUL>()) col[k] = ld_stream(tgt + e0 + i, nt);
+        if constexpr (HW) wv[k] = ld_stream(w + e0 + i, nt);
       }
     }
 #pragma unroll
@@ -169,24 +198,148 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
           tot = tp ? F::combine(tot, wpart[k], e_err) : wpart[k];
           tp = 1;
         }
-      const uint64_t row = b.row0;
-      if (b.npieces == 1) {
-        E res = tp ? F::finish(tot, e_err) : E();
-        write_elem<E, ADD>(wa, row, &y[row], tp != 0, res, e_err);
-      } else {
-        partials[b.pbase + b.piece] = tot;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        uint32_t prev = atomicAdd(&tickets[b.ticket], 1u);
-        if (prev == b.npieces - 1) {
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          A all = load_coherent(&partials[b.pbase]);
-          for (uint32_t p = 1; p < b.npieces; ++p)
-            all = F::combine(all, load_coherent(&partials[b.pbase + p]), e_err);
-          tickets[b.ticket] = 0;
-          E res = F::finish(all, e_err);
-          write_elem<E, ADD>(wa, row, &y[row], true, res, e_err);
+      // A split row's pieces each leave a partial; spmv_long folds them in
+      // piece order once every piece has run (the kernel boundary orders
+      // them: no cross-workgroup fence, which on gfx950 would write back
+      // the XCD's L2).
+      partials[b.pbase + b.piece] = tot;
+    }
+  }
+  if (e_err) raise_error(err, b.row0, e_err);
+}
+
+// Folds each split row's piece partials in piece order and writes the row.
+template <class E, int ADD>
+__global__ __launch_bounds__(256) void spmv_long(const uint32_t* __restrict__ long_rows,
+                                                 const uint32_t* __restrict__ long_pbase,
+                                                 uint32_t nlong,
+                                                 const typename Fold<E, ADD>::A* __restrict__ partials,
+                                                 E* __restrict__ y, WriteArgs wa,
+                                                 unsigned long long* __restrict__ err) {
+  using F = Fold<E, ADD>;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nlong) return;
+  uint32_t e_err = 0;
+  const uint32_t p0 = long_pbase[i], p1 = long_pbase[i + 1];
+  typename F::A all = partials[p0];
+  for (uint32_t p = p0 + 1; p < p1; ++p) all = F::combine(all, partials[p], e_err);
+  const uint64_t row = long_rows[i];
+  E res = F::finish(all, e_err);
+  write_elem<E, ADD>(wa, row, &y[row], true, res, e_err);
+  if (e_err) raise_error(err, row, e_err);
+}
+
+// The LDS-free variant for monoids whose result does not depend on the
+// grouping (all but i64 plus and float min/max): each row's group of T lanes
+// reads the row's entries directly, lane l taking entries l, l + T, ...
+// (coalesced across the wave, since neighbouring groups hold neighbouring
+// rows), four loads in flight per lane; the group then combines its lanes in
+// lane order, so `any` still keeps the row's first entry. Long-row pieces
+// stride the whole workgroup over the piece the same way.
+template <class E, int ADD>
+constexpr bool grouping_free() {
+  return !(std::is_same<E, int64_t>::value && ADD == PLUS) && !(is_float_v<E> && (ADD == MIN || ADD == MAX));
+}
+
+template <class E, int ADD, int MUL, bool HW>
+__global__ __launch_bounds__(kBlockThreads) void spmv_direct(
+    const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
+    E* __restrict__ y, WriteArgs wa, typename Fold<E, ADD>::A* __restrict__ partials,
+    unsigned long long* __restrict__ err, int variant) {
+  using F = Fold<E, ADD>;
+  using A = typename F::A;
+  __shared__ A wpart[kBlockThreads / 64];
+  __shared__ int wpres[kBlockThreads / 64];
+  const bool nt = variant & kVarNT;
+  const Block b = blocks[block_index(variant & kVarXcd)];
+  const uint32_t tid = threadIdx.x;
+  uint32_t e_err = 0;
+  // Folds entries i, i + step, ... below end into (acc, pres), four at a time.
+  auto fold_strided = [&](uint64_t i, uint64_t end, uint32_t step, A& acc, int& pres) {
+    for (; i < end; i += 4 * (uint64_t)step) {
+      uint32_t c[4];
+      E wk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint64_t j = i + (uint64_t)k * step;
+        if (j < end) {
+          if constexpr (reads_x<MUL>()) c[k] = ld_stream(tgt + j, nt);
+          if constexpr (HW) wk[k] = ld_stream(w + j, nt);
         }
       }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint64_t j = i + (uint64_t)k * step;
+        if (j < end) {
+          E wv = E(1);
+          if constexpr (HW) wv = wk[k];
+          E xv = E();
+          if constexpr (reads_x<MUL>()) xv = x[c[k]];
+          A v = F::lift(multiply<E, ADD, MUL>(wv, xv, e_err));
+          acc = pres ? F::combine(acc, v, e_err) : v;
+          pres = 1;
+        }
+      }
+    }
+  };
+  if (b.nrows) {
+    const uint32_t nr = b.nrows;
+    uint32_t T = 1;
+    if (nr < kBlockThreads) {
+      uint32_t q = kBlockThreads / nr;
+      T = 1u << (31 - __builtin_clz(q));
+      if (T > 64) T = 64;
+    }
+    const uint32_t groups = kBlockThreads / T, g = tid / T, l = tid % T;
+    for (uint32_t r = g; r < nr; r += groups) {
+      const uint64_t row = (uint64_t)b.row0 + r;
+      A acc{};
+      int pres = 0;
+      fold_strided(off[row] + l, off[row + 1], T, acc, pres);
+      for (uint32_t d = 1; d < T; d <<= 1) {
+        A o = shfl_down(acc, d, T);
+        int op = __shfl_down(pres, d, T);
+        if ((l & (2 * d - 1)) == 0 && op) {
+          acc = pres ? F::combine(acc, o, e_err) : o;
+          pres = 1;
+        }
+      }
+      if (l == 0) {
+        E res = pres ? F::finish(acc, e_err) : E();
+        write_elem<E, ADD>(wa, row, &y[row], pres != 0, res, e_err);
+      }
+    }
+  } else {
+    const uint64_t rb = off[b.row0], re = off[b.row0 + 1];
+    const uint64_t e0 = rb + (uint64_t)b.piece * kBlockNnz;
+    const uint64_t e1 = e0 + kBlockNnz < re ? e0 + kBlockNnz : re;
+    A acc{};
+    int pres = 0;
+    fold_strided(e0 + tid, e1, kBlockThreads, acc, pres);
+    const uint32_t l = tid & 63;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      A o = shfl_down(acc, d, 64);
+      int op = __shfl_down(pres, d, 64);
+      if ((l & (2 * d - 1)) == 0 && op) {
+        acc = pres ? F::combine(acc, o, e_err) : o;
+        pres = 1;
+      }
+    }
+    if (l == 0) {
+      wpart[tid >> 6] = acc;
+      wpres[tid >> 6] = pres;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      A tot{};
+      int tp = 0;
+      for (int k = 0; k < (int)(kBlockThreads / 64); ++k)
+        if (wpres[k]) {
+          tot = tp ? F::combine(tot, wpart[k], e_err) : wpart[k];
+          tp = 1;
+        }
+      partials[b.pbase + b.piece] = tot;
     }
   }
   if (e_err) raise_error(err, b.row0, e_err);
@@ -199,9 +352,8 @@ __global__ __launch_bounds__(256) void spmm_rows(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ tgt, const E* __restrict__ w,
     const E* __restrict__ x, E* __restrict__ y, uint64_t width, uint64_t n, uint32_t lpr,
     const uint32_t* __restrict__ extra, uint32_t nextra, const uint32_t* __restrict__ ticket_of_row,
-    const uint32_t* __restrict__ pbase, uint32_t* __restrict__ tickets,
-    typename Fold<E, ADD>::A* __restrict__ partials, WriteArgs wa,
-    unsigned long long* __restrict__ err) {
+    const uint32_t* __restrict__ pbase, typename Fold<E, ADD>::A* __restrict__ partials,
+    WriteArgs wa, unsigned long long* __restrict__ err) {
   using F = Fold<E, ADD>;
   using A = typename F::A;
   const uint32_t tasks_per_block = 256 / lpr;
@@ -243,27 +395,29 @@ __global__ __launch_bounds__(256) void spmm_rows(
       partials[(uint64_t)(pbase[tk] + piece) * width + c] = acc;
     }
   }
-  if (np > 1) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    // Every lane of the task has stored its columns; lane 0 takes the ticket
-    // once they all have (the group's lanes are in one wave).
-    __builtin_amdgcn_wave_barrier();
-    uint32_t prev = 0;
-    if (lane == 0) prev = atomicAdd(&tickets[tk], 1u);
-    prev = __shfl(prev, (int)(threadIdx.x & 63) - (int)lane, 64);
-    if (prev == np - 1) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      const uint64_t base = pbase[tk];
-      for (uint64_t c = lane; c < width; c += lpr) {
-        A all = load_coherent(&partials[base * width + c]);
-        for (uint32_t p = 1; p < np; ++p)
-          all = F::combine(all, load_coherent(&partials[(base + p) * width + c]), e_err);
-        E res = F::finish(all, e_err);
-        write_elem<E, ADD>(wa, row, &y[row * width + c], true, res, e_err);
-      }
-      if (lane == 0) tickets[tk] = 0;
-    }
-  }
+  if (e_err) raise_error(err, row, e_err);
+}
+
+// Folds each split row's piece partials in piece order, column by column.
+template <class E, int ADD>
+__global__ __launch_bounds__(256) void spmm_long(const uint32_t* __restrict__ long_rows,
+                                                 const uint32_t* __restrict__ pbase, uint32_t nlong,
+                                                 uint64_t width,
+                                                 const typename Fold<E, ADD>::A* __restrict__ partials,
+                                                 E* __restrict__ y, WriteArgs wa,
+                                                 unsigned long long* __restrict__ err) {
+  using F = Fold<E, ADD>;
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (uint64_t)nlong * width) return;
+  const uint32_t t = (uint32_t)(i / width);
+  const uint64_t c = i % width;
+  uint32_t e_err = 0;
+  const uint32_t p0 = pbase[t], p1 = pbase[t + 1];
+  typename F::A all = partials[(uint64_t)p0 * width + c];
+  for (uint32_t p = p0 + 1; p < p1; ++p) all = F::combine(all, partials[(uint64_t)p * width + c], e_err);
+  const uint64_t row = long_rows[t];
+  E res = F::finish(all, e_err);
+  write_elem<E, ADD>(wa, row, &y[row * width + c], true, res, e_err);
   if (e_err) raise_error(err, row, e_err);
 }
 
@@ -285,12 +439,63 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       if ((s = ensure_scratch(ctx, (size_t)std::max<uint32_t>(csr.npartials, 1) * sizeof(A)))) return s;
       if (csr.nblocks == 0) return WG_OK;
       A* part = (A*)ctx->scratch;
-      if (wv)
-        spmv_blocks<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-            csr.blocks, csr.offsets, csr.targets, wv, x, y, wa, part, csr.tickets, ctx->d_err);
-      else
-        spmv_blocks<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-            csr.blocks, csr.offsets, csr.targets, nullptr, x, y, wa, part, csr.tickets, ctx->d_err);
+      constexpr bool direct_ok = grouping_free<E, ADD>();
+      auto launch = [&](int variant, E* out) {
+        if (direct_ok && (variant & kVarDirect)) {
+          if (wv)
+            spmv_direct<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.offsets, csr.targets, wv, x, out, wa, part, ctx->d_err, variant);
+          else
+            spmv_direct<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.offsets, csr.targets, nullptr, x, out, wa, part, ctx->d_err, variant);
+        } else {
+          if (wv)
+            spmv_blocks<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.offsets, csr.targets, wv, x, out, wa, part, ctx->d_err, variant);
+          else
+            spmv_blocks<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.offsets, csr.targets, nullptr, x, out, wa, part, ctx->d_err, variant);
+        }
+      };
+      int variant = 0;
+      const int key = ((DtypeOf<E>::v * 16 + ADD) * 16 + MUL) * 2 + (wv ? 1 : 0);
+      if (ctx->spmv_variant >= 0) {
+        variant = ctx->spmv_variant;
+      } else if (csr.tuned.count(key)) {
+        variant = csr.tuned[key];
+      } else if (csr.nnz >= kTuneMinNnz) {
+        // Autotune on this graph: time every eligible variant into a scratch
+        // y (the caller's y may be accumulated into), keep the fastest.
+        void* ys = nullptr;
+        if ((s = dalloc(ctx, &ys, csr.n * sizeof(E)))) return s;
+        hipEvent_t e0, e1;
+        WG_HIP(ctx, hipEventCreate(&e0));
+        WG_HIP(ctx, hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int v = 0; v < 8; ++v) {
+          if ((v & kVarDirect) && !direct_ok) continue;
+          launch(v, (E*)ys);
+          WG_HIP(ctx, hipEventRecord(e0, ctx->stream));
+          for (int r = 0; r < 3; ++r) launch(v, (E*)ys);
+          WG_HIP(ctx, hipEventRecord(e1, ctx->stream));
+          WG_HIP(ctx, hipEventSynchronize(e1));
+          float ms = 0;
+          WG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            variant = v;
+          }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        dfree(ctx, ys, csr.n * sizeof(E));
+        csr.tuned[key] = variant;
+        if ((s = clear_error(ctx))) return s;
+      }
+      launch(variant, y);
+      if (csr.nlong)
+        spmv_long<E, ADD><<<(csr.nlong + 255) / 256, 256, 0, ctx->stream>>>(
+            csr.long_rows, csr.long_pbase, csr.nlong, part, y, wa, ctx->d_err);
     } else {
       if ((s = prepare_spmm(ctx, csr))) return s;
       if ((s = ensure_scratch(ctx, (size_t)std::max<uint32_t>(csr.spmm_npartials, 1) * width * sizeof(A))))
@@ -304,12 +509,16 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       if (wv)
         spmm_rows<E, ADD, MUL, true><<<(unsigned)grid, 256, 0, ctx->stream>>>(
             csr.offsets, csr.targets, wv, x, y, width, csr.n, lpr, csr.spmm_extra, csr.spmm_nextra,
-            csr.spmm_ticket_of_row, csr.spmm_pbase, csr.spmm_tickets, part, wa, ctx->d_err);
+            csr.spmm_ticket_of_row, csr.spmm_pbase, part, wa, ctx->d_err);
       else
         spmm_rows<E, ADD, MUL, false><<<(unsigned)grid, 256, 0, ctx->stream>>>(
             csr.offsets, csr.targets, nullptr, x, y, width, csr.n, lpr, csr.spmm_extra,
-            csr.spmm_nextra, csr.spmm_ticket_of_row, csr.spmm_pbase, csr.spmm_tickets, part, wa,
-            ctx->d_err);
+            csr.spmm_nextra, csr.spmm_ticket_of_row, csr.spmm_pbase, part, wa, ctx->d_err);
+      if (csr.spmm_nlong) {
+        const uint64_t items = (uint64_t)csr.spmm_nlong * width;
+        spmm_long<E, ADD><<<(unsigned)((items + 255) / 256), 256, 0, ctx->stream>>>(
+            csr.spmm_long_rows, csr.spmm_pbase, csr.spmm_nlong, width, part, y, wa, ctx->d_err);
+      }
     }
     WG_HIP(ctx, hipGetLastError());
     return WG_OK;
@@ -394,6 +603,25 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
 }  // namespace wg
 
 extern "C" {
+
+void wg_set_spmv_variant(wg_context* ctx, int variant) {
+  if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 7);
+}
+
+int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted) {
+  static const int add_of[13] = {wg::PLUS, wg::MIN, wg::MAX, wg::MAX, wg::MIN, wg::LOR, wg::ANY,
+                                 wg::PLUS, wg::PLUS, wg::PLUS, wg::MIN, wg::MIN, wg::MAX};
+  static const int mul_of[13] = {wg::TIMES, wg::PLUS, wg::PLUS, wg::TIMES, wg::TIMES, wg::LAND,
+                                 wg::PAIR, wg::PAIR, wg::FIRST, wg::SECOND, wg::FIRST, wg::SECOND,
+                                 wg::SECOND};
+  if (!g || semiring < 0 || semiring > 12) return -1;
+  const wg::DevCsr& c = transpose ? g->rev : g->fwd;
+  const int mul = mul_of[semiring];
+  const bool hw = weighted && c.weights && mul != wg::SECOND && mul != wg::PAIR;
+  const int key = ((dtype * 16 + add_of[semiring]) * 16 + mul) * 2 + (hw ? 1 : 0);
+  auto it = c.tuned.find(key);
+  return it == c.tuned.end() ? -1 : it->second;
+}
 
 int wg_spmv(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
             wg_vector* y) {

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/warren_gpu.h"
@@ -45,18 +46,21 @@ struct DevCsr {
   uint32_t nblocks = 0;
   uint32_t nlong = 0;        // rows split into pieces
   uint32_t npartials = 0;    // pieces of split rows
-  uint32_t* tickets = nullptr;  // nlong counters, zero between launches
+  uint32_t* long_rows = nullptr;   // nlong: the row of each split row
+  uint32_t* long_pbase = nullptr;  // nlong + 1: its partial slots
   uint64_t max_degree = 0;
   void* wview[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int wstatus[5] = {-1, -1, -1, -1, -1};  // -1: not cast yet
   // SpMM: long rows split into pieces of at most kSpmmPiece entries.
   uint32_t* spmm_extra = nullptr;  // (row, piece) pairs beyond piece 0
   uint32_t spmm_nextra = 0;
-  uint32_t* spmm_ticket_of_row = nullptr;  // per row: ticket index or ~0
-  uint32_t* spmm_pbase = nullptr;          // per ticket: first partial slot
+  uint32_t* spmm_ticket_of_row = nullptr;  // per row: split-row index or ~0
+  uint32_t* spmm_pbase = nullptr;          // nlong + 1: partial slots per split row
+  uint32_t* spmm_long_rows = nullptr;      // nlong: the row of each split row
   uint32_t spmm_nlong = 0, spmm_npartials = 0;
-  uint32_t* spmm_tickets = nullptr;
   bool spmm_ready = false;
+  // Autotuned SpMV launch variant per (dtype, monoid, multiply, weighted).
+  std::map<int, int> tuned;
 };
 
 constexpr uint32_t kSpmmPiece = 2048;
@@ -76,6 +80,7 @@ struct wg_context {
   unsigned long long* h_err = nullptr;  // pinned mirror
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
+  int spmv_variant = -1;  // -1: autotune; else the launch variant to force
 };
 
 struct wg_vector {
@@ -91,6 +96,10 @@ struct wg_graph {
   uint64_t n = 0, nnz = 0;
   bool weighted = false;
   wg::DevCsr fwd, rev;
+  // Search workspace sized by the graph, allocated on first use and kept
+  // (charged to the context) until the graph is destroyed.
+  std::vector<std::pair<void*, size_t>> workspace;
+  void* bfs_ws[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace wg {

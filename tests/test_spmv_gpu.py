@@ -174,10 +174,13 @@ ADD_OF = {"plus_times": "plus", "min_plus": "min", "max_plus": "max", "max_times
           "max_second": "max"}
 
 
+@pytest.mark.parametrize("variant", range(8))
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool", "u32"])
-def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype):
+def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype, variant):
     """Integer-valued inputs: every partial sum is exact, so even float plus
-    must agree to the bit; NaN and -0 exercise the SQL min/max order."""
+    must agree to the bit; NaN and -0 exercise the SQL min/max order. Every
+    launch variant the autotuner may pick is checked."""
+    W.set_spmv_variant(ctx, variant)
     g = hub_graph(oracle, 1)
     rng = np.random.default_rng(2)
     for sr in W.SEMIRINGS:
@@ -197,10 +200,13 @@ def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype):
                     continue
                 got = run_gpu(ctx, g, x, sr, dtype, transpose=transpose, weighted=weighted)
                 assert got.tobytes() == want.tobytes(), (sr, dtype, transpose, weighted)
+    W.set_spmv_variant(ctx, -1)
 
 
+@pytest.mark.parametrize("variant", [0, 3, 7])
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool"])
-def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype):
+def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype, variant):
+    W.set_spmv_variant(ctx, variant)
     g = hub_graph(oracle, 3)
     rng = np.random.default_rng(4)
     for sr in ["plus_times", "min_plus", "lor_land", "max_second"]:
@@ -221,9 +227,12 @@ def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype):
                         continue
                     got = run_gpu(ctx, g, x, sr, dtype, **kw)
                     assert got.tobytes() == want.tobytes(), (sr, dtype, complement, replace, acc)
+    W.set_spmv_variant(ctx, -1)
 
 
-def test_float_plus_within_rounding_bound(ctx, oracle):
+@pytest.mark.parametrize("variant", [0, 1, 6, 7])
+def test_float_plus_within_rounding_bound(ctx, oracle, variant):
+    W.set_spmv_variant(ctx, variant)
     g = hub_graph(oracle, 5)
     rng = np.random.default_rng(6)
     rev = g.transpose()
@@ -238,6 +247,7 @@ def test_float_plus_within_rounding_bound(ctx, oracle):
             deg = np.diff(m.offsets.astype(np.int64))
             bound = 2 * deg * EPS[dtype] * abssum
             assert np.all(np.abs(got - want) <= bound + 1e-300)
+    W.set_spmv_variant(ctx, -1)
 
 
 @pytest.mark.parametrize("width", [2, 5, 64, 70])
@@ -282,6 +292,24 @@ def test_rmat_scale18_three_semirings(ctx, oracle):
     got = run_gpu(ctx, gf, xb, "lor_land", "bool")
     want = run_oracle(oracle, gf, xb, "lor_land", "bool")
     assert got.tobytes() == want.tobytes()
+
+
+def test_autotuner_choice_is_a_valid_variant(ctx, oracle):
+    """Above 2^22 entries the first product tunes; the result matches the
+    forced default variant bit for bit on integer-valued input."""
+    off, tgt = W.rmat(19, 16, seed=2)
+    n = len(off) - 1
+    w = W.edge_weights(off, tgt, "int", seed=5, p0=1, p1=4)
+    g = W.Graph(ctx, n, off, tgt, w)
+    x = np.random.default_rng(3).integers(0, 4, n).astype(np.float32)
+    xv = W.Vector(ctx, x)
+    tuned = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
+    choice = W.spmv_variant(g, "plus_times", "f32")
+    assert 0 <= choice < 8
+    W.set_spmv_variant(ctx, 0)
+    plain = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
+    W.set_spmv_variant(ctx, -1)
+    assert tuned.tobytes() == plain.tobytes()
 
 
 def test_repeated_launches_are_bit_identical(ctx, oracle):
