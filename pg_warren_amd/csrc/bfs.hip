@@ -2,24 +2,27 @@
 // (traversal/level.rs:88-337, bfs.rs:34-111, reachable.rs:79-90).
 //
 // Each level is pushed or pulled by Beamer's switch with the reference's
-// constants (alpha = 1, beta = 24), decided on the host from three counters
-// the kernels keep on the device: the next frontier's size, its row entries
-// (m_f) and the reverse-row entries of every visited node (so the unvisited
-// nodes' entries m_u follow by subtraction).
-//   push  a wavefront per frontier node, lanes striding its row; an unvisited
-//         target takes atomicMin(via, frontier node) and is claimed once by
-//         an atomicOr on the next-frontier bitmap, then appended to the next
-//         queue with one wave-aggregated atomic.
-//   pull  over the scanned CSR's row blocks (the SpMV partition): a group of
-//         T lanes per unvisited row reads T consecutive entries at a time
-//         (coalesced), and the lowest lane whose entry is in the frontier
-//         bitmap (a group ballot) gives the first frontier hit in ascending
-//         order; rows longer than a block search their pieces in parallel
-//         and bfs_pull_long keeps the first piece that hit. The block's
-//         next-frontier bits are gathered in LDS and written word by word.
-// Either way a node's via is its smallest frontier predecessor, which is
-// what the reference reports in both directions, its depth is exact, and
-// the row entries scanned are counted as the reference counts them.
+// constants (alpha = 1, beta = 24), decided on the host from counters of
+// the frontier: its size, its row entries (m_f) and the reverse-row entries
+// of every visited node (so the unvisited nodes' entries m_u follow by
+// subtraction).
+//   push  bfs_mark: a wavefront per frontier node strides its row and marks
+//         each unvisited target a candidate (a plain byte store: every
+//         writer writes 1, so no atomic is needed); the pull kernel then
+//         resolves only the candidates.
+//   pull  bfs_pull_blocks over the scanned (reverse) CSR's row blocks, the
+//         SpMV partition: a group of T lanes per row reads T consecutive
+//         entries at a time (coalesced), and the lowest lane whose entry is
+//         in the frontier bitmap (a group ballot) is the first frontier hit
+//         in ascending order -- the smallest frontier predecessor, which is
+//         the via the reference reports both ways. Rows longer than a block
+//         search their pieces in parallel and bfs_pull_long keeps the first
+//         piece that hit. A block's next-frontier bits are gathered in LDS
+//         and stored a word at a time.
+// The next frontier's queue and counters come from the claim bitmap by a
+// count / scan / write compaction with per-block partials. The hot path has
+// no device-scope atomics (on gfx950 they resolve past the per-XCD L2 and
+// serialise on a shared address).
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -30,122 +33,82 @@
 namespace wg {
 
 constexpr uint32_t kUnvisited = 0xffffffffu;
+constexpr uint32_t kWordsPerCountBlock = 512;  // bfs_count / bfs_write: 2 words a thread
 
 struct BfsCounters {
   unsigned long long next_size;    // nodes claimed this level
   unsigned long long next_entries; // their rows' entries (m_f of the next level)
   unsigned long long visited_rev;  // reverse-row entries of the nodes claimed
-  unsigned long long edges;        // row entries scanned
+  unsigned long long edges;        // row entries scanned by a pulled level
 };
 
 __device__ __forceinline__ bool bit_of(const uint32_t* bm, uint32_t v) {
   return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
-// Appends the lanes with `take` set to the queue with one atomic per wave.
-__device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* queue,
-                                            unsigned long long* size) {
-  const unsigned long long mask = __ballot(take);
-  if (!mask) return;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t leader = __ffsll((long long)mask) - 1;
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(size, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (take) {
-    const uint32_t rank = __popcll(mask & ((1ull << lane) - 1));
-    queue[base + rank] = v;
-  }
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+template <class T>
+__device__ __forceinline__ T block_reduce_sum(T v, T* sh) {
   for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-  return v;
-}
-
-__device__ __forceinline__ void add_counters(BfsCounters* c, unsigned long long edges,
-                                             unsigned long long entries, unsigned long long vrev) {
-  edges = wave_sum(edges);
-  entries = wave_sum(entries);
-  vrev = wave_sum(vrev);
-  if ((threadIdx.x & 63) == 0) {
-    if (edges) atomicAdd(&c->edges, edges);
-    if (entries) atomicAdd(&c->next_entries, entries);
-    if (vrev) atomicAdd(&c->visited_rev, vrev);
-  }
+  const uint32_t tid = threadIdx.x;
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  T t = 0;
+  if (tid == 0)
+    for (uint32_t k = 0; k < blockDim.x / 64; ++k) t += sh[k];
+  __syncthreads();
+  return t;
 }
 
 __global__ __launch_bounds__(256) void bfs_seed(const uint32_t* __restrict__ starts, uint32_t ns,
                                                 uint32_t* __restrict__ depth,
                                                 uint32_t* __restrict__ visited,
-                                                uint32_t* __restrict__ frontier,
-                                                const uint64_t* __restrict__ adj_off,
-                                                const uint64_t* __restrict__ rev_off,
-                                                BfsCounters* c) {
+                                                uint32_t* __restrict__ frontier) {
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < ns; i += gridDim.x * 256) {
     const uint32_t s = starts[i];
     depth[s] = 0;
     atomicOr(&visited[s >> 5], 1u << (s & 31));
     atomicOr(&frontier[s >> 5], 1u << (s & 31));
-    atomicAdd(&c->next_entries, (unsigned long long)(adj_off[s + 1] - adj_off[s]));
-    atomicAdd(&c->visited_rev, (unsigned long long)(rev_off[s + 1] - rev_off[s]));
   }
 }
 
-__global__ __launch_bounds__(256) void bfs_push(
-    const uint32_t* __restrict__ queue, uint64_t qn, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ tgt, const uint64_t* __restrict__ rev_off,
-    const uint32_t* __restrict__ visited, const uint8_t* __restrict__ filter,
-    uint32_t* __restrict__ depth, uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm,
-    uint32_t* __restrict__ next_queue, uint32_t d1, BfsCounters* c) {
+// Push, first half: candidates of the next level.
+__global__ __launch_bounds__(256) void bfs_mark(const uint32_t* __restrict__ queue, uint64_t qn,
+                                                const uint64_t* __restrict__ off,
+                                                const uint32_t* __restrict__ tgt,
+                                                const uint32_t* __restrict__ visited,
+                                                const uint8_t* __restrict__ filter,
+                                                uint8_t* __restrict__ cand) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
   const uint64_t nwaves = (uint64_t)gridDim.x * 4;
-  unsigned long long edges = 0, entries = 0, vrev = 0;
   for (uint64_t q = wave; q < qn; q += nwaves) {
     const uint32_t u = queue[q];
     const uint64_t a = off[u], b = off[u + 1];
-    if (lane == 0) edges += b - a;
-    for (uint64_t base = a; base < b; base += 64) {
-      const uint64_t i = base + lane;
-      bool take = false;
-      uint32_t v = 0;
-      if (i < b) {
-        v = tgt[i];
-        if (!bit_of(visited, v) && !(filter && !filter[v])) {
-          atomicMin(&via[v], u);
-          const uint32_t bit = 1u << (v & 31);
-          const uint32_t old = atomicOr(&next_bm[v >> 5], bit);
-          if (!(old & bit)) {
-            take = true;
-            depth[v] = d1;
-            entries += off[v + 1] - off[v];
-            vrev += rev_off[v + 1] - rev_off[v];
-          }
-        }
-      }
-      wave_append(take, v, next_queue, &c->next_size);
+    for (uint64_t i = a + lane; i < b; i += 64) {
+      const uint32_t v = tgt[i];
+      if (!bit_of(visited, v) && !(filter && !filter[v])) cand[v] = 1;
     }
   }
-  add_counters(c, edges, entries, vrev);
 }
 
-// Pull over the scanned CSR's row blocks. `off` are the adjacency's offsets
-// (the found node's next-level entries), roff/rtgt the scanned (reverse)
-// CSR. Pieces of split rows record their first hit (offset within the row)
-// or ~0 in piece_hit for bfs_pull_long.
+// Pull over the scanned CSR's row blocks (roff/rtgt). candidates == 0: every
+// unvisited row admitted by the filter searches; otherwise only rows marked
+// in cand (the push's second half), which are cleared. Pieces of split rows
+// record their first hit (offset within the row) or ~0 in piece_hit for
+// bfs_pull_long. A pulled level's scanned entries go to part_edges.
 __global__ __launch_bounds__(256) void bfs_pull_blocks(
-    const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
-    const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rtgt,
-    const uint32_t* __restrict__ visited, const uint32_t* __restrict__ frontier,
-    const uint8_t* __restrict__ filter, uint32_t* __restrict__ depth, uint32_t* __restrict__ via,
-    uint32_t* __restrict__ next_bm, uint32_t* __restrict__ next_queue, uint32_t* __restrict__ piece_hit,
-    uint32_t d1, BfsCounters* c) {
+    const Block* __restrict__ blocks, const uint64_t* __restrict__ roff,
+    const uint32_t* __restrict__ rtgt, const uint32_t* __restrict__ visited,
+    const uint32_t* __restrict__ frontier, const uint8_t* __restrict__ filter,
+    uint8_t* __restrict__ cand, int candidates, uint32_t* __restrict__ depth,
+    uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm, uint32_t* __restrict__ piece_hit,
+    unsigned long long* __restrict__ part_edges, uint32_t d1) {
   __shared__ uint32_t lbm[kBlockRows / 32 + 2];
   __shared__ uint32_t whit[kBlockThreads / 64];
+  __shared__ unsigned long long red[kBlockThreads / 64];
   const Block b = blocks[blockIdx.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63;
-  unsigned long long edges = 0, entries = 0, vrev = 0;
+  unsigned long long edges = 0;
   if (b.nrows) {
     const uint32_t w0 = b.row0 >> 5, nw = ((b.row0 + b.nrows - 1) >> 5) - w0 + 1;
     for (uint32_t k = tid; k < nw; k += kBlockThreads) lbm[k] = 0;
@@ -162,8 +125,13 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
     const unsigned long long gmask = T == 64 ? ~0ull : ((1ull << T) - 1);
     for (uint32_t r = g; r < nr; r += groups) {
       const uint32_t v = b.row0 + r;
-      bool take = false;
-      if (!bit_of(visited, v) && !(filter && !filter[v])) {
+      bool search;
+      if (candidates) {
+        search = cand[v] != 0;
+      } else {
+        search = !bit_of(visited, v) && !(filter && !filter[v]);
+      }
+      if (search) {
         const uint64_t a = roff[v], e = roff[v + 1];
         bool found = false;
         for (uint64_t base = a; base < e; base += T) {
@@ -180,10 +148,7 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
             if (l == first) {
               via[v] = u;
               depth[v] = d1;
-              take = true;
               atomicOr(&lbm[(v >> 5) - w0], 1u << (v & 31));
-              entries += off[v + 1] - off[v];
-              vrev += e - a;
             }
             if (l == 0) edges += (base - a) + first + 1;
             found = true;
@@ -191,12 +156,12 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
           }
         }
         if (!found && l == 0) edges += e - a;
+        if (candidates && l == 0) cand[v] = 0;
       }
-      wave_append(take, v, next_queue, &c->next_size);
     }
     __syncthreads();
-    // Words wholly inside the block are its own; the two edge words may be
-    // shared with the neighbouring blocks.
+    // Words wholly inside the block are its own; the edge words may be
+    // shared with neighbouring blocks (or hold a split row's bit).
     for (uint32_t k = tid; k < nw; k += kBlockThreads) {
       const uint32_t word = w0 + k, bits = lbm[k];
       const bool whole = word * 32 >= b.row0 && word * 32 + 31 < b.row0 + nr;
@@ -208,7 +173,8 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
   } else {
     const uint32_t v = b.row0;
     uint32_t hit_at = ~0u;
-    if (!bit_of(visited, v) && !(filter && !filter[v])) {
+    const bool search = candidates ? cand[v] != 0 : (!bit_of(visited, v) && !(filter && !filter[v]));
+    if (search) {
       const uint64_t a = roff[v], re = roff[v + 1];
       const uint64_t e0 = a + (uint64_t)b.piece * kBlockNnz;
       const uint64_t e1 = e0 + kBlockNnz < re ? e0 + kBlockNnz : re;
@@ -229,24 +195,27 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
     }
     if (tid == 0) piece_hit[b.pbase + b.piece] = hit_at;
   }
-  add_counters(c, edges, entries, vrev);
+  if (!candidates) {
+    const unsigned long long t = block_reduce_sum(edges, red);
+    if (tid == 0) part_edges[blockIdx.x] = t;
+  }
 }
 
-// A split row found in the pull: the first piece that met the frontier.
+// A split row in the pull: the first piece that met the frontier.
 __global__ __launch_bounds__(256) void bfs_pull_long(
     const uint32_t* __restrict__ long_rows, const uint32_t* __restrict__ long_pbase, uint32_t nlong,
-    const uint64_t* __restrict__ off, const uint64_t* __restrict__ roff,
-    const uint32_t* __restrict__ rtgt, const uint32_t* __restrict__ visited,
-    const uint8_t* __restrict__ filter, const uint32_t* __restrict__ piece_hit,
-    uint32_t* __restrict__ depth, uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm,
-    uint32_t* __restrict__ next_queue, uint32_t d1, BfsCounters* c) {
+    const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rtgt,
+    const uint32_t* __restrict__ visited, const uint8_t* __restrict__ filter, uint8_t* __restrict__ cand,
+    int candidates, const uint32_t* __restrict__ piece_hit, uint32_t* __restrict__ depth,
+    uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm,
+    unsigned long long* __restrict__ part_edges, uint32_t d1) {
+  __shared__ unsigned long long red[4];
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  unsigned long long edges = 0, entries = 0, vrev = 0;
-  bool take = false;
-  uint32_t v = 0;
+  unsigned long long edges = 0;
   if (i < nlong) {
-    v = long_rows[i];
-    if (!bit_of(visited, v) && !(filter && !filter[v])) {
+    const uint32_t v = long_rows[i];
+    const bool search = candidates ? cand[v] != 0 : (!bit_of(visited, v) && !(filter && !filter[v]));
+    if (search) {
       const uint64_t a = roff[v], e = roff[v + 1];
       uint32_t at = ~0u;
       for (uint32_t p = long_pbase[i]; p < long_pbase[i + 1] && at == ~0u; ++p) at = piece_hit[p];
@@ -254,35 +223,121 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
         via[v] = rtgt[a + at];
         depth[v] = d1;
         atomicOr(&next_bm[v >> 5], 1u << (v & 31));
-        take = true;
         edges = (unsigned long long)at + 1;
-        entries = off[v + 1] - off[v];
-        vrev = e - a;
       } else {
         edges = e - a;
       }
+      if (candidates) cand[v] = 0;
     }
   }
-  wave_append(take, v, next_queue, &c->next_size);
-  add_counters(c, edges, entries, vrev);
-}
-
-// visited |= next; the old frontier bitmap is cleared to become the next
-// level's claim bitmap.
-__global__ __launch_bounds__(256) void bfs_merge(uint64_t words, uint32_t* __restrict__ visited,
-                                                 const uint32_t* __restrict__ next_bm,
-                                                 uint32_t* __restrict__ old_frontier) {
-  for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < words; w += (uint64_t)gridDim.x * 256) {
-    visited[w] |= next_bm[w];
-    old_frontier[w] = 0;
+  if (!candidates) {
+    const unsigned long long t = block_reduce_sum(edges, red);
+    if (threadIdx.x == 0) part_edges[blockIdx.x] = t;
   }
 }
 
-__global__ void bfs_clear_counters(BfsCounters* c) {
-  c->next_size = 0;
-  c->next_entries = 0;
-  c->visited_rev = 0;
-  c->edges = 0;
+// Compaction, first pass: per block of words, the claimed nodes and their
+// forward / reverse row entries; also folds the claims into visited and
+// clears the old frontier bitmap (it becomes the next level's claim map).
+__global__ __launch_bounds__(256) void bfs_count(uint64_t words, const uint32_t* __restrict__ claim,
+                                                 uint32_t* __restrict__ visited,
+                                                 uint32_t* __restrict__ old_frontier,
+                                                 const uint64_t* __restrict__ off,
+                                                 const uint64_t* __restrict__ roff,
+                                                 unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long red[4];
+  unsigned long long cnt = 0, ent = 0, vr = 0;
+  const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerCountBlock + threadIdx.x * 2;
+  for (uint64_t w = w0; w < w0 + 2 && w < words; ++w) {
+    uint32_t bits = claim[w];
+    visited[w] |= bits;
+    old_frontier[w] = 0;
+    cnt += __popc(bits);
+    while (bits) {
+      const uint32_t v = (uint32_t)(w * 32) + (__ffs(bits) - 1);
+      bits &= bits - 1;
+      ent += off[v + 1] - off[v];
+      vr += roff[v + 1] - roff[v];
+    }
+  }
+  cnt = block_reduce_sum(cnt, red);
+  ent = block_reduce_sum(ent, red);
+  vr = block_reduce_sum(vr, red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = cnt;
+    part[3 * blockIdx.x + 1] = ent;
+    part[3 * blockIdx.x + 2] = vr;
+  }
+}
+
+// Compaction, second pass (one workgroup): the blocks' queue offsets by an
+// exclusive scan of their counts, and the level's totals.
+__global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long* __restrict__ part,
+                                                 unsigned long long* __restrict__ offsets,
+                                                 const unsigned long long* __restrict__ part_edges,
+                                                 uint32_t nedges, BfsCounters* c) {
+  __shared__ unsigned long long sh[1024];
+  __shared__ unsigned long long red[16];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (nb + 1023) / 1024, b0 = tid * per;
+  unsigned long long local = 0, ent = 0, vr = 0, ed = 0;
+  for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
+    local += part[3 * b];
+    ent += part[3 * b + 1];
+    vr += part[3 * b + 2];
+  }
+  for (uint32_t k = tid; k < nedges; k += 1024) ed += part_edges[k];
+  sh[tid] = local;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+    unsigned long long t = tid >= d ? sh[tid - d] : 0;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  unsigned long long run = sh[tid] - local;
+  for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
+    offsets[b] = run;
+    run += part[3 * b];
+  }
+  ent = block_reduce_sum(ent, red);
+  vr = block_reduce_sum(vr, red);
+  ed = block_reduce_sum(ed, red);
+  if (tid == 0) {
+    c->next_size = sh[1023];
+    c->next_entries = ent;
+    c->visited_rev = vr;
+    c->edges = ed;
+  }
+}
+
+// Compaction, third pass: the claimed nodes in ascending order into the
+// queue, each block at its scanned offset.
+__global__ __launch_bounds__(256) void bfs_write(uint64_t words, const uint32_t* __restrict__ claim,
+                                                 const unsigned long long* __restrict__ offsets,
+                                                 uint32_t* __restrict__ queue) {
+  __shared__ uint32_t sh[256];
+  const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerCountBlock + threadIdx.x * 2;
+  uint32_t bits[2] = {0, 0};
+  for (int k = 0; k < 2; ++k)
+    if (w0 + k < words) bits[k] = claim[w0 + k];
+  const uint32_t mine = __popc(bits[0]) + __popc(bits[1]);
+  sh[threadIdx.x] = mine;
+  __syncthreads();
+  for (uint32_t d = 1; d < 256; d <<= 1) {
+    uint32_t t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  unsigned long long pos = offsets[blockIdx.x] + sh[threadIdx.x] - mine;
+  for (int k = 0; k < 2; ++k) {
+    uint32_t b = bits[k];
+    while (b) {
+      queue[pos++] = (uint32_t)((w0 + k) * 32) + (__ffs(b) - 1);
+      b &= b - 1;
+    }
+  }
 }
 
 struct BfsRun {
@@ -292,19 +347,29 @@ struct BfsRun {
   uint32_t levels = 0;
 };
 
-// The search's workspace, kept with the graph: three bitmaps, two queues,
-// the counters, the split rows' piece hits and a starts buffer.
+// The search's workspace, kept with the graph: bitmaps, queues, candidate
+// bytes, split-row piece hits, compaction partials and counters.
 struct BfsWorkspace {
-  uint32_t *visited, *bm_a, *bm_b, *q_a, *q_b, *piece_hit, *starts;
+  uint32_t *visited, *bm_a, *bm_b, *q_a, *q_b, *piece_hit;
+  uint8_t* cand;
+  unsigned long long *part, *offsets, *part_edges;
   BfsCounters* cnt;
 };
 
+static uint32_t edge_parts(const DevCsr& c) { return c.nblocks + (c.nlong + 255) / 256; }
+
 static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorkspace& w) {
+  const uint64_t nbc = (words + kWordsPerCountBlock - 1) / kWordsPerCountBlock;
   if (!g->bfs_ws[0]) {
     const uint64_t n = std::max<uint64_t>(g->n, 1);
-    const size_t sizes[8] = {words * 4, words * 4, words * 4, n * 4, n * 4,
+    const size_t sizes[8] = {3 * words * 4,
+                             2 * n * 4,
                              std::max<uint32_t>(std::max(g->fwd.npartials, g->rev.npartials), 1) * 4,
-                             n * 4, sizeof(BfsCounters)};
+                             n,
+                             std::max<uint64_t>(nbc, 1) * 3 * 8,
+                             std::max<uint64_t>(nbc, 1) * 8,
+                             std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8,
+                             sizeof(BfsCounters)};
     for (int k = 0; k < 8; ++k) {
       void* p = nullptr;
       int s = dalloc(ctx, &p, sizes[k]);
@@ -312,10 +377,15 @@ static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorksp
       g->workspace.push_back({p, sizes[k]});
       g->bfs_ws[k] = p;
     }
+    // Candidate bytes start clear and every resolve clears what it reads.
+    WG_HIP(ctx, hipMemsetAsync(g->bfs_ws[3], 0, n, ctx->stream));
   }
-  w = BfsWorkspace{(uint32_t*)g->bfs_ws[0], (uint32_t*)g->bfs_ws[1], (uint32_t*)g->bfs_ws[2],
-                   (uint32_t*)g->bfs_ws[3], (uint32_t*)g->bfs_ws[4], (uint32_t*)g->bfs_ws[5],
-                   (uint32_t*)g->bfs_ws[6], (BfsCounters*)g->bfs_ws[7]};
+  uint32_t* bms = (uint32_t*)g->bfs_ws[0];
+  uint32_t* qs = (uint32_t*)g->bfs_ws[1];
+  w = BfsWorkspace{bms, bms + words, bms + 2 * words, qs, qs + std::max<uint64_t>(g->n, 1),
+                   (uint32_t*)g->bfs_ws[2], (uint8_t*)g->bfs_ws[3],
+                   (unsigned long long*)g->bfs_ws[4], (unsigned long long*)g->bfs_ws[5],
+                   (unsigned long long*)g->bfs_ws[6], (BfsCounters*)g->bfs_ws[7]};
   return WG_OK;
 }
 
@@ -331,32 +401,31 @@ int bfs_run(wg_context* ctx, wg_graph* g, int direction, const std::vector<uint3
   DevCsr& rev = direction == WG_IN ? g->fwd : g->rev;
   hipStream_t st = ctx->stream;
   const uint64_t words = (n + 63) / 64 * 2;  // 32-bit words, whole 64-node waves
+  const uint32_t nbc = (uint32_t)((words + kWordsPerCountBlock - 1) / kWordsPerCountBlock);
   BfsWorkspace w;
   int s = bfs_workspace(ctx, g, words, w);
   if (s) return s;
   WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
   WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
-  WG_HIP(ctx, hipMemsetAsync(w.visited, 0, words * 4, st));
-  WG_HIP(ctx, hipMemsetAsync(w.bm_a, 0, words * 4, st));
-  WG_HIP(ctx, hipMemsetAsync(w.bm_b, 0, words * 4, st));
+  WG_HIP(ctx, hipMemsetAsync(w.visited, 0, 3 * words * 4, st));  // visited, bm_a, bm_b
+  uint64_t mf = 0, visited_rev = 0;
   if (!starts.empty()) {
-    WG_HIP(ctx, hipMemcpyAsync(w.starts, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
     WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
-  }
-  bfs_clear_counters<<<1, 1, 0, st>>>(w.cnt);
-  uint32_t* frontier = w.bm_a;  // bitmap of the current frontier
-  uint32_t* claim = w.bm_b;     // bitmap of the nodes claimed this level
-  if (!starts.empty())
     bfs_seed<<<(unsigned)std::min<size_t>((starts.size() + 255) / 256, 1024), 256, 0, st>>>(
-        w.starts, (uint32_t)starts.size(), depth, w.visited, frontier, adj.offsets, rev.offsets, w.cnt);
-  BfsCounters hc{};
-  WG_HIP(ctx, hipMemcpyAsync(&hc, w.cnt, sizeof hc, hipMemcpyDeviceToHost, st));
-  WG_HIP(ctx, hipStreamSynchronize(st));
-  uint64_t qn = starts.size(), mf = hc.next_entries, visited_rev = hc.visited_rev;
+        w.q_a, (uint32_t)starts.size(), depth, w.visited, w.bm_a);
+    for (uint32_t v : starts) {
+      mf += adj.h_offsets[v + 1] - adj.h_offsets[v];
+      visited_rev += rev.h_offsets[v + 1] - rev.h_offsets[v];
+    }
+  }
+  uint64_t qn = starts.size();
   run.visited = qn;
   run.level_size.assign(1, qn);
   uint32_t d = 0;
-  uint32_t *q_cur = w.q_a, *q_next = w.q_b;
+  uint32_t* frontier = w.bm_a;  // bitmap of the current frontier
+  uint32_t* claim = w.bm_b;     // bitmap of the nodes claimed this level
+  bool queue_ready = true;      // q_a holds the frontier as a list
+  BfsCounters hc{};
   while (qn > 0 && !(max_depth >= 0 && (int64_t)d >= max_depth) &&
          !(stop_after >= 0 && run.visited > (uint64_t)stop_after)) {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
@@ -364,32 +433,33 @@ int bfs_run(wg_context* ctx, wg_graph* g, int direction, const std::vector<uint3
     // unvisited nodes' reverse entries are fewer than alpha * m_f.
     const uint64_t m_u = rev.nnz - visited_rev;
     const bool pull = pulling && qn * 24 > n && m_u < mf;
-    bfs_clear_counters<<<1, 1, 0, st>>>(w.cnt);
-    if (pull) {
-      if (rev.nblocks)
-        bfs_pull_blocks<<<rev.nblocks, kBlockThreads, 0, st>>>(
-            rev.blocks, adj.offsets, rev.offsets, rev.targets, w.visited, frontier, filter_d, depth, via,
-            claim, q_next, w.piece_hit, d + 1, w.cnt);
-      if (rev.nlong)
-        bfs_pull_long<<<(rev.nlong + 255) / 256, 256, 0, st>>>(
-            rev.long_rows, rev.long_pbase, rev.nlong, adj.offsets, rev.offsets, rev.targets, w.visited,
-            filter_d, w.piece_hit, depth, via, claim, q_next, d + 1, w.cnt);
-    } else {
-      const unsigned grid = (unsigned)std::min<uint64_t>((qn + 3) / 4, 256 * 64);
-      bfs_push<<<grid, 256, 0, st>>>(q_cur, qn, adj.offsets, adj.targets, rev.offsets, w.visited,
-                                     filter_d, depth, via, claim, q_next, d + 1, w.cnt);
+    if (!pull) {
+      if (!queue_ready)
+        bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
+      bfs_mark<<<(unsigned)std::min<uint64_t>((qn + 3) / 4, 256 * 64), 256, 0, st>>>(
+          w.q_a, qn, adj.offsets, adj.targets, w.visited, filter_d, w.cand);
     }
-    bfs_merge<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 255) / 256, 4096)), 256, 0, st>>>(
-        words, w.visited, claim, frontier);
+    const int cands = pull ? 0 : 1;
+    if (rev.nblocks)
+      bfs_pull_blocks<<<rev.nblocks, kBlockThreads, 0, st>>>(
+          rev.blocks, rev.offsets, rev.targets, w.visited, frontier, filter_d, w.cand, cands, depth,
+          via, claim, w.piece_hit, w.part_edges, d + 1);
+    if (rev.nlong)
+      bfs_pull_long<<<(rev.nlong + 255) / 256, 256, 0, st>>>(
+          rev.long_rows, rev.long_pbase, rev.nlong, rev.offsets, rev.targets, w.visited, filter_d,
+          w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev.nblocks, d + 1);
+    bfs_count<<<nbc, 256, 0, st>>>(words, claim, w.visited, frontier, adj.offsets, rev.offsets, w.part);
+    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(rev) : 0, w.cnt);
     WG_HIP(ctx, hipGetLastError());
     WG_HIP(ctx, hipMemcpyAsync(&hc, w.cnt, sizeof hc, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipStreamSynchronize(st));
     std::swap(frontier, claim);
-    std::swap(q_cur, q_next);
+    queue_ready = false;
+    // A pushed level scans every entry of every frontier row.
+    run.edges += pull ? hc.edges : mf;
     qn = hc.next_size;
     mf = hc.next_entries;
     visited_rev += hc.visited_rev;
-    run.edges += hc.edges;
     run.visited += qn;
     run.level_pulled.push_back(pull ? 1 : 0);
     run.level_size.push_back(qn);
@@ -486,8 +556,8 @@ static int emit(wg_context* ctx, wg_graph* g, int direction, const uint64_t* sta
     WG_HIP(ctx, hipMemcpy(depth.data(), dd.p, n * 4, hipMemcpyDeviceToHost));
     WG_HIP(ctx, hipMemcpy(via.data(), vd.p, n * 4, hipMemcpyDeviceToHost));
   }
-  // Counting sort by depth (nodes ascending within), then per level the
-  // pushed order (via, node) by a stable counting pass over via.
+  // Counting sort by depth (nodes ascending within), then per pushed level
+  // a stable sort by via, which gives (via, node).
   const uint32_t levels = run.levels;
   std::vector<uint64_t> start_of(levels + 2, 0);
   for (uint64_t v = 0; v < n; ++v)
