@@ -165,6 +165,19 @@ int wg_pagerank(wg_context* ctx, const wg_graph* g, const wg_vector* tombstones,
                 const wg_pagerank_opts* opts, const wg_vector* seed, wg_vector* ranks,
                 uint32_t* iterations);
 
+/* Batched personalised PageRank: R (1..256) seed sets run as the columns of
+ * one power iteration; column c equals wg_pagerank with the seed vector that
+ * holds seed_weights[i] (1 when NULL) at seed_nodes[i] for i in
+ * [seed_offsets[c], seed_offsets[c+1]) (repeats add up), and stops on its own
+ * test (L1 change < live nodes * tol), frozen at that iteration while the
+ * others go on. No counterpart in the reference, which takes one seed vector
+ * per call (compute/pagerank/mod.rs:44-46). ranks: f64, id space n, width R;
+ * iterations: R host counters. */
+int wg_pagerank_batch(wg_context* ctx, const wg_graph* g, const wg_vector* tombstones,
+                      const wg_pagerank_opts* opts, const uint64_t* seed_offsets,
+                      const uint64_t* seed_nodes, const double* seed_weights, uint32_t R,
+                      wg_vector* ranks, uint32_t* iterations);
+
 /* ---- traversal: Visit / TraversalStats (traversal/bfs.rs:15-23,
  * traversal/stats.rs:12-20) ------------------------------------------------ */
 typedef struct {
@@ -211,6 +224,19 @@ int wg_bfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* st
 int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
                      uint64_t nstarts, int64_t max_depth, uint64_t* out, uint64_t* count,
                      wg_traversal_stats* stats);
+
+/* Multi-source BFS: SourceBatch::search (compute/centrality/multi_source.rs:
+ * 34-193) widened from 64 sources to up to 512, one bit per source. Every
+ * node's distance from every source along direction, level by level until no
+ * node gains a source (or max_depth >= 0 levels). Outputs, each optional:
+ * depths -- u32, id space n, width nsources (row v, column s = distance from
+ * source s, 0xffffffff unreached); reach -- u32 n, the sources reaching each
+ * node (the count SourceBatch reports per distance, summed); farness -- i64
+ * n, the sum of those distances (closeness centrality's numerator). Sources
+ * may repeat; each position is its own source. levels: levels run. */
+int wg_msbfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* sources,
+             uint32_t nsources, int64_t max_depth, wg_vector* depths, wg_vector* reach,
+             wg_vector* farness, uint32_t* levels);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference: a streaming copy kernel (16-byte loads and

@@ -266,6 +266,32 @@ def pagerank(ctx, graph, *, damping=0.85, tol=1e-6, max_iter=100, seed=None, tom
     return out, it.value
 
 
+def pagerank_batch(ctx, graph, seed_sets, *, seed_weights=None, damping=0.85, tol=1e-6,
+                   max_iter=100, tombstones=None, weighted=None, out=None):
+    """Personalised PageRank for many seed sets at once: column c is
+    pagerank(seed = the vector holding seed_weights[c] (or 1) at
+    seed_sets[c]). Returns (ranks Vector f64 n x R, iterations per column)."""
+    R = len(seed_sets)
+    offs = np.zeros(R + 1, np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in seed_sets])
+    nodes = np.ascontiguousarray(np.concatenate([np.asarray(s, np.uint64) for s in seed_sets])
+                                 if R else np.zeros(0, np.uint64), dtype=np.uint64)
+    wts = None
+    if seed_weights is not None:
+        wts = np.ascontiguousarray(np.concatenate([np.asarray(w, np.float64) for w in seed_weights]))
+    if tombstones is not None and not isinstance(tombstones, Vector):
+        tombstones = Vector(ctx, np.asarray(tombstones, dtype=np.uint8), dtype="bool")
+    if out is None:
+        out = Vector(ctx, dtype="f64", id_space=graph.n, width=R)
+    opts = _lib.PagerankOpts(damping, 0.0 if tol is None else tol, int(tol is not None),
+                             max_iter, -1 if weighted is None else int(weighted), 0)
+    its = np.zeros(R, np.uint32)
+    ctx.check(ctx.lib.wg_pagerank_batch(ctx.h, graph.h, None if tombstones is None else tombstones.h,
+                                        C.byref(opts), _p(offs), _p(nodes), _p(wts), R, out.h,
+                                        _p(its)))
+    return out, its
+
+
 DIRECTIONS = {"out": 0, "in": 1}
 
 
@@ -328,6 +354,24 @@ def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_dep
                                     _p(flt), -1 if max_depth is None else max_depth,
                                     int(pulling), depth.h, via.h, C.byref(stats)))
     return depth, via, _stats(stats)
+
+
+def msbfs(ctx, graph, sources, *, direction="out", max_depth=None, depths=True, reach=True,
+          farness=True):
+    """Multi-source BFS (compute/centrality/multi_source.rs:34-193, up to 512
+    sources): returns (depths Vector u32 n x ns or None, reach Vector u32 or
+    None, farness Vector i64 or None, levels)."""
+    src = _starts(sources)
+    ns = len(src)
+    dv = Vector(ctx, dtype="u32", id_space=graph.n, width=ns) if depths else None
+    rv = Vector(ctx, dtype="u32", id_space=graph.n) if reach else None
+    fv = Vector(ctx, dtype="i64", id_space=graph.n) if farness else None
+    lv = C.c_uint32()
+    ctx.check(ctx.lib.wg_msbfs(ctx.h, graph.h, DIRECTIONS[direction], _p(src), ns,
+                               -1 if max_depth is None else max_depth,
+                               None if dv is None else dv.h, None if rv is None else rv.h,
+                               None if fv is None else fv.h, C.byref(lv)))
+    return dv, rv, fv, lv.value
 
 
 def set_spmv_variant(ctx, variant):

@@ -72,9 +72,11 @@ SIGNATURES = [
     ("wg_set_spmv_variant", None, [P, I32]),
     ("wg_spmv_variant", I32, [P, I32, I32, I32, I32]),
     ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
+    ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
     ("wg_bfs_levels", I32, [P, P, I32, P, U64, P, I64, I32, P, P, P]),
     ("wg_bfs", I32, [P, P, I32, P, U64, P, I64, I64, P, P, P]),
     ("wg_reachable_set", I32, [P, P, I32, P, U64, I64, P, P, P]),
+    ("wg_msbfs", I32, [P, P, I32, P, U32, I64, P, P, P, P]),
     ("wg_device_copy", I32, [P, P, P, U64]),
     ("wg_rmat_csr", I32, [U32, U32, D, D, D, U64, I32, P, P, U64, P, I32]),
     ("wg_edge_weights", I32, [U64, P, P, U64, I32, D, D, P, I32]),
