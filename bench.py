@@ -103,6 +103,85 @@ def cpu_baseline(off, tgt, wf, x, budget_s=15.0):
                       f"{el / reps * 1e3:.1f} ms each"}
 
 
+def power_law_seed_sets(rng, nodes, count, smax=64, alpha=2.0):
+    """Seed sets whose sizes follow P(s) ~ s^-alpha on 1..smax, members drawn
+    uniformly without replacement from `nodes`."""
+    sizes = np.arange(1, smax + 1)
+    p = sizes ** -alpha
+    p /= p.sum()
+    return [rng.choice(nodes, rng.choice(sizes, p=p), replace=False) for _ in range(count)]
+
+
+def bench_batched_ppr(W, ctx, g, n, deg, R=256):
+    """256 personalised PageRanks (tol 1e-10, at most 100 iterations) in one
+    batched run against 256 plain GPU runs, one after another; every column
+    is compared with its plain run (L1)."""
+    rng = np.random.default_rng(256)
+    sets = power_law_seed_sets(rng, np.flatnonzero(deg > 0), R)
+    W.pagerank_batch(ctx, g, sets[:4], tol=1e-10, max_iter=2, weighted=False)  # warm
+    t1 = time.perf_counter()
+    ranks, its = W.pagerank_batch(ctx, g, sets, tol=1e-10, max_iter=100, weighted=False)
+    t_batch = time.perf_counter() - t1
+    batch = ranks.numpy()
+    ranks.close()
+    log(f"batched ppr: {R} seed sets in {t_batch * 1e3:.1f} ms, iterations {its.min()}..{its.max()}")
+    t_plain, worst, it_equal = 0.0, 0.0, 0
+    seed_v = W.Vector(ctx, dtype="f64", id_space=n)
+    out = W.Vector(ctx, dtype="f64", id_space=n)
+    for c, s in enumerate(sets):
+        dense = np.zeros(n)
+        np.add.at(dense, s, 1.0)
+        seed_v.write(dense)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        _, it = W.pagerank(ctx, g, tol=1e-10, max_iter=100, weighted=False, seed=seed_v, out=out)
+        t_plain += time.perf_counter() - t1
+        worst = max(worst, float(np.abs(out.numpy() - batch[:, c]).sum()))
+        it_equal += int(it == its[c])
+    seed_v.close()
+    out.close()
+    log(f"plain ppr x{R}: {t_plain * 1e3:.1f} ms; speedup {t_plain / t_batch:.2f}x; worst L1 {worst:.2e}")
+    return {"seed_sets": R, "ms_batched": round(t_batch * 1e3, 2),
+            "ms_256_plain_runs": round(t_plain * 1e3, 2),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "iterations_min_max": [int(its.min()), int(its.max())],
+            "iterations_equal_to_plain": it_equal,
+            "max_l1_batched_vs_plain": worst}
+
+
+def bench_msbfs(W, ctx, g, n, deg, ns=512):
+    """512-source bitset BFS against 512 single-source searches, one after
+    another; reach counts and farness sums compared node for node."""
+    rng = np.random.default_rng(512)
+    sources = rng.choice(np.flatnonzero(deg > 0), ns, replace=False)
+    W.msbfs(ctx, g, sources[:8], depths=False)  # warm
+    t1 = time.perf_counter()
+    _, rv, fv, levels = W.msbfs(ctx, g, sources, depths=False)
+    t_ms = time.perf_counter() - t1
+    reach, farness = rv.numpy(), fv.numpy()
+    log(f"msbfs: {ns} sources in {t_ms * 1e3:.1f} ms, {levels} levels")
+    depth = W.Vector(ctx, dtype="u32", id_space=n)
+    via = W.Vector(ctx, dtype="u32", id_space=n)
+    t_single = 0.0
+    r_chk = np.zeros(n, np.int64)
+    f_chk = np.zeros(n, np.int64)
+    for s in sources:
+        t1 = time.perf_counter()
+        W.bfs_levels(ctx, g, [int(s)], depth=depth, via=via)
+        t_single += time.perf_counter() - t1
+        d = depth.numpy()
+        hit = d != 0xFFFFFFFF
+        r_chk += hit
+        f_chk += np.where(hit, d, 0)
+    mism = int(np.count_nonzero(r_chk != reach) + np.count_nonzero(f_chk != farness))
+    log(f"single-source bfs x{ns}: {t_single * 1e3:.1f} ms; speedup {t_single / t_ms:.2f}x; "
+        f"mismatches {mism}")
+    return {"sources": ns, "ms": round(t_ms * 1e3, 2), "levels": levels,
+            "ms_512_single_source": round(t_single * 1e3, 2),
+            "speedup_vs_single_source": round(t_single / t_ms, 3),
+            "reach_farness_mismatches_vs_single_source": mism}
+
+
 def oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs, bfs_depths, pr_gpu, pr_iters):
     """Each measured result against the oracle on the same inputs: the f32
     product's largest relative error against the f64 product, mismatches of
@@ -139,6 +218,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true",
+                    help="leave out the batched PageRank and multi-source BFS legs")
     ap.add_argument("--only-spmv", action="store_true",
                     help="profiling runs: the three SpMV legs and the copy kernel only")
     args = ap.parse_args()
@@ -260,6 +341,11 @@ def main():
                         "gbs_per_iteration": round(pr_bytes * iters / el / 1e9, 1)}
     log(f"pagerank: {iters} iterations, {el * 1e3:.1f} ms")
     pr_gpu = ranks.numpy()
+    ranks.close()
+
+    if not args.skip_batched:
+        legs["ppr_batched_256"] = bench_batched_ppr(W, ctx, gf, n, deg)
+        legs["msbfs_512"] = bench_msbfs(W, ctx, gf, n, deg)
 
     head = legs["plus_times_f32"]
     tuned = head.get("tuned_variant", 0)

@@ -259,6 +259,26 @@ static void free_csr(wg_context* ctx, DevCsr& c) {
   c = DevCsr();
 }
 
+int graph_buffer(wg_context* ctx, wg_graph* g, int slot, size_t bytes, void** out) {
+  bytes = std::max<size_t>(bytes, 8);
+  auto it = g->cached.find(slot);
+  if (it != g->cached.end() && it->second.second >= bytes) {
+    *out = it->second.first;
+    return WG_OK;
+  }
+  if (it != g->cached.end()) {
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, it->second.first, it->second.second);
+    g->cached.erase(it);
+  }
+  void* p = nullptr;
+  int s = dalloc(ctx, &p, bytes);
+  if (s) return s;
+  g->cached[slot] = {p, bytes};
+  *out = p;
+  return WG_OK;
+}
+
 int prepare_spmm(wg_context* ctx, DevCsr& c) {
   if (c.spmm_ready) return WG_OK;
   std::vector<uint32_t> extra, ticket_of_row(c.n, ~0u), pbase, lrows;
@@ -294,6 +314,47 @@ int prepare_spmm(wg_context* ctx, DevCsr& c) {
     WG_HIP(ctx, hipMemcpyAsync(c.spmm_long_rows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   c.spmm_ready = true;
+  return WG_OK;
+}
+
+__global__ __launch_bounds__(256) void renumber_sources(const uint32_t* __restrict__ src,
+                                                        const uint32_t* __restrict__ index,
+                                                        uint32_t* __restrict__ out, uint64_t nnz) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * 256)
+    out[i] = index[src[i]];
+}
+
+int prepare_hot_order(wg_context* ctx, wg_graph* g) {
+  if (g->hot_src) return WG_OK;
+  const uint64_t n = g->n;
+  const std::vector<uint64_t>& fo = g->fwd.h_offsets;
+  // Counting sort by out-degree, descending; ascending ids within a degree.
+  const uint64_t maxd = g->fwd.max_degree;
+  std::vector<uint64_t> next(maxd + 2, 0);
+  for (uint64_t v = 0; v < n; ++v) next[fo[v + 1] - fo[v]]++;
+  uint64_t pos = 0;
+  for (uint64_t d = maxd + 1; d-- > 1;) {
+    const uint64_t c = next[d];
+    next[d] = pos;
+    pos += c;
+  }
+  std::vector<uint32_t> index(n, ~0u);
+  for (uint64_t v = 0; v < n; ++v) {
+    const uint64_t d = fo[v + 1] - fo[v];
+    if (d) index[v] = (uint32_t)next[d]++;
+  }
+  void *pi, *ps;
+  int s;
+  if ((s = graph_buffer(ctx, g, kSlotHotIndex, std::max<uint64_t>(n, 1) * 4, &pi))) return s;
+  if ((s = graph_buffer(ctx, g, kSlotHotSrc, std::max<uint64_t>(g->nnz, 1) * 4, &ps))) return s;
+  if (n) WG_HIP(ctx, hipMemcpyAsync(pi, index.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (g->nnz)
+    renumber_sources<<<2048, 256, 0, ctx->stream>>>(g->rev.targets, (const uint32_t*)pi, (uint32_t*)ps, g->nnz);
+  WG_HIP(ctx, hipGetLastError());
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  g->hot_index = (uint32_t*)pi;
+  g->hot_src = (uint32_t*)ps;
+  g->nhot = pos;
   return WG_OK;
 }
 
@@ -485,6 +546,7 @@ void wg_graph_destroy(wg_graph* g) {
   free_csr(g->ctx, g->fwd);
   free_csr(g->ctx, g->rev);
   for (auto& w : g->workspace) dfree(g->ctx, w.first, w.second);
+  for (auto& c : g->cached) dfree(g->ctx, c.second.first, c.second.second);
   delete g;
 }
 

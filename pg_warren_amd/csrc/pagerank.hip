@@ -291,23 +291,24 @@ __global__ void pr_set_state(PrState* st, double dangling0) {
 
 constexpr unsigned kReduceGrid = 1024;
 
-// Device buffers of one PageRank run, charged to the context.
+// PageRank's buffers, kept with the graph between calls (one slot each), so
+// a run allocates nothing once the graph has been ranked before.
 struct PrBuffers {
   wg_context* ctx;
-  std::vector<std::pair<void*, size_t>> held;
-  explicit PrBuffers(wg_context* c) : ctx(c) {}
+  wg_graph* g;
+  int next_slot = 100;
+  PrBuffers(wg_context* c, wg_graph* gr) : ctx(c), g(gr) {}
   template <class T>
   int get(T** p, size_t count) {
+    return get_at(p, count, next_slot++);
+  }
+  template <class T>
+  int get_at(T** p, size_t count, int slot) {
     void* q = nullptr;
-    int s = dalloc(ctx, &q, count * sizeof(T));
+    int s = graph_buffer(ctx, g, slot, count * sizeof(T), &q);
     if (s) return s;
-    held.push_back({q, count * sizeof(T)});
     *p = (T*)q;
     return WG_OK;
-  }
-  ~PrBuffers() {
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto& h : held) dfree(ctx, h.first, h.second);
   }
 };
 
@@ -339,7 +340,7 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
   hipStream_t st = ctx->stream;
   const uint8_t* tomb = tombstones ? (const uint8_t*)tombstones->data : nullptr;
   double* x = (double*)result->data;
-  PrBuffers buf(ctx);
+  PrBuffers buf(ctx, g);
   int s;
   unsigned long long* live_d;
   if ((s = buf.get(&live_d, 1))) return s;
@@ -370,7 +371,7 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
         return fail(ctx, WG_WIDTH_MISMATCH, "width mismatch: expected " + std::to_string(seed->width) + ", found 1");
     } else {
       if (seed->id_space != n) return fail(ctx, WG_ID_SPACE_MISMATCH, "seed id space mismatch");
-      if ((s = buf.get(&part, kReduceGrid))) return s;
+      if ((s = buf.get_at(&part, kReduceGrid, 150))) return s;
       switch (seed->dtype) {
         case WG_F64: pr_seed_sum<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, part); break;
         case WG_F32: pr_seed_sum<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, part); break;
@@ -382,7 +383,7 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
       WG_HIP(ctx, hipStreamSynchronize(st));
       const double total = sqlf<DIV_>(1.0, host_sum(hp), herr);
       if (herr) return fail(ctx, (int)herr, "pagerank: 1 / sum(seed): division by zero");
-      if ((s = buf.get(&p, std::max<uint64_t>(n, 1)))) return s;
+      if ((s = buf.get_at(&p, std::max<uint64_t>(n, 1), 151))) return s;
       switch (seed->dtype) {
         case WG_F64: pr_shares<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, total, p, ctx->d_err); break;
         case WG_F32: pr_shares<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, total, p, ctx->d_err); break;

@@ -6,16 +6,25 @@
 // stop test (L1 change < live nodes * tol); a column that stops is frozen at
 // that iteration while the others go on.
 //
-// One fused kernel per iteration (ppr_iterate): persistent workgroups whose
-// wavefronts take rows in turn; the lanes of a wavefront hold the active
-// columns, so a neighbour's row of messages is one coalesced read of
-// 8 x (active columns) bytes; the gather y = A^T msg is followed in registers
-// by the advance x' = d (y + dangling_c p) + (1 - d) p and the next spread
-// msg' = x' / out. Per-column L1 changes and dangling ranks go to per-block
-// partials, which ppr_finish (one workgroup, a thread per column) sums in a
-// fixed order, deciding which columns stop. Rows longer than kPprPiece
-// entries are gathered in pieces by whole workgroups and advanced by
-// ppr_long. No atomics and no cross-workgroup fences.
+// Layout: the ranks X are n x R row-major (the caller's vector). The
+// messages msg = x / out live in column tiles of kTw columns, tile-major
+// (tile t holds a 128-byte line per message row), holding the active
+// columns only: when columns stop, the survivors are repacked, so an
+// iteration reads ceil(active / kTw) tiles. Message rows are the nodes with
+// out-edges, hottest (largest out-degree) first, so the most gathered lines
+// sit together and stay in L2 and the infinity cache.
+//
+// A tile pass walks the reverse CSR in work items of whole rows (at most
+// kItemNnz entries and kItemRows rows) or pieces of a longer row. A
+// persistent workgroup takes items in turn: it stages the item's offsets
+// and sources in LDS; its kG lane groups (kL lanes, kC columns a lane, one
+// 16-byte load a lane per message line) each walk an equal run of entries
+// with kGatherBatch lines in flight; rows that cross runs are summed in run
+// order; then the rows advance x' = d (y + dangling_c p) + (1 - d) p and
+// spread msg' = x' / out, several rows' loads in flight a group. Pieces of
+// long rows go to a buffer that ppr_long sums in piece order. Per-column L1
+// changes and dangling ranks go to per-workgroup partial slots that
+// ppr_finish sums in a fixed order. No atomics and no cross-workgroup fences.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -27,18 +36,55 @@ namespace wg {
 
 int launch_out_weights(wg_context* ctx, wg_graph* g, bool weighted, const double* x, double* out);
 
-constexpr uint32_t kPprPiece = 4096;   // rows longer than this are split
-constexpr uint32_t kPprMaxCols = 256;  // columns a launch handles (4 per lane)
-constexpr uint32_t kPprGrid = 2048;    // persistent workgroups of ppr_iterate
+constexpr uint32_t kTw = 16;             // columns per tile: a 128-byte message line
+constexpr uint32_t kC = 2;               // columns per lane in the gather kernels
+constexpr uint32_t kL = kTw / kC;        // lanes per row group
+constexpr uint32_t kG = 256 / kL;        // row groups per workgroup
+constexpr uint32_t kRowStep = 256 / kTw; // rows per workgroup step, a lane per column
+constexpr uint32_t kItemNnz = 2048;      // entries per work item
+constexpr uint32_t kItemRows = 128;      // rows per work item
+constexpr uint32_t kGatherBatch = 12;    // message lines in flight a lane
+constexpr uint32_t kPprMaxCols = 256;
+
+// Streaming data (the CSR, ranks, per-row inputs, new messages) bypasses
+// the caches' keep policy so that the gathered message lines stay resident.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+// A lane's kC columns of a message line (one 16-byte access).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+struct Cols {
+  double v[kC];
+};
+__device__ __forceinline__ Cols load_cols(const double* p) {
+  const dvec2 q = *reinterpret_cast<const dvec2*>(p);
+  return Cols{{q.x, q.y}};
+}
+
+// A work item: entries [b0, b0 + nent) of the reverse CSR, being the rows
+// [row0, row0 + nrows), or (nrows == 0) a piece of row row0 whose partial
+// sum goes to piece slot `slot`.
+struct PprItem {
+  uint64_t b0;
+  uint32_t row0, nrows, slot, nent;
+};
 
 struct PprArgs {
   const uint64_t* off;    // reverse CSR
-  const uint32_t* src;
+  const uint32_t* src;    // its sources as message rows
   const double* w;        // reverse weights or null
   uint64_t n;
-  uint32_t R;             // row stride of X, M (columns of the batch)
-  uint32_t nact;          // active columns this iteration
-  const uint32_t* cols;   // their indices
+  const uint32_t* hot;    // n: message row of each node (~0: no out-edges)
+  uint64_t nm;            // message rows
+  uint32_t R;             // row stride of X
+  uint32_t nact;          // active columns (positions 0..nact-1)
+  const uint32_t* cols;   // column of each position
   const double* out;      // out-weights
   const uint8_t* tomb;
   const uint32_t* seed_row;  // n: index into P, or ~0
@@ -46,193 +92,358 @@ struct PprArgs {
   double damping, teleport;
   const double* dangling;    // R: this iteration's dangling rank per column
   double* X;
-  const double* Min;
+  const double* Min;         // tiles x nm x kTw, positions of the active list
   double* Mout;
-  double* part;              // (grid + nlong) x 2 x R: change, dangling
-  double* piece;             // split-row pieces x R
+  double* part;              // per tile: nparts x 2 x kTw (change, dangling)
+  double* piece;             // per tile: pieces x kTw
+  uint32_t nparts;           // slots: ppr_block then ppr_long workgroups
   unsigned long long* err;
 };
 
-// Rows handled by ppr_iterate: all but the split ones.
-__device__ __forceinline__ double ppr_update(const PprArgs& a, uint64_t v, uint32_t c, double y,
-                                             double& change, double& dang, uint32_t& err) {
-  const uint64_t idx = v * a.R + c;
-  const double xo = a.X[idx];
-  double next = 0.0;
-  if (!(a.tomb && a.tomb[v])) {
-    const uint32_t sr = a.seed_row[v];
-    const double p = sr == ~0u ? 0.0 : a.P[(uint64_t)sr * a.R + c];
-    const double flowed = sqlf<ADD_>(y, sqlf<MUL_>(a.dangling[c], p, err), err);
-    next = sqlf<ADD_>(sqlf<MUL_>(a.damping, flowed, err), sqlf<MUL_>(a.teleport, p, err), err);
+// A lane's columns: position t * kTw + kC * k + q is column c[q] when on[q].
+struct LaneCols {
+  uint32_t c[kC];
+  bool on[kC];
+};
+__device__ __forceinline__ LaneCols lane_cols(const PprArgs& a, uint32_t t, uint32_t k) {
+  LaneCols lc;
+#pragma unroll
+  for (uint32_t q = 0; q < kC; ++q) {
+    const uint32_t pos = t * kTw + kC * k + q;
+    lc.on[q] = pos < a.nact;
+    lc.c[q] = lc.on[q] ? a.cols[pos] : 0;
   }
-  a.X[idx] = next;
-  const double o = a.out[v];
-  if (sql_lt(0.0, o)) {
-    a.Mout[idx] = sqlf<DIV_>(next, o, err);
-  } else {
-    dang += next;
-    a.Mout[idx] = 0.0;
-  }
-  change += fabs(next - xo);
-  return next;
+  return lc;
 }
 
-template <bool HW>
-__global__ __launch_bounds__(256) void ppr_iterate(PprArgs a, uint32_t nlong_pieces) {
-  __shared__ double red[2][4][kPprMaxCols];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  constexpr int J = kPprMaxCols / 64;
-  uint32_t col[J];
-  bool on[J];
+// What a row's update reads besides its gathered sums.
+struct PprRowIn {
+  double xo[kC];
+  double o;
+  uint32_t h, sr;
+  bool dead;
+};
+
+__device__ __forceinline__ PprRowIn ppr_load(const PprArgs& a, uint64_t v, const LaneCols& lc) {
+  PprRowIn in;
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    const uint32_t k = lane + 64 * j;
-    on[j] = k < a.nact;
-    col[j] = on[j] ? a.cols[k] : 0;
-  }
-  double change[J], dang[J];
+  for (uint32_t q = 0; q < kC; ++q) in.xo[q] = lc.on[q] ? ld_nt(a.X + v * a.R + lc.c[q]) : 0.0;
+  in.o = ld_nt(a.out + v);
+  in.h = ld_nt(a.hot + v);
+  in.sr = ld_nt(a.seed_row + v);
+  in.dead = a.tomb && ld_nt(a.tomb + v);
+  return in;
+}
+
+__device__ __forceinline__ void ppr_advance(const PprArgs& a, uint64_t v, const LaneCols& lc, uint32_t t,
+                                            uint32_t k, const double* y, const PprRowIn& in, double* change,
+                                            double* dang, uint32_t& err) {
+  double m[kC];
 #pragma unroll
-  for (int j = 0; j < J; ++j) change[j] = dang[j] = 0.0;
-  uint32_t e_err = 0;
-  const uint64_t nwaves = (uint64_t)gridDim.x * 4;
-  for (uint64_t v = (uint64_t)blockIdx.x * 4 + wv; v < a.n; v += nwaves) {
-    const uint64_t b = a.off[v], e = a.off[v + 1];
-    if (e - b > kPprPiece) continue;  // split rows: ppr_piece / ppr_long
-    double acc[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) acc[j] = 0.0;
-    for (uint64_t i = b; i < e; ++i) {
-      const uint64_t u = a.src[i];
-      const double wgt = HW ? a.w[i] : 1.0;
-      const double* mrow = a.Min + u * a.R;
-#pragma unroll
-      for (int j = 0; j < J; ++j)
-        if (on[j]) acc[j] += HW ? wgt * mrow[col[j]] : mrow[col[j]];
+  for (uint32_t q = 0; q < kC; ++q) {
+    m[q] = 0.0;
+    if (!lc.on[q]) continue;
+    const uint32_t c = lc.c[q];
+    double next = 0.0;
+    if (!in.dead) {
+      const double p = in.sr == ~0u ? 0.0 : a.P[(uint64_t)in.sr * a.R + c];
+      const double flowed = sqlf<ADD_>(y[q], sqlf<MUL_>(a.dangling[c], p, err), err);
+      next = sqlf<ADD_>(sqlf<MUL_>(a.damping, flowed, err), sqlf<MUL_>(a.teleport, p, err), err);
     }
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (on[j]) ppr_update(a, v, col[j], acc[j], change[j], dang[j], e_err);
+    st_nt(a.X + v * a.R + c, next);
+    if (sql_lt(0.0, in.o)) m[q] = sqlf<DIV_>(next, in.o, err);
+    else dang[q] += next;
+    change[q] += fabs(next - in.xo[q]);
   }
+  if (in.h != ~0u) {
+    const dvec2 mm = {m[0], m[1]};
+    st_nt(reinterpret_cast<dvec2*>(a.Mout + ((uint64_t)t * a.nm + in.h) * kTw + kC * k), mm);
+  }
+}
+
+// Sums (change, dangling) of the kG row groups of a workgroup (lane k's
+// columns of each) into the workgroup's partial slot; `red` holds
+// 2 x kG x kTw doubles.
+__device__ __forceinline__ void ppr_block_partial(const PprArgs& a, uint32_t t, uint32_t slot,
+                                                  const double* change, const double* dang, double* red) {
+  const uint32_t tid = threadIdx.x, k = tid % kL, g = tid / kL;
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    red[0][wv][lane + 64 * j] = change[j];
-    red[1][wv][lane + 64 * j] = dang[j];
+  for (uint32_t q = 0; q < kC; ++q) {
+    red[(0 * kG + g) * kTw + kC * k + q] = change[q];
+    red[(1 * kG + g) * kTw + kC * k + q] = dang[q];
   }
   __syncthreads();
-  for (uint32_t k = tid; k < a.nact; k += 256) {
-    double c = 0.0, d = 0.0;
-    for (int q = 0; q < 4; ++q) {
-      c += red[0][q][k];
-      d += red[1][q][k];
-    }
-    a.part[(uint64_t)blockIdx.x * 2 * a.R + k] = c;
-    a.part[(uint64_t)blockIdx.x * 2 * a.R + a.R + k] = d;
+  if (tid < 2 * kTw) {
+    const uint32_t which = tid / kTw, col = tid % kTw;
+    double s = 0.0;
+    for (uint32_t q = 0; q < kG; ++q) s += red[(which * kG + q) * kTw + col];
+    a.part[((uint64_t)t * a.nparts + slot) * 2 * kTw + tid] = s;
   }
+}
+
+// Four workgroups a CU unweighted (LDS and registers sized for it), three
+// weighted (the weights take another 16 KB of LDS).
+template <bool HW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4))) void ppr_block(PprArgs a, uint32_t t, const PprItem* __restrict__ items,
+                                                 uint32_t nitems, uint32_t npieces) {
+  __shared__ uint32_t ss[kItemNnz];
+  __shared__ double sw[HW ? kItemNnz : 1];
+  __shared__ uint64_t so[kItemRows + 1];
+  __shared__ double Y[kItemRows * kTw];          // rows summed within one run
+  __shared__ double segf[kG * kTw], segl[kG * kTw];  // a run's first / last row, when cut
+  const uint32_t tid = threadIdx.x, k = tid % kL, g = tid / kL;
+  const LaneCols lc = lane_cols(a, t, k);
+  const bool any_on = lc.on[0];
+  const double* tile = a.Min + (uint64_t)t * a.nm * kTw + kC * k;
+  double change[kC] = {}, dang[kC] = {};
+  uint32_t e_err = 0;
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const PprItem item = items[it];
+    const bool piece = item.nrows == 0;
+    const uint32_t nr = piece ? 1 : item.nrows;
+    const uint64_t b0 = item.b0;
+    const uint32_t nent = item.nent;
+    if (piece) {
+      if (tid < 2) so[tid] = b0 + (tid ? nent : 0);
+    } else {
+      for (uint32_t i = tid; i <= nr; i += 256) so[i] = ld_nt(a.off + (uint64_t)item.row0 + i);
+    }
+    for (uint32_t i = tid; i < nent; i += 256) {
+      ss[i] = ld_nt(a.src + b0 + i);
+      if (HW) sw[i] = ld_nt(a.w + b0 + i);
+    }
+    __syncthreads();
+    // Group g's run of entries [c0, c1) (item-relative).
+    const uint32_t chunk = (nent + kG - 1) / kG;
+    const uint32_t c0 = min(g * chunk, nent), c1 = min(c0 + chunk, nent);
+    if (c0 < c1) {
+      uint32_t lo = 0, hi = nr;  // so[lo] <= b0 + c0 < so[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (so[mid] - b0 <= c0) lo = mid;
+        else hi = mid;
+      }
+      uint32_t r = lo, rend = (uint32_t)(so[r + 1] - b0);
+      double acc[kC] = {};
+      auto flush = [&]() {
+        double* dst = so[r] - b0 < c0 ? &segf[g * kTw] : rend > c1 ? &segl[g * kTw] : &Y[r * kTw];
+#pragma unroll
+        for (uint32_t q = 0; q < kC; ++q) dst[kC * k + q] = acc[q];
+      };
+      for (uint32_t e = c0; e < c1; e += kGatherBatch) {
+        const uint32_t nb = min(kGatherBatch, c1 - e);
+        Cols m[kGatherBatch];
+        double ws[kGatherBatch];
+        if (nb == kGatherBatch) {
+#pragma unroll
+          for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            m[j] = load_cols(tile + (uint64_t)ss[e + j] * kTw);
+            ws[j] = HW ? sw[e + j] : 1.0;
+          }
+        } else {
+#pragma unroll
+          for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            m[j] = Cols{};
+            ws[j] = 1.0;
+            if (j < nb) {
+              m[j] = load_cols(tile + (uint64_t)ss[e + j] * kTw);
+              if (HW) ws[j] = sw[e + j];
+            }
+          }
+        }
+        if (e + nb <= rend) {
+          // No row ends before the batch does.
+#pragma unroll
+          for (uint32_t j = 0; j < kGatherBatch; ++j)
+            if (j < nb)
+#pragma unroll
+              for (uint32_t q = 0; q < kC; ++q) acc[q] += HW ? ws[j] * m[j].v[q] : m[j].v[q];
+        } else {
+#pragma unroll
+          for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            if (j < nb) {
+              while (e + j >= rend) {
+                flush();
+#pragma unroll
+                for (uint32_t q = 0; q < kC; ++q) acc[q] = 0.0;
+                ++r;
+                rend = (uint32_t)(so[r + 1] - b0);
+              }
+#pragma unroll
+              for (uint32_t q = 0; q < kC; ++q) acc[q] += HW ? ws[j] * m[j].v[q] : m[j].v[q];
+            }
+          }
+        }
+      }
+      flush();
+    }
+    __syncthreads();
+    // A row's sums: within one run, or its runs' parts in run order.
+    auto row_sum = [&](uint32_t r, double* y) {
+      const uint32_t rb = (uint32_t)(so[r] - b0), re = (uint32_t)(so[r + 1] - b0);
+#pragma unroll
+      for (uint32_t q = 0; q < kC; ++q) y[q] = 0.0;
+      if (rb == re) return;
+      const uint32_t ga = rb / chunk, gb = (re - 1) / chunk;
+      if (ga == gb) {
+#pragma unroll
+        for (uint32_t q = 0; q < kC; ++q) y[q] = Y[r * kTw + kC * k + q];
+        return;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kC; ++q) y[q] = segl[ga * kTw + kC * k + q];
+      for (uint32_t s = ga + 1; s <= gb; ++s)
+#pragma unroll
+        for (uint32_t q = 0; q < kC; ++q) y[q] += segf[s * kTw + kC * k + q];
+    };
+    if (piece) {
+      if (g == 0) {
+        double y[kC];
+        row_sum(0, y);
+#pragma unroll
+        for (uint32_t q = 0; q < kC; ++q) a.piece[((uint64_t)t * npieces + item.slot) * kTw + kC * k + q] = y[q];
+      }
+    } else if (any_on) {
+      constexpr uint32_t kBatch = 2;
+      for (uint32_t r0 = g; r0 < nr; r0 += kBatch * kG) {
+        PprRowIn in[kBatch];
+        double y[kBatch][kC];
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+          const uint32_t r = r0 + b * kG;
+          if (r < nr) {
+            in[b] = ppr_load(a, (uint64_t)item.row0 + r, lc);
+            row_sum(r, y[b]);
+          }
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+          const uint32_t r = r0 + b * kG;
+          if (r < nr) ppr_advance(a, (uint64_t)item.row0 + r, lc, t, k, y[b], in[b], change, dang, e_err);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  ppr_block_partial(a, t, blockIdx.x, change, dang, Y);
   if (e_err) raise_error(a.err, blockIdx.x, e_err);
 }
 
-// One piece of a split row: four wavefronts take its entries in turn and
-// their column sums meet in LDS.
-template <bool HW>
-__global__ __launch_bounds__(256) void ppr_piece(PprArgs a, const uint32_t* __restrict__ pieces) {
-  __shared__ double red[4][kPprMaxCols];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  constexpr int J = kPprMaxCols / 64;
-  const uint32_t v = pieces[3 * blockIdx.x], p = pieces[3 * blockIdx.x + 1];
-  const uint32_t slot = pieces[3 * blockIdx.x + 2];
-  const uint64_t b = a.off[v], e = a.off[v + 1];
-  const uint64_t p0 = b + (uint64_t)p * kPprPiece, p1 = p0 + kPprPiece < e ? p0 + kPprPiece : e;
-  double acc[J];
-  uint32_t col[J];
-  bool on[J];
-#pragma unroll
-  for (int j = 0; j < J; ++j) {
-    acc[j] = 0.0;
-    const uint32_t k = lane + 64 * j;
-    on[j] = k < a.nact;
-    col[j] = on[j] ? a.cols[k] : 0;
-  }
-  for (uint64_t i = p0 + wv; i < p1; i += 4) {
-    const uint64_t u = a.src[i];
-    const double wgt = HW ? a.w[i] : 1.0;
-    const double* mrow = a.Min + u * a.R;
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (on[j]) acc[j] += HW ? wgt * mrow[col[j]] : mrow[col[j]];
-  }
-#pragma unroll
-  for (int j = 0; j < J; ++j) red[wv][lane + 64 * j] = acc[j];
-  __syncthreads();
-  for (uint32_t k = tid; k < a.nact; k += 256)
-    a.piece[(uint64_t)slot * a.R + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
-}
-
-// A split row: its pieces' sums in piece order, then the update.
-__global__ __launch_bounds__(256) void ppr_long(PprArgs a, const uint32_t* __restrict__ rows,
+// The long rows of a tile, kG a workgroup: pieces summed in piece order,
+// then the update.
+__global__ __launch_bounds__(256) void ppr_long(PprArgs a, uint32_t t, const uint32_t* __restrict__ rows,
                                                 const uint32_t* __restrict__ pbase, uint32_t nlong,
-                                                uint32_t grid_parts) {
-  const uint32_t i = blockIdx.x;
-  const uint64_t v = rows[i];
+                                                uint32_t npieces, uint32_t slot0) {
+  __shared__ double red[2 * kG * kTw];
+  const uint32_t i = blockIdx.x * kG + threadIdx.x / kL, k = threadIdx.x % kL;
+  const LaneCols lc = lane_cols(a, t, k);
   uint32_t e_err = 0;
-  for (uint32_t k = threadIdx.x; k < a.nact; k += 256) {
-    const uint32_t c = a.cols[k];
-    double y = 0.0;
-    for (uint32_t p = pbase[i]; p < pbase[i + 1]; ++p) y += a.piece[(uint64_t)p * a.R + k];
-    double change = 0.0, dang = 0.0;
-    ppr_update(a, v, c, y, change, dang, e_err);
-    a.part[(uint64_t)(grid_parts + i) * 2 * a.R + k] = change;
-    a.part[(uint64_t)(grid_parts + i) * 2 * a.R + a.R + k] = dang;
+  double change[kC] = {}, dang[kC] = {};
+  if (i < nlong && lc.on[0]) {
+    const uint64_t v = rows[i];
+    double y[kC] = {};
+    for (uint32_t p = pbase[i]; p < pbase[i + 1]; ++p)
+#pragma unroll
+      for (uint32_t q = 0; q < kC; ++q) y[q] += a.piece[((uint64_t)t * npieces + p) * kTw + kC * k + q];
+    ppr_advance(a, v, lc, t, k, y, ppr_load(a, v, lc), change, dang, e_err);
   }
-  if (e_err) raise_error(a.err, v, e_err);
+  ppr_block_partial(a, t, slot0 + blockIdx.x, change, dang, red);
+  if (e_err) raise_error(a.err, blockIdx.x, e_err);
 }
 
-// Per column, in a fixed order: the iteration's L1 change and the next
-// dangling rank; a column whose change falls below the bound stops.
+// A workgroup per active position, in a fixed order: the iteration's L1
+// change and the next dangling rank; a column whose change falls below the
+// bound stops (when count is set; the set-up pass only takes the dangling).
 __global__ __launch_bounds__(256) void ppr_finish(const double* __restrict__ part, uint32_t nparts,
-                                                  uint32_t R, uint32_t nact, const uint32_t* __restrict__ cols,
-                                                  double* __restrict__ dangling, double* __restrict__ delta,
-                                                  uint32_t* __restrict__ iterations, uint8_t* __restrict__ active,
-                                                  double bound, int has_tol) {
-  for (uint32_t k = threadIdx.x; k < nact; k += blockDim.x) {
-    double c = 0.0, d = 0.0;
-    for (uint32_t b = 0; b < nparts; ++b) {
-      c += part[(uint64_t)b * 2 * R + k];
-      d += part[(uint64_t)b * 2 * R + R + k];
+                                                  const uint32_t* __restrict__ cols,
+                                                  double* __restrict__ dangling, uint32_t* __restrict__ iterations,
+                                                  uint8_t* __restrict__ active, double bound, int has_tol,
+                                                  int count) {
+  __shared__ double sc[256], sd[256];
+  const uint32_t pos = blockIdx.x, t = pos / kTw, k = pos % kTw, tid = threadIdx.x;
+  const double* p = part + (uint64_t)t * nparts * 2 * kTw;
+  double c = 0.0, d = 0.0;
+  for (uint32_t b = tid; b < nparts; b += 256) {
+    c += p[(uint64_t)b * 2 * kTw + k];
+    d += p[(uint64_t)b * 2 * kTw + kTw + k];
+  }
+  sc[tid] = c;
+  sd[tid] = d;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      sc[tid] += sc[tid + s];
+      sd[tid] += sd[tid + s];
     }
-    const uint32_t col = cols[k];
-    delta[col] = c;
-    dangling[col] = d;
-    iterations[col] += 1;
-    if (has_tol && sql_lt(c, bound)) active[col] = 0;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const uint32_t col = cols[pos];
+    dangling[col] = sd[0];
+    if (count) {
+      iterations[col] += 1;
+      if (has_tol && sql_lt(sc[0], bound)) active[col] = 0;
+    }
   }
 }
 
-// x = start at live rows, msg = x / out, and per-block dangling partials.
+// x = start at live rows, msg = x / out in the first layout (every column
+// active, position = column), and per-workgroup dangling partials.
 __global__ __launch_bounds__(256) void ppr_start(uint64_t n, uint32_t R, double start,
                                                  const uint8_t* __restrict__ tomb,
+                                                 const uint32_t* __restrict__ hot, uint64_t nm,
                                                  const double* __restrict__ out, double* __restrict__ X,
                                                  double* __restrict__ M, double* __restrict__ part,
-                                                 unsigned long long* err) {
-  // Thread k of a block owns column k (R <= 256); blocks take rows in turn.
-  const uint32_t k = threadIdx.x;
+                                                 uint32_t nparts, unsigned long long* err) {
+  __shared__ double red[kRowStep][kTw];
+  const uint32_t tid = threadIdx.x, k = tid % kTw, r = tid / kTw;
   uint32_t e_err = 0;
-  double dang = 0.0;
-  if (k < R) {
-    for (uint64_t v = blockIdx.x; v < n; v += gridDim.x) {
+  const uint32_t ntiles = (R + kTw - 1) / kTw;
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    const uint32_t c = t * kTw + k;
+    double dang = 0.0;
+    for (uint64_t v = (uint64_t)blockIdx.x * kRowStep + r; v < n; v += (uint64_t)gridDim.x * kRowStep) {
+      if (c >= R) continue;
       const double x = (tomb && tomb[v]) ? 0.0 : start;
-      X[v * R + k] = x;
+      X[v * R + c] = x;
       const double o = out[v];
-      if (sql_lt(0.0, o)) {
-        M[v * R + k] = sqlf<DIV_>(x, o, e_err);
-      } else {
-        dang += x;
-        M[v * R + k] = 0.0;
-      }
+      double m = 0.0;
+      if (sql_lt(0.0, o)) m = sqlf<DIV_>(x, o, e_err);
+      else dang += x;
+      if (hot[v] != ~0u) M[((uint64_t)t * nm + hot[v]) * kTw + k] = m;
     }
-    part[(uint64_t)blockIdx.x * 2 * R + k] = 0.0;
-    part[(uint64_t)blockIdx.x * 2 * R + R + k] = dang;
+    red[r][k] = dang;
+    __syncthreads();
+    if (tid < kTw) {
+      double ds = 0.0;
+      for (uint32_t q = 0; q < kRowStep; ++q) ds += red[q][tid];
+      double* p = part + ((uint64_t)t * nparts + blockIdx.x) * 2 * kTw;
+      p[tid] = 0.0;
+      p[kTw + tid] = ds;
+    }
+    __syncthreads();
   }
   if (e_err) raise_error(err, blockIdx.x, e_err);
+}
+
+// Moves the surviving positions' messages into a packed layout: position
+// newpos takes old position from[newpos].
+__global__ __launch_bounds__(256) void ppr_repack(uint64_t n, const double* __restrict__ Mold,
+                                                  double* __restrict__ Mnew, const uint32_t* __restrict__ from,
+                                                  uint32_t nact) {
+  const uint32_t k = threadIdx.x % kTw;
+  const uint32_t ntiles = (nact + kTw - 1) / kTw;
+  for (uint64_t i = (uint64_t)blockIdx.x * kRowStep + threadIdx.x / kTw; i < n * ntiles;
+       i += (uint64_t)gridDim.x * kRowStep) {
+    const uint64_t t = i / n, v = i % n;
+    const uint32_t pos = (uint32_t)(t * kTw + k);
+    double m = 0.0;
+    if (pos < nact) {
+      const uint32_t f = from[pos];
+      m = Mold[((uint64_t)(f / kTw) * n + v) * kTw + f % kTw];
+    }
+    Mnew[(t * n + v) * kTw + k] = m;
+  }
 }
 
 __global__ void ppr_count_live(uint64_t n, const uint8_t* __restrict__ tomb, unsigned long long* live) {
@@ -241,6 +452,62 @@ __global__ void ppr_count_live(uint64_t n, const uint8_t* __restrict__ tomb, uns
     c += !(tomb && tomb[i]);
   for (int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(live, c);
+}
+
+// Work items over the reverse CSR, in row order: runs of whole rows (at
+// most kItemNnz entries and kItemRows rows) and, for a row longer than
+// kItemNnz, one item per kItemNnz-entry piece. Built once per graph.
+int prepare_ppr_plan(wg_context* ctx, wg_graph* g) {
+  if (g->ppr_items) return WG_OK;
+  const std::vector<uint64_t>& off = g->rev.h_offsets;
+  const uint64_t n = g->n;
+  std::vector<PprItem> items;
+  std::vector<uint32_t> lrows, lbase;
+  uint32_t npieces = 0;
+  for (uint64_t v = 0; v < n;) {
+    const uint64_t deg = off[v + 1] - off[v];
+    if (deg > kItemNnz) {
+      const uint32_t np = (uint32_t)((deg + kItemNnz - 1) / kItemNnz);
+      lrows.push_back((uint32_t)v);
+      lbase.push_back(npieces);
+      for (uint32_t p = 0; p < np; ++p) {
+        const uint64_t p0 = off[v] + (uint64_t)p * kItemNnz;
+        items.push_back({p0, (uint32_t)v, 0, npieces + p, (uint32_t)std::min<uint64_t>(kItemNnz, off[v + 1] - p0)});
+      }
+      npieces += np;
+      ++v;
+      continue;
+    }
+    const uint64_t r0 = v;
+    uint64_t ent = 0;
+    while (v < n && v - r0 < kItemRows) {
+      const uint64_t d = off[v + 1] - off[v];
+      if (d > kItemNnz || ent + d > kItemNnz) break;
+      ent += d;
+      ++v;
+    }
+    items.push_back({off[r0], (uint32_t)r0, (uint32_t)(v - r0), 0, (uint32_t)ent});
+  }
+  lbase.push_back(npieces);
+  void *pi, *pl, *pb;
+  int s;
+  if ((s = graph_buffer(ctx, g, kSlotPprItems, std::max<size_t>(items.size(), 1) * sizeof(PprItem), &pi)) ||
+      (s = graph_buffer(ctx, g, kSlotPprLrows, std::max<size_t>(lrows.size(), 1) * 4, &pl)) ||
+      (s = graph_buffer(ctx, g, kSlotPprLbase, lbase.size() * 4, &pb)))
+    return s;
+  if (!items.empty())
+    WG_HIP(ctx, hipMemcpyAsync(pi, items.data(), items.size() * sizeof(PprItem), hipMemcpyHostToDevice, ctx->stream));
+  if (!lrows.empty())
+    WG_HIP(ctx, hipMemcpyAsync(pl, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  WG_HIP(ctx, hipMemcpyAsync(pb, lbase.data(), lbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  g->ppr_items = pi;
+  g->ppr_lrows = (uint32_t*)pl;
+  g->ppr_lbase = (uint32_t*)pb;
+  g->ppr_nitems = (uint32_t)items.size();
+  g->ppr_npieces = npieces;
+  g->ppr_nlong = (uint32_t)lrows.size();
+  return WG_OK;
 }
 
 }  // namespace wg
@@ -346,88 +613,85 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   const double bound = o->has_tol ? sqlf<MUL_>(nodes, o->tol, bound_err) : 0.0;
   const bool weighted = o->weighted < 0 ? g->weighted : o->weighted != 0;
   DevCsr& rev = g->rev;
-  // Split rows and their pieces.
-  std::vector<uint32_t> lrows, lbase, pieces;
-  for (uint64_t v = 0; v < n; ++v) {
-    const uint64_t deg = rev.h_offsets[v + 1] - rev.h_offsets[v];
-    if (deg <= kPprPiece) continue;
-    lbase.push_back((uint32_t)(pieces.size() / 3));
-    const uint32_t np = (uint32_t)((deg + kPprPiece - 1) / kPprPiece);
-    for (uint32_t p = 0; p < np; ++p) {
-      pieces.push_back((uint32_t)v);
-      pieces.push_back(p);
-      pieces.push_back((uint32_t)(pieces.size() / 3));
-    }
-    lrows.push_back((uint32_t)v);
-  }
-  const uint32_t npieces = (uint32_t)(pieces.size() / 3), nlong = (uint32_t)lrows.size();
-  lbase.push_back(npieces);
-  const size_t nR = (size_t)std::max<uint64_t>(n, 1) * R;
-  double *X = (double*)ranks->data, *M0, *M1, *out, *part, *piece, *Pd, *dangling, *delta;
-  uint32_t *seed_row_d, *cols_d, *iters_d, *lrows_d, *lbase_d, *pieces_d;
+  if ((s = prepare_ppr_plan(ctx, g)) || (s = prepare_hot_order(ctx, g))) return s;
+  const uint32_t nitems = g->ppr_nitems, npieces = g->ppr_npieces, nlong = g->ppr_nlong;
+  const PprItem* items = (const PprItem*)g->ppr_items;
+  // One persistent workgroup per resident slot.
+  int per_cu = 0, cus = 0;
+  WG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, weighted ? (const void*)ppr_block<true> : (const void*)ppr_block<false>, 256, 0));
+  WG_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(std::max(per_cu, 1) * std::max(cus, 1),
+                                                                 std::max<uint32_t>(nitems, 1)));
+  const uint32_t ntiles = (R + kTw - 1) / kTw;
+  const uint32_t long_blocks = (nlong + kG - 1) / kG;
+  const uint32_t nparts = grid + long_blocks;
+  const uint64_t nm = g->nhot;
+  const size_t tiles_bytes = (size_t)ntiles * std::max<uint64_t>(nm, 1) * kTw * 8;
+  double *X = (double*)ranks->data, *M0, *M1, *out, *part, *piece, *Pd, *dangling;
+  uint32_t *seed_row_d, *cols_d, *from_d, *iters_d;
   uint8_t* active_d;
-  const uint32_t nparts = kPprGrid + nlong;
-  if ((s = get((void**)&M0, nR * 8)) || (s = get((void**)&M1, nR * 8)) ||
+  if ((s = get((void**)&M0, tiles_bytes)) || (s = get((void**)&M1, tiles_bytes)) ||
       (s = get((void**)&out, std::max<uint64_t>(n, 1) * 8)) ||
-      (s = get((void**)&part, (size_t)nparts * 2 * R * 8)) ||
-      (s = get((void**)&piece, (size_t)std::max<uint32_t>(npieces, 1) * R * 8)) ||
+      (s = get((void**)&part, (size_t)ntiles * nparts * 2 * kTw * 8)) ||
+      (s = get((void**)&piece, (size_t)ntiles * std::max<uint32_t>(npieces, 1) * kTw * 8)) ||
       (s = get((void**)&Pd, P.size() * 8)) || (s = get((void**)&dangling, R * 8)) ||
-      (s = get((void**)&delta, R * 8)) || (s = get((void**)&seed_row_d, std::max<uint64_t>(n, 1) * 4)) ||
-      (s = get((void**)&cols_d, R * 4)) || (s = get((void**)&iters_d, R * 4)) ||
-      (s = get((void**)&active_d, R)) || (s = get((void**)&lrows_d, std::max<uint32_t>(nlong, 1) * 4)) ||
-      (s = get((void**)&lbase_d, lbase.size() * 4)) ||
-      (s = get((void**)&pieces_d, std::max<size_t>(pieces.size(), 1) * 4)))
+      (s = get((void**)&seed_row_d, std::max<uint64_t>(n, 1) * 4)) ||
+      (s = get((void**)&cols_d, R * 4)) || (s = get((void**)&from_d, R * 4)) ||
+      (s = get((void**)&iters_d, R * 4)) || (s = get((void**)&active_d, R)))
     return s;
   WG_HIP(ctx, hipMemcpyAsync(Pd, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
   if (n) WG_HIP(ctx, hipMemcpyAsync(seed_row_d, seed_row.data(), n * 4, hipMemcpyHostToDevice, st));
-  if (nlong) WG_HIP(ctx, hipMemcpyAsync(lrows_d, lrows.data(), nlong * 4, hipMemcpyHostToDevice, st));
-  WG_HIP(ctx, hipMemcpyAsync(lbase_d, lbase.data(), lbase.size() * 4, hipMemcpyHostToDevice, st));
-  if (npieces) WG_HIP(ctx, hipMemcpyAsync(pieces_d, pieces.data(), pieces.size() * 4, hipMemcpyHostToDevice, st));
   WG_HIP(ctx, hipMemsetAsync(iters_d, 0, R * 4, st));
   WG_HIP(ctx, hipMemsetAsync(active_d, 1, R, st));
   if ((s = clear_error(ctx))) return s;
   // out = PlusFirst(A) . 1 (the values do not matter to (x) first).
   if ((s = launch_out_weights(ctx, g, weighted, out, out))) return s;
-  ppr_start<<<kPprGrid, 256, 0, st>>>(n, R, start, tomb, out, X, M0, part, ctx->d_err);
-  std::vector<uint32_t> all(R);
-  for (uint32_t c = 0; c < R; ++c) all[c] = c;
-  WG_HIP(ctx, hipMemcpyAsync(cols_d, all.data(), R * 4, hipMemcpyHostToDevice, st));
-  // The first dangling ranks: ppr_finish over the start partials, leaving
-  // iterations and activity as they were.
-  ppr_finish<<<1, 256, 0, st>>>(part, kPprGrid, R, R, cols_d, dangling, delta, iters_d, active_d, 0.0, 0);
-  WG_HIP(ctx, hipMemsetAsync(iters_d, 0, R * 4, st));
+  // Slots other than ppr_start's hold zeros for the set-up sums.
+  WG_HIP(ctx, hipMemsetAsync(part, 0, (size_t)ntiles * nparts * 2 * kTw * 8, st));
+  ppr_start<<<grid, 256, 0, st>>>(n, R, start, tomb, g->hot_index, nm, out, X, M0, part, nparts, ctx->d_err);
+  std::vector<uint32_t> active_cols(R);
+  for (uint32_t c = 0; c < R; ++c) active_cols[c] = c;
+  WG_HIP(ctx, hipMemcpyAsync(cols_d, active_cols.data(), R * 4, hipMemcpyHostToDevice, st));
+  // The first dangling ranks, from the start partials.
+  ppr_finish<<<R, 256, 0, st>>>(part, nparts, cols_d, dangling, iters_d, active_d, 0.0, 0, 0);
   if ((s = collect_error(ctx, "pagerank batch"))) return s;
   const double* wv = weighted && rev.weights ? rev.weights : nullptr;
   const uint32_t max_iter = bound_err ? 1 : o->max_iter;
   std::vector<uint8_t> act(R, 1);
-  std::vector<uint32_t> active_cols = all;
   double *Min = M0, *Mout = M1;
   for (uint32_t it = 0; it < max_iter && !active_cols.empty(); ++it) {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
     const uint32_t nact = (uint32_t)active_cols.size();
-    WG_HIP(ctx, hipMemcpyAsync(cols_d, active_cols.data(), nact * 4, hipMemcpyHostToDevice, st));
-    PprArgs a{rev.offsets, rev.targets, wv, n, R, nact, cols_d, out, tomb, seed_row_d, Pd,
-              o->damping, teleport, dangling, X, Min, Mout, part, piece, ctx->d_err};
-    if (wv)
-      ppr_iterate<true><<<kPprGrid, 256, 0, st>>>(a, npieces);
-    else
-      ppr_iterate<false><<<kPprGrid, 256, 0, st>>>(a, npieces);
-    if (npieces) {
-      if (wv)
-        ppr_piece<true><<<npieces, 256, 0, st>>>(a, pieces_d);
-      else
-        ppr_piece<false><<<npieces, 256, 0, st>>>(a, pieces_d);
-      ppr_long<<<nlong, 256, 0, st>>>(a, lrows_d, lbase_d, nlong, kPprGrid);
+    const uint32_t tiles = (nact + kTw - 1) / kTw;
+    PprArgs a{rev.offsets, g->hot_src, wv, n, g->hot_index, nm, R, nact, cols_d, out, tomb, seed_row_d, Pd,
+              o->damping, teleport, dangling, X, Min, Mout, part, piece, nparts, ctx->d_err};
+    for (uint32_t t = 0; t < tiles; ++t) {
+      if (wv) ppr_block<true><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
+      else ppr_block<false><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
+      if (long_blocks)
+        ppr_long<<<long_blocks, 256, 0, st>>>(a, t, g->ppr_lrows, g->ppr_lbase, nlong, npieces, grid);
     }
-    ppr_finish<<<1, 256, 0, st>>>(part, nparts, R, nact, cols_d, dangling, delta, iters_d, active_d, bound,
-                                  o->has_tol && !bound_err);
+    ppr_finish<<<nact, 256, 0, st>>>(part, nparts, cols_d, dangling, iters_d, active_d, bound,
+                                     o->has_tol && !bound_err, 1);
     WG_HIP(ctx, hipGetLastError());
     std::swap(Min, Mout);
     WG_HIP(ctx, hipMemcpyAsync(act.data(), active_d, R, hipMemcpyDeviceToHost, st));
     if ((s = collect_error(ctx, "pagerank batch"))) return s;
-    active_cols.clear();
-    for (uint32_t c = 0; c < R; ++c)
-      if (act[c]) active_cols.push_back(c);
+    // Survivors keep their relative order; the messages are repacked.
+    std::vector<uint32_t> next_cols, from;
+    for (uint32_t p = 0; p < nact; ++p)
+      if (act[active_cols[p]]) {
+        next_cols.push_back(active_cols[p]);
+        from.push_back(p);
+      }
+    if (next_cols.size() != nact && !next_cols.empty()) {
+      WG_HIP(ctx, hipMemcpyAsync(from_d, from.data(), from.size() * 4, hipMemcpyHostToDevice, st));
+      ppr_repack<<<2048, 256, 0, st>>>(nm, Min, Mout, from_d, (uint32_t)next_cols.size());
+      std::swap(Min, Mout);
+      WG_HIP(ctx, hipMemcpyAsync(cols_d, next_cols.data(), next_cols.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    active_cols.swap(next_cols);
   }
   if (iterations) WG_HIP(ctx, hipMemcpy(iterations, iters_d, R * 4, hipMemcpyDeviceToHost));
   if (bound_err) return fail(ctx, (int)bound_err, "pagerank: live nodes * tol out of range");

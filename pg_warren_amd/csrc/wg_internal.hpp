@@ -100,7 +100,33 @@ struct wg_graph {
   // (charged to the context) until the graph is destroyed.
   std::vector<std::pair<void*, size_t>> workspace;
   void* bfs_ws[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Other kernels' per-graph buffers by slot: (pointer, bytes).
+  std::map<int, std::pair<void*, size_t>> cached;
+  // Hot-first numbering of message sources (built on first use): nodes with
+  // out-edges ranked by descending out-degree (ties by id), hot_index[v]
+  // their rank (~0 for nodes without out-edges), and the reverse CSR's
+  // sources in that numbering. The most gathered messages then share the
+  // fewest cache lines.
+  uint32_t* hot_index = nullptr;
+  uint32_t* hot_src = nullptr;
+  uint64_t nhot = 0;
+  // The batched PageRank's work items over the reverse CSR (built on first
+  // use): runs of whole rows and pieces of long rows, the long rows and
+  // their first piece slots.
+  void* ppr_items = nullptr;
+  uint32_t* ppr_lrows = nullptr;
+  uint32_t* ppr_lbase = nullptr;
+  uint32_t ppr_nitems = 0, ppr_npieces = 0, ppr_nlong = 0;
 };
+
+namespace wg {
+// A device buffer of at least `bytes` kept with the graph under `slot`
+// (charged to the context, released with the graph).
+int graph_buffer(wg_context* ctx, wg_graph* g, int slot, size_t bytes, void** out);
+constexpr int kSlotHotIndex = 190, kSlotHotSrc = 191;
+constexpr int kSlotPprItems = 192, kSlotPprLrows = 193, kSlotPprLbase = 194;
+int prepare_hot_order(wg_context* ctx, wg_graph* g);
+}  // namespace wg
 
 namespace wg {
 
