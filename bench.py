@@ -182,6 +182,95 @@ def bench_msbfs(W, ctx, g, n, deg, ns=512):
             "reach_farness_mismatches_vs_single_source": mism}
 
 
+def bench_adjoint(W, ctx, scale=20, iters=200, fd_edges=200):
+    """Differentiable PageRank on R-MAT `scale` with log-normal(0, 1) weights,
+    row-normalised: personalised PageRank from 32 seeds, loss = sum of
+    squared ranks over 1,000 targets, dL/dw for every edge by the adjoint,
+    checked by central differences (h = 1e-6, f64) on fd_edges edges (half
+    the largest gradients, half uniform) with the weights moved in place.
+    Forward and adjoint both run `iters` iterations (no tolerance)."""
+    off, tgt = W.rmat(scale, 16, seed=4)
+    n, m = len(off) - 1, len(tgt)
+    deg = np.diff(off.astype(np.int64))
+    w = W.edge_weights(off, tgt, "lognormal", seed=4, p0=0.0, p1=1.0)
+    rows = np.repeat(np.arange(n), deg)
+    w = w / np.bincount(rows, weights=w, minlength=n)[rows]
+    g = W.Graph(ctx, n, off, tgt, w)
+    rng = np.random.default_rng(20)
+    seed = np.zeros(n)
+    seed[rng.choice(np.flatnonzero(deg > 0), 32, replace=False)] = 1.0
+    sv = W.Vector(ctx, seed)
+    targets = rng.choice(n, 1000, replace=False)
+    out = W.Vector(ctx, dtype="f64", id_space=n)
+
+    def forward():
+        W.pagerank(ctx, g, tol=None, max_iter=iters, seed=sv, out=out)
+        xs = out.numpy()
+        return float(np.sum(xs[targets] ** 2)), xs
+
+    forward()  # warm
+    t1 = time.perf_counter()
+    _, x = forward()
+    t_fwd = time.perf_counter() - t1
+    gx = np.zeros(n)
+    gx[targets] = 2.0 * x[targets]
+    xv, gv = W.Vector(ctx, x), W.Vector(ctx, gx)
+    dwv = W.Vector(ctx, dtype="f64", id_space=m)
+    W.pagerank_backward(ctx, g, xv, gv, tol=None, max_iter=iters, seed=sv, out=dwv)  # warm
+    t1 = time.perf_counter()
+    _, bit = W.pagerank_backward(ctx, g, xv, gv, tol=None, max_iter=iters, seed=sv, out=dwv)
+    t_bwd = time.perf_counter() - t1
+    dw = dwv.numpy()
+    scale_g = float(np.abs(dw).max())
+    edges = np.concatenate([np.argsort(-np.abs(dw))[:fd_edges // 2],
+                            rng.choice(m, fd_edges - fd_edges // 2, replace=False)])
+    h = 1e-6
+    fd = np.zeros(len(edges))
+    t1 = time.perf_counter()
+    for i, e in enumerate(edges):
+        w0 = float(w[e])
+        g.update_weights([e], [w0 + h])
+        lp, _ = forward()
+        g.update_weights([e], [w0 - h])
+        lm, _ = forward()
+        g.update_weights([e], [w0])
+        fd[i] = (lp - lm) / (2 * h)
+    t_fd = time.perf_counter() - t1
+    err = np.abs(fd - dw[edges])
+    top = slice(0, fd_edges // 2)
+    rel_top = err[top] / np.abs(dw[edges][top])
+    res = {"scale": scale, "n": n, "m": m, "iterations": iters,
+           "ms_forward": round(t_fwd * 1e3, 2), "ms_backward": round(t_bwd * 1e3, 2),
+           "backward_iterations": bit,
+           "fd_edges": len(edges), "fd_seconds": round(t_fd, 1),
+           "fd_max_abs_err_over_max_grad": float(err.max() / scale_g),
+           "fd_median_rel_err_top_half": float(np.median(rel_top)),
+           "max_abs_grad": scale_g}
+    log(f"adjoint s{scale}: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms; "
+        f"FD on {len(edges)} edges: max err / max grad {res['fd_max_abs_err_over_max_grad']:.2e}")
+    state = {"off": off, "tgt": tgt, "w": w, "seed": seed, "x": x, "gx": gx, "dw": dw,
+             "iters": iters}
+    for v in (sv, out, xv, gv, dwv):
+        v.close()
+    g.close()
+    return res, state
+
+
+def adjoint_oracle_errors(st, threads):
+    """The adjoint leg against the oracle's FP64 forward and adjoint."""
+    import oracle as O
+
+    csr = O.Csr(len(st["off"]) - 1, st["off"], st["tgt"], st["w"])
+    t1 = time.perf_counter()
+    x, _ = O.pagerank(csr, tol=None, max_iter=st["iters"], seed=st["seed"], threads=threads)
+    dw, _, _ = O.pagerank_backward(csr, st["x"], st["gx"], tol=None, max_iter=st["iters"],
+                                   seed=st["seed"], threads=threads)
+    return {"ranks_l1_vs_oracle": float(np.abs(x - st["x"]).sum()),
+            "grad_max_abs_err_over_max_grad": float(np.abs(dw - st["dw"]).max() /
+                                                    np.abs(dw).max()),
+            "oracle_seconds": round(time.perf_counter() - t1, 1), "threads": threads}
+
+
 def oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs, bfs_depths, pr_gpu, pr_iters):
     """Each measured result against the oracle on the same inputs: the f32
     product's largest relative error against the f64 product, mismatches of
@@ -220,6 +309,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true",
                     help="leave out the batched PageRank and multi-source BFS legs")
+    ap.add_argument("--skip-adjoint", action="store_true",
+                    help="leave out the PageRank adjoint leg (R-MAT two scales below)")
     ap.add_argument("--only-spmv", action="store_true",
                     help="profiling runs: the three SpMV legs and the copy kernel only")
     args = ap.parse_args()
@@ -346,6 +437,9 @@ def main():
     if not args.skip_batched:
         legs["ppr_batched_256"] = bench_batched_ppr(W, ctx, gf, n, deg)
         legs["msbfs_512"] = bench_msbfs(W, ctx, gf, n, deg)
+    adj_state = None
+    if not args.skip_adjoint:
+        legs["pagerank_adjoint"], adj_state = bench_adjoint(W, ctx, scale=max(args.scale - 2, 8))
 
     head = legs["plus_times_f32"]
     tuned = head.get("tuned_variant", 0)
@@ -380,6 +474,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(off, tgt, wf, xf)
         result["errors_vs_oracle"] = oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs,
                                                    bfs_depths, pr_gpu, iters)
+        if adj_state is not None:
+            result["errors_vs_oracle"]["pagerank_adjoint"] = adjoint_oracle_errors(adj_state, 16)
         log(f"errors vs oracle: {result['errors_vs_oracle']}")
     print(json.dumps(result), flush=True)
 

@@ -105,6 +105,14 @@ int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted);
  * weights may be NULL). */
 int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* offsets,
                           uint32_t* targets, double* weights);
+/* Sets the weights of `count` edges, named by their index in the forward CSR
+ * (the order of the uploaded targets), in both orientations, without a
+ * rebuild; weights cast to other element types are cast again on next use.
+ * No counterpart in the reference (a warren-core graph is immutable, weights
+ * change by a rebuild); it serves the finite-difference checks of the
+ * PageRank adjoint. Host arrays; the graph must carry weights. */
+int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges,
+                            const double* values, uint64_t count);
 
 /* ---- vectors: AnyVector (compute/vector.rs:9-50), row-major id_space x
  * width, resident in device memory and charged to the context's budget --- */
@@ -177,6 +185,22 @@ int wg_pagerank_batch(wg_context* ctx, const wg_graph* g, const wg_vector* tombs
                       const wg_pagerank_opts* opts, const uint64_t* seed_offsets,
                       const uint64_t* seed_nodes, const double* seed_weights, uint32_t R,
                       wg_vector* ranks, uint32_t* iterations);
+
+/* Adjoint of wg_pagerank with respect to the edge weights: given the ranks x
+ * of a weighted run (same graph, tombstones, options and seed) and
+ * dl_dranks = dL/dx for a loss L, writes dl_dweights = dL/dw, one f64 per
+ * edge in forward-CSR order (a vector with id space nnz). Solves the adjoint
+ * fixed point lambda = d (P lambda) + d (p . lambda) [dangling] + dL/dx by
+ * pulls over the forward CSR (the forward run pulls over the reverse), with
+ * the forward's stop rule (L1 change < live nodes * tol, or max_iter), then
+ * dL/dw_ab = (d x_a / s_a) (lambda_b - (P lambda)_a) for s_a > 0 (s = the
+ * out-weight; 0 elsewhere), tombstoned rows taking no part. adjoint: an f64
+ * vector over the graph that receives lambda, or NULL. No counterpart in the
+ * reference (whose PageRank has no backward pass). */
+int wg_pagerank_backward(wg_context* ctx, const wg_graph* g, const wg_vector* tombstones,
+                         const wg_pagerank_opts* opts, const wg_vector* seed, const wg_vector* ranks,
+                         const wg_vector* dl_dranks, wg_vector* dl_dweights, wg_vector* adjoint,
+                         uint32_t* iterations);
 
 /* ---- traversal: Visit / TraversalStats (traversal/bfs.rs:15-23,
  * traversal/stats.rs:12-20) ------------------------------------------------ */

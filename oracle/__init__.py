@@ -81,6 +81,8 @@ def lib():
                               C.c_int, C.c_int, C.c_uint64, P, P, C.c_int]
         L.or_pagerank.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int, C.c_uint32, P, P, P,
                                   C.c_int]
+        L.or_pagerank_backward.argtypes = [P, P, C.c_double, C.c_double, C.c_int, C.c_uint32, P,
+                                           P, P, P, P, P, C.c_int]
         L.or_bfs.argtypes = [P, P, P, C.c_uint64, P, C.c_int64, C.c_int64, P, P, P]
         L.or_bfs_depths.argtypes = [P, P, C.c_uint64, P, C.c_int]
         L.or_apply.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
@@ -177,6 +179,24 @@ def pagerank(fwd, rev=None, *, damping=0.85, tol=1e-6, max_iter=100, seed=None,
                              damping, 0.0 if tol is None else tol, int(tol is not None),
                              max_iter, _p(sd), _p(out), _p(it), threads))
     return out, int(it[0])
+
+
+def pagerank_backward(fwd, x, g, *, damping=0.85, tol=1e-6, max_iter=100, seed=None,
+                      tombstones=None, threads=0):
+    """Adjoint of pagerank over the weighted fwd CSR: (dL/dw per forward
+    entry, the adjoint vector, iterations) from the ranks x and g = dL/dx."""
+    n = fwd.n
+    lam = np.zeros(n, np.float64)
+    dw = np.zeros(max(fwd.nnz, 1), np.float64)
+    it = np.zeros(1, np.uint32)
+    tomb = None if tombstones is None else np.ascontiguousarray(tombstones, np.uint8)
+    sd = None if seed is None else np.ascontiguousarray(seed, np.float64)
+    xs = np.ascontiguousarray(x, np.float64)
+    gs = np.ascontiguousarray(g, np.float64)
+    _check(lib().or_pagerank_backward(fwd.ref(), _p(tomb), damping, 0.0 if tol is None else tol,
+                                      int(tol is not None), max_iter, _p(sd), _p(xs), _p(gs),
+                                      _p(lam), _p(dw), _p(it), threads))
+    return dw[:fwd.nnz], lam, int(it[0])
 
 
 def bfs(adj, starts, *, pull_rev=None, node_filter=None, max_depth=None, max_results=None):

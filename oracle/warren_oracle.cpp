@@ -761,3 +761,77 @@ double or_uniform(uint64_t seed, uint64_t stream, uint64_t node) {
 }
 
 }  // extern "C"
+
+// Adjoint of PageRank with respect to the edge weights (no reference
+// counterpart; the forward is compute/pagerank/mod.rs:96-287). With s = the
+// out-weight of each row, p = the personalisation and lt = lambda at live
+// rows (0 at tombstoned ones), iterate from lambda = g:
+//   lambda_i = d * (s_i > 0 ? (A lt)_i / s_i : p . lt) + g_i
+// until the L1 change < live * tol (or max_iter), then for edge a -> b with
+// s_a > 0: dL/dw_ab = (d x_a / s_a) (lt_b - (A lt)_a / s_a), else 0. All
+// sums in ascending order, f64.
+int or_pagerank_backward(const or_csr* fwd, const uint8_t* tomb, double damping, double tol,
+                         int has_tol, uint32_t max_iter, const double* seed, const double* x,
+                         const double* g, double* lambda, double* dldw, uint32_t* iterations,
+                         int threads) {
+  const uint64_t n = fwd->n;
+  if (!fwd->weights) return OR_INVALID_ARGUMENT;
+  auto dead = [&](uint64_t i) { return tomb && tomb[i]; };
+  uint64_t live = 0;
+  for (uint64_t i = 0; i < n; ++i) live += !dead(i);
+  const double nodes = (double)live;
+  Err e;
+  const double start = sqlf(DIV_, 1.0, nodes, e);
+  if (e.code) return e.code;
+  std::vector<double> ones(n, 1.0), out(n, 0.0), p(n, 0.0), lt(n, 0.0), u(n, 0.0);
+  ProductArgs pf{PLUS, FIRST, 1, nullptr, 0, 0, 0, -1, 1};
+  int c = pull<double>(fwd, pf, ones.data(), out.data(), threads);
+  if (c) return c;
+  if (seed) {
+    double sum = 0.0;
+    for (uint64_t i = 0; i < n; ++i)
+      if (!dead(i)) sum += seed[i];
+    const double total = sqlf(DIV_, 1.0, sum, e);
+    if (e.code) return e.code;
+    for (uint64_t i = 0; i < n; ++i)
+      if (!dead(i)) p[i] = seed[i] * total;
+  } else {
+    for (uint64_t i = 0; i < n; ++i) p[i] = dead(i) ? 0.0 : start;
+  }
+  const double bound = has_tol ? nodes * tol : 0.0;
+  double pi = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    lambda[i] = g[i];
+    lt[i] = dead(i) ? 0.0 : g[i];
+    pi += p[i] * lt[i];
+  }
+  ProductArgs pt{PLUS, TIMES, 1, nullptr, 0, 0, 0, -1, 1};
+  uint32_t it = 0;
+  while (it < max_iter) {
+    ++it;
+    c = pull<double>(fwd, pt, lt.data(), u.data(), threads);
+    if (c) return c;
+    double change = 0.0, pin = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+      const double z = sql_cmp_f(out[i], 0.0) > 0 ? u[i] / out[i] : pi;
+      const double l = damping * z + g[i];
+      change += std::fabs(l - lambda[i]);
+      lambda[i] = l;
+      lt[i] = dead(i) ? 0.0 : l;
+      pin += p[i] * lt[i];
+    }
+    pi = pin;
+    if (has_tol && sql_cmp_f(change, bound) < 0) break;
+  }
+  c = pull<double>(fwd, pt, lt.data(), u.data(), threads);
+  if (c) return c;
+  for (uint64_t a = 0; a < n; ++a) {
+    const bool flows = sql_cmp_f(out[a], 0.0) > 0;
+    const double coef = flows ? damping * x[a] / out[a] : 0.0;
+    const double pa = flows ? u[a] / out[a] : 0.0;
+    for (uint64_t k = fwd->offsets[a]; k < fwd->offsets[a + 1]; ++k)
+      dldw[k] = flows ? coef * (lt[fwd->targets[k]] - pa) : 0.0;
+  }
+  if (iterations) *iterations = it;
+  return OR_OK;
+}

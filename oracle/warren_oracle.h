@@ -86,6 +86,14 @@ int or_pagerank(const or_csr* fwd, const or_csr* rev, const uint8_t* tombstones,
                 double damping, double tol, int has_tol, uint32_t max_iter,
                 const double* seed, double* out, uint32_t* iterations, int threads);
 
+/* Adjoint of or_pagerank with respect to the edge weights of fwd (which must
+ * carry weights): from the forward ranks x and g = dL/dx, the adjoint
+ * lambda (n) and dL/dw (one per forward entry). */
+int or_pagerank_backward(const or_csr* fwd, const uint8_t* tombstones, double damping,
+                         double tol, int has_tol, uint32_t max_iter, const double* seed,
+                         const double* x, const double* g, double* lambda, double* dldw,
+                         uint32_t* iterations, int threads);
+
 /* A BFS visit (traversal/bfs.rs:15-23); via = UINT64_MAX for a start. */
 typedef struct {
   uint64_t node;

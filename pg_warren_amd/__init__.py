@@ -192,6 +192,14 @@ class Graph:
                                                           _p(w)))
         return off, tgt[:self.nnz], None if w is None else w[:self.nnz]
 
+    def update_weights(self, edges, values):
+        """Sets the weights of the given edges (forward-CSR indices) in place."""
+        e = np.ascontiguousarray(edges, np.uint64)
+        v = np.ascontiguousarray(values, np.float64)
+        if e.shape != v.shape:
+            raise ValueError("edges and values differ in length")
+        self.ctx.check(self.ctx.lib.wg_graph_update_weights(self.ctx.h, self.h, _p(e), _p(v), len(e)))
+
     def close(self):
         if getattr(self, "h", None) and self.ctx.h is not None:
             self.ctx.lib.wg_graph_destroy(self.h)
@@ -264,6 +272,33 @@ def pagerank(ctx, graph, *, damping=0.85, tol=1e-6, max_iter=100, seed=None, tom
                                   C.byref(opts), None if seed is None else seed.h, out.h,
                                   C.byref(it)))
     return out, it.value
+
+
+def pagerank_backward(ctx, graph, ranks, dl_dranks, *, damping=0.85, tol=1e-6, max_iter=100,
+                      seed=None, tombstones=None, out=None, adjoint=False):
+    """dL/dw for every edge (forward-CSR order) of a weighted pagerank run
+    with these options, from its ranks and dL/dranks. Returns (dL/dw Vector
+    over nnz, iterations) or, with adjoint=True, (dL/dw, iterations, the
+    adjoint Vector)."""
+    if seed is not None and not isinstance(seed, Vector):
+        seed = Vector(ctx, np.asarray(seed))
+    if tombstones is not None and not isinstance(tombstones, Vector):
+        tombstones = Vector(ctx, np.asarray(tombstones, dtype=np.uint8), dtype="bool")
+    if not isinstance(ranks, Vector):
+        ranks = Vector(ctx, np.asarray(ranks, np.float64))
+    if not isinstance(dl_dranks, Vector):
+        dl_dranks = Vector(ctx, np.asarray(dl_dranks, np.float64))
+    if out is None:
+        out = Vector(ctx, dtype="f64", id_space=graph.nnz)
+    adj = Vector(ctx, dtype="f64", id_space=graph.n) if adjoint else None
+    opts = _lib.PagerankOpts(damping, 0.0 if tol is None else tol, int(tol is not None),
+                             max_iter, 1, 0)
+    it = C.c_uint32()
+    ctx.check(ctx.lib.wg_pagerank_backward(ctx.h, graph.h, None if tombstones is None else tombstones.h,
+                                           C.byref(opts), None if seed is None else seed.h, ranks.h,
+                                           dl_dranks.h, out.h, None if adj is None else adj.h,
+                                           C.byref(it)))
+    return (out, it.value, adj) if adjoint else (out, it.value)
 
 
 def pagerank_batch(ctx, graph, seed_sets, *, seed_weights=None, damping=0.85, tol=1e-6,

@@ -61,6 +61,7 @@ SIGNATURES = [
     ("wg_graph_destroy", None, [P]),
     ("wg_graph_info", I32, [P, P, P, P]),
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
+    ("wg_graph_update_weights", I32, [P, P, P, P, U64]),
     ("wg_vector_create", I32, [P, I32, U64, U64, P, P]),
     ("wg_vector_destroy", None, [P]),
     ("wg_vector_write", I32, [P, P, P]),
@@ -73,6 +74,7 @@ SIGNATURES = [
     ("wg_spmv_variant", I32, [P, I32, I32, I32, I32]),
     ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
     ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
+    ("wg_pagerank_backward", I32, [P, P, P, P, P, P, P, P, P, P]),
     ("wg_bfs_levels", I32, [P, P, I32, P, U64, P, I64, I32, P, P, P]),
     ("wg_bfs", I32, [P, P, I32, P, U64, P, I64, I64, P, P, P]),
     ("wg_reachable_set", I32, [P, P, I32, P, U64, I64, P, P, P]),

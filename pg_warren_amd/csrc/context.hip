@@ -317,6 +317,17 @@ int prepare_spmm(wg_context* ctx, DevCsr& c) {
   return WG_OK;
 }
 
+__global__ __launch_bounds__(256) void set_weights(const uint64_t* __restrict__ edges,
+                                                   const double* __restrict__ values,
+                                                   const uint64_t* __restrict__ rev_pos, uint64_t count,
+                                                   double* __restrict__ fwd_w, double* __restrict__ rev_w) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t e = edges[i];
+    fwd_w[e] = values[i];
+    rev_w[rev_pos[e]] = values[i];
+  }
+}
+
 __global__ __launch_bounds__(256) void renumber_sources(const uint32_t* __restrict__ src,
                                                         const uint32_t* __restrict__ index,
                                                         uint32_t* __restrict__ out, uint64_t nnz) {
@@ -567,6 +578,59 @@ int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* off, uin
     if (w && g->rev.weights)
       WG_HIP(ctx, hipMemcpy(w, g->rev.weights, g->nnz * 8, hipMemcpyDeviceToHost));
   }
+  return ok(ctx);
+}
+
+int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges, const double* values,
+                            uint64_t count) {
+  if (!ctx || !g || (count && (!edges || !values))) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (!g->fwd.weights) return fail(ctx, WG_INVALID_ARGUMENT, "the graph has no edge weights");
+  for (uint64_t i = 0; i < count; ++i)
+    if (edges[i] >= g->nnz) return fail(ctx, WG_INVALID_ARGUMENT, "edge index outside the graph");
+  if (count == 0) return ok(ctx);
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const uint64_t m = g->nnz;
+  void* rp;
+  int s;
+  const bool built = g->cached.count(kSlotRevPos) != 0;
+  if ((s = graph_buffer(ctx, g, kSlotRevPos, m * 8, &rp))) return s;
+  if (!built) {
+    // Each forward entry's place in the reverse CSR, as the transpose at
+    // upload placed it (sources scanned in order).
+    std::vector<uint32_t> tgt(m);
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    WG_HIP(ctx, hipMemcpy(tgt.data(), g->fwd.targets, m * 4, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> next(g->rev.h_offsets.begin(), g->rev.h_offsets.end() - 1), pos(m);
+    for (uint64_t p = 0; p < m; ++p) pos[p] = next[tgt[p]]++;
+    WG_HIP(ctx, hipMemcpy(rp, pos.data(), m * 8, hipMemcpyHostToDevice));
+  }
+  void *de, *dv;
+  if ((s = dalloc(ctx, &de, count * 8))) return s;
+  if ((s = dalloc(ctx, &dv, count * 8))) {
+    dfree(ctx, de, count * 8);
+    return s;
+  }
+  hipError_t e = hipMemcpyAsync(de, edges, count * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, values, count * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    const unsigned grid = (unsigned)std::min<uint64_t>((count + 255) / 256, 4096);
+    set_weights<<<grid, 256, 0, st>>>((const uint64_t*)de, (const double*)dv, (const uint64_t*)rp, count,
+                                      g->fwd.weights, g->rev.weights);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dfree(ctx, de, count * 8);
+  dfree(ctx, dv, count * 8);
+  if (e != hipSuccess) return hip_fail(ctx, e, "update weights");
+  // Weights cast to other element types are cast again on next use.
+  for (DevCsr* c : {&g->fwd, &g->rev})
+    for (int d = 0; d < 5; ++d) {
+      if (d == WG_F64) continue;
+      if (c->wview[d]) dfree(ctx, c->wview[d], c->nnz * dtype_size(d));
+      c->wview[d] = nullptr;
+      c->wstatus[d] = -1;
+    }
   return ok(ctx);
 }
 

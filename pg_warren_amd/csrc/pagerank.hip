@@ -5,10 +5,11 @@
 //   advance  x_i' = d (y_i + dangling p_i) + (1 - d) p_i, SQL float8 checks
 //   spread   msg_i' = x_i' / out_i, or 0 and x_i' joins the next dangling sum
 // Block partials of the L1 change and of the next dangling rank are summed
-// in a fixed order by the last block to finish, which also takes the stop
-// decision (L1 change < live nodes * tol, in SQL order) into a device flag;
-// later launches of the batch see the flag and return at once, so the host
-// enqueues iterations in batches and reads the flag once per batch.
+// in a fixed order by a one-workgroup pr_finish launch, which also takes the
+// stop decision (L1 change < live nodes * tol, in SQL order) into a device
+// flag; later launches of the batch see the flag and return at once, so the
+// host enqueues iterations in batches and reads the flag once per batch.
+// The file ends with the adjoint (wg_pagerank_backward).
 #include <algorithm>
 #include <cmath>
 
@@ -451,5 +452,232 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
   }
   if (iterations) *iterations = rev.nblocks ? hs.iterations : max_iter;
   if (bound_err) return fail(ctx, (int)bound_err, "pagerank: live nodes * tol out of range");
+  return ok(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of PageRank. The forward fixed point is, at live rows j,
+//   x_j = d (sum_i w_ij x_i / s_i [s_i > 0] + delta p_j) + (1 - d) p_j,
+//   delta = sum_{i: s_i <= 0} x_i,  s_i = sum_j w_ij (the out-weight),
+// and x_j = 0 at tombstoned rows (compute/pagerank/mod.rs:96-287). For a
+// loss L(x) with gradient g = dL/dx, the adjoint lambda solves
+//   lambda_i = d (sum_j w_ij lt_j / s_i) + g_i      (s_i > 0)
+//   lambda_i = d (sum_j p_j lt_j) + g_i             (s_i <= 0)
+// with lt = lambda at live rows and 0 at tombstoned ones: a pull over the
+// forward CSR (the forward pass pulls over the reverse), iterated from
+// lambda = g with the forward's stop rule (L1 change below live * tol).
+// Then, for an edge a -> b with s_a > 0,
+//   dL/dw_ab = (d x_a / s_a) (lt_b - sum_j w_aj lt_j / s_a),
+// and 0 where s_a <= 0 (the dangling indicator has no derivative).
+namespace wg {
+
+// p_i: the seed share, or start at live rows without a seed.
+__device__ __forceinline__ double adj_share(const double* p, double start, const uint8_t* tomb, uint64_t i) {
+  if (tomb && tomb[i]) return 0.0;
+  return p ? p[i] : start;
+}
+
+__global__ __launch_bounds__(256) void adj_start(const double* __restrict__ g, const uint8_t* __restrict__ tomb,
+                                                 const double* __restrict__ p, double start, uint64_t n,
+                                                 double* __restrict__ lambda, double* __restrict__ lt,
+                                                 double* __restrict__ part) {
+  __shared__ double wsum[4];
+  double pi = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double l = g[i];
+    const double t = (tomb && tomb[i]) ? 0.0 : l;
+    lambda[i] = l;
+    lt[i] = t;
+    pi += adj_share(p, start, tomb, i) * t;
+  }
+  const double s = block_sum(pi, wsum);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = 0.0;
+    part[gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// One adjoint step from u = A lt and pi = sum p lt of the previous step.
+__global__ __launch_bounds__(256) void adj_advance(const double* __restrict__ u, const double* __restrict__ out,
+                                                   const double* __restrict__ g, const uint8_t* __restrict__ tomb,
+                                                   const double* __restrict__ p, double start, uint64_t n,
+                                                   double damping, const double* __restrict__ scal,
+                                                   double* __restrict__ lambda, double* __restrict__ lt,
+                                                   double* __restrict__ part) {
+  __shared__ double wsum[8];
+  const double pi = scal[0];
+  double change = 0.0, pin = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double s = out[i];
+    const double z = sql_lt(0.0, s) ? u[i] / s : pi;
+    const double l = damping * z + g[i];
+    change += fabs(l - lambda[i]);
+    lambda[i] = l;
+    const double t = (tomb && tomb[i]) ? 0.0 : l;
+    lt[i] = t;
+    pin += adj_share(p, start, tomb, i) * t;
+  }
+  const double c = block_sum(change, wsum);
+  const double q = block_sum(pin, wsum + 4);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = c;
+    part[gridDim.x + blockIdx.x] = q;
+  }
+}
+
+// The partials in a fixed order: scal[0] = pi, scal[1] = the L1 change.
+__global__ __launch_bounds__(256) void adj_reduce(const double* __restrict__ part, uint32_t grid,
+                                                  double* __restrict__ scal) {
+  __shared__ double sc[256], sq[256];
+  const uint32_t tid = threadIdx.x;
+  double c = 0.0, q = 0.0;
+  for (uint32_t b = tid; b < grid; b += 256) {
+    c += part[b];
+    q += part[grid + b];
+  }
+  sc[tid] = c;
+  sq[tid] = q;
+  __syncthreads();
+  for (uint32_t d = 128; d > 0; d >>= 1) {
+    if (tid < d) {
+      sc[tid] += sc[tid + d];
+      sq[tid] += sq[tid + d];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    scal[0] = sq[0];
+    scal[1] = sc[0];
+  }
+}
+
+// dL/dw for every edge of the forward CSR, a wavefront a row.
+__global__ __launch_bounds__(256) void adj_gradient(const uint64_t* __restrict__ off, const uint32_t* __restrict__ tgt,
+                                                    uint64_t n, const double* __restrict__ x,
+                                                    const double* __restrict__ out, const double* __restrict__ u,
+                                                    const double* __restrict__ lt, double damping,
+                                                    double* __restrict__ dldw) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * 4;
+  for (uint64_t a = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); a < n; a += waves) {
+    const uint64_t b = off[a], e = off[a + 1];
+    const double s = out[a];
+    const bool flows = sql_lt(0.0, s);
+    const double coef = flows ? damping * x[a] / s : 0.0;
+    const double pa = flows ? u[a] / s : 0.0;
+    for (uint64_t k = b + lane; k < e; k += 64) dldw[k] = flows ? coef * (lt[tgt[k]] - pa) : 0.0;
+  }
+}
+
+int launch_weighted_f64(wg_context* ctx, wg_graph* g, const double* x, double* y);
+
+}  // namespace wg
+
+extern "C" int wg_pagerank_backward(wg_context* ctx, const wg_graph* gc, const wg_vector* tombstones,
+                                    const wg_pagerank_opts* o, const wg_vector* seed, const wg_vector* ranks,
+                                    const wg_vector* dl_dranks, wg_vector* dl_dweights, wg_vector* adjoint,
+                                    uint32_t* iterations) {
+  if (!ctx || !gc || !o || !ranks || !dl_dranks || !dl_dweights)
+    return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  wg_graph* g = const_cast<wg_graph*>(gc);
+  const uint64_t n = g->n, m = g->nnz;
+  if (iterations) *iterations = 0;
+  for (const wg_vector* v : {ranks, dl_dranks, (const wg_vector*)adjoint})
+    if (v && (v->dtype != WG_F64 || v->width != 1 || v->id_space != n))
+      return fail(ctx, WG_INVALID_ARGUMENT, "ranks, their gradient and the adjoint must be f64, width 1, over the graph");
+  if (dl_dweights->dtype != WG_F64 || dl_dweights->width != 1 || dl_dweights->id_space != m)
+    return fail(ctx, WG_INVALID_ARGUMENT, "the weight gradient must be f64, width 1, one row per edge");
+  if (tombstones && (tombstones->dtype != WG_BOOL || tombstones->id_space != n || tombstones->width != 1))
+    return fail(ctx, WG_INVALID_ARGUMENT, "tombstones must be a bool vector over the graph");
+  const bool weighted = o->weighted < 0 ? g->weighted : o->weighted != 0;
+  if (!weighted || !g->fwd.weights)
+    return fail(ctx, WG_INVALID_ARGUMENT, "pagerank backward differentiates the edge weights: the run must be weighted");
+  if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const uint8_t* tomb = tombstones ? (const uint8_t*)tombstones->data : nullptr;
+  PrBuffers buf(ctx, g);
+  buf.next_slot = 170;
+  int s;
+  unsigned long long* live_d;
+  double *scratch, *out, *p = nullptr, *lambda, *lt, *u, *part, *scal;
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, kReduceGrid));
+  if ((s = buf.get(&live_d, 1)) || (s = buf.get(&scratch, std::max<uint64_t>(n, 1))) ||
+      (s = buf.get(&out, std::max<uint64_t>(n, 1))) || (s = buf.get(&lt, std::max<uint64_t>(n, 1))) ||
+      (s = buf.get(&u, std::max<uint64_t>(n, 1))) || (s = buf.get(&part, 2 * kReduceGrid)) ||
+      (s = buf.get(&scal, 2)))
+    return s;
+  if (adjoint) {
+    lambda = (double*)adjoint->data;
+  } else if ((s = buf.get(&lambda, std::max<uint64_t>(n, 1)))) {
+    return s;
+  }
+  WG_HIP(ctx, hipMemsetAsync(live_d, 0, 8, st));
+  if ((s = clear_error(ctx))) return s;
+  // The live count and start, as the forward run takes them.
+  pr_init<<<grid, 256, 0, st>>>(scratch, tomb, n, 0.0, live_d);
+  unsigned long long live = 0;
+  WG_HIP(ctx, hipMemcpyAsync(&live, live_d, 8, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  uint32_t herr = 0;
+  const double nodes = (double)live;
+  const double start = sqlf<DIV_>(1.0, nodes, herr);
+  if (herr) return fail(ctx, (int)herr, "pagerank: 1 / live nodes: division by zero");
+  if ((s = launch_out_weights(ctx, g, true, scratch, out))) return s;
+  if (seed) {
+    if (seed->dtype == WG_BOOL) return fail(ctx, WG_DTYPE_MISMATCH, "dtype mismatch: expected f64, found bool");
+    if (seed->width != 1) return fail(ctx, WG_WIDTH_MISMATCH, "width mismatch: expected " + std::to_string(seed->width) + ", found 1");
+    if (seed->id_space != n) return fail(ctx, WG_ID_SPACE_MISMATCH, "seed id space mismatch");
+    switch (seed->dtype) {
+      case WG_F64: pr_seed_sum<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, part); break;
+      case WG_F32: pr_seed_sum<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, part); break;
+      case WG_I64: pr_seed_sum<int64_t><<<grid, 256, 0, st>>>((const int64_t*)seed->data, tomb, n, part); break;
+      case WG_U32: pr_seed_sum<uint32_t><<<grid, 256, 0, st>>>((const uint32_t*)seed->data, tomb, n, part); break;
+    }
+    std::vector<double> hp(grid);
+    WG_HIP(ctx, hipMemcpyAsync(hp.data(), part, grid * 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    const double total = sqlf<DIV_>(1.0, host_sum(hp), herr);
+    if (herr) return fail(ctx, (int)herr, "pagerank: 1 / sum(seed): division by zero");
+    if ((s = buf.get(&p, std::max<uint64_t>(n, 1)))) return s;
+    switch (seed->dtype) {
+      case WG_F64: pr_shares<double><<<grid, 256, 0, st>>>((const double*)seed->data, tomb, n, total, p, ctx->d_err); break;
+      case WG_F32: pr_shares<float><<<grid, 256, 0, st>>>((const float*)seed->data, tomb, n, total, p, ctx->d_err); break;
+      case WG_I64: pr_shares<int64_t><<<grid, 256, 0, st>>>((const int64_t*)seed->data, tomb, n, total, p, ctx->d_err); break;
+      case WG_U32: pr_shares<uint32_t><<<grid, 256, 0, st>>>((const uint32_t*)seed->data, tomb, n, total, p, ctx->d_err); break;
+    }
+    if ((s = collect_error(ctx, "pagerank seed shares"))) return s;
+  }
+  uint32_t bound_err = 0;
+  const double bound = o->has_tol ? sqlf<MUL_>(nodes, o->tol, bound_err) : 0.0;
+  if (bound_err) return fail(ctx, (int)bound_err, "pagerank: live nodes * tol out of range");
+  const double* g_in = (const double*)dl_dranks->data;
+  adj_start<<<grid, 256, 0, st>>>(g_in, tomb, p, start, n, lambda, lt, part);
+  adj_reduce<<<1, 256, 0, st>>>(part, grid, scal);
+  uint32_t it = 0;
+  double hs[2] = {0.0, 0.0};
+  while (it < o->max_iter) {
+    if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+    ++it;
+    if ((s = launch_weighted_f64(ctx, g, lt, u))) return s;
+    adj_advance<<<grid, 256, 0, st>>>(u, out, g_in, tomb, p, start, n, o->damping, scal, lambda, lt, part);
+    adj_reduce<<<1, 256, 0, st>>>(part, grid, scal);
+    WG_HIP(ctx, hipGetLastError());
+    if (o->has_tol) {
+      WG_HIP(ctx, hipMemcpyAsync(hs, scal, 16, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      if (sql_lt(hs[1], bound)) break;
+    }
+  }
+  // The gradient from the final adjoint.
+  if ((s = launch_weighted_f64(ctx, g, lt, u))) return s;
+  if (m) {
+    const unsigned ggrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, 65536));
+    adj_gradient<<<ggrid, 256, 0, st>>>(g->fwd.offsets, g->fwd.targets, n, (const double*)ranks->data, out, u,
+                                        lt, o->damping, (double*)dl_dweights->data);
+  }
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = collect_error(ctx, "pagerank backward"))) return s;
+  if (iterations) *iterations = it;
   return ok(ctx);
 }

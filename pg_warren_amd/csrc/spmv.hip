@@ -554,6 +554,12 @@ int launch_out_weights(wg_context* ctx, wg_graph* g, bool weighted, const double
   return launch_product<double, PLUS, FIRST>(ctx, g->fwd, weighted, x, out, 1, wa);
 }
 
+// y = A (+.x) x over the forward CSR in f64, with the edge weights.
+int launch_weighted_f64(wg_context* ctx, wg_graph* g, const double* x, double* y) {
+  WriteArgs wa{nullptr, 0, 0, 0, -1};
+  return launch_product<double, PLUS, TIMES>(ctx, g->fwd, true, x, y, 1, wa);
+}
+
 // The checks of spmv / spmm / multiply (compute/product/mod.rs:77-153).
 static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
                    wg_vector* y, bool spmv) {

@@ -125,6 +125,7 @@ namespace wg {
 int graph_buffer(wg_context* ctx, wg_graph* g, int slot, size_t bytes, void** out);
 constexpr int kSlotHotIndex = 190, kSlotHotSrc = 191;
 constexpr int kSlotPprItems = 192, kSlotPprLrows = 193, kSlotPprLbase = 194;
+constexpr int kSlotRevPos = 195;  // forward entry -> reverse position
 int prepare_hot_order(wg_context* ctx, wg_graph* g);
 }  // namespace wg
 
