@@ -39,7 +39,11 @@ enum wg_status {
   WG_NODE_FILTER_ID_SPACE = 13,  /* TraversalError::NodeFilterIdSpace */
   WG_INVALID_CSR = 14,         /* GraphError from Csr::from_parts (graph/csr/mod.rs:78-137) */
   WG_UNSUPPORTED = 15,         /* a dtype / semiring pair with no meaning */
-  WG_DEVICE_ERROR = 16         /* a HIP runtime failure */
+  WG_DEVICE_ERROR = 16,        /* a HIP runtime failure */
+  WG_NOT_SYMMETRIC = 17,       /* SpectralError::NotSymmetric */
+  WG_NON_POSITIVE_DEGREE = 18, /* SpectralError::NonPositiveDegree */
+  WG_EIGENPAIR_COUNT = 19,     /* SpectralError::EigenpairCount */
+  WG_NOT_FINITE = 20           /* SpectralError::NotFinite */
 };
 
 /* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
@@ -261,6 +265,62 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const ui
 int wg_msbfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* sources,
              uint32_t nsources, int64_t max_depth, wg_vector* depths, wg_vector* reach,
              wg_vector* farness, uint32_t* levels);
+
+/* ---- eigenpairs: eigen (spectral/lanczos.rs:69-224) ----------------------
+ * The matrix (spectral/operator.rs:33-45): S = (A + A^T)/2 with symmetrize
+ * (one product when the reverse CSR equals the forward one), else A, which
+ * must be symmetric (Freivalds check, 1e-9); D = the row sums of S; over the
+ * active nodes (those with an edge of S). */
+enum wg_matrix {
+  WG_ADJACENCY = 0,            /* S */
+  WG_LAPLACIAN = 1,            /* D - S */
+  WG_NORMALIZED_LAPLACIAN = 2, /* I - D^-1/2 S D^-1/2 */
+  WG_RANDOM_WALK = 3           /* D^-1 S, solved through D^-1/2 S D^-1/2 */
+};
+enum wg_which { WG_LARGEST = 0, WG_SMALLEST = 1 };
+
+/* EigenOptions (lanczos.rs:140-156) and GraphMatrix (operator.rs:79-93). */
+typedef struct {
+  int32_t matrix;       /* wg_matrix */
+  int32_t which;        /* wg_which */
+  int32_t weighted;     /* -1: if the graph has weights */
+  int32_t symmetrize;   /* 1: (A + A^T)/2; 0: A, checked symmetric */
+  uint32_t k;           /* eigenpairs */
+  uint32_t basis_size;  /* Krylov basis vectors; 0: max(2k + 1, 20) (working_set.rs:79-88) */
+  double tol;           /* converged when |beta s_last| <= tol * max|theta| (1e-6) */
+  uint64_t max_iter;    /* operator applications; 0: 1000 x basis size, raised to k */
+  int32_t precision;    /* WG_F32 or WG_F64: the basis and eigenvectors */
+  int32_t pad;
+  uint64_t seed;        /* start vectors: random::uniform(seed, stream, node) */
+} wg_eigen_opts;
+
+/* Eigen (lanczos.rs:174-198): steps taken, the connected components of the
+ * active nodes, whether every pair met tol, how many pairs came exactly
+ * from the components (deflation.rs), and the thick restarts made. */
+typedef struct {
+  uint64_t iterations;
+  uint64_t components;
+  uint32_t converged;
+  uint32_t exact;
+  uint32_t restarts;
+  uint32_t basis_size;
+} wg_eigen_info;
+
+/* Thick-restart Lanczos with full reorthogonalisation (lanczos.rs:412-743).
+ * The operator runs in f64 over the device CSR pair; the basis (n x size,
+ * f32 or f64 per precision, every sum in f64) stays in device memory; the
+ * restart rotation is an in-place FP64 block product on the matrix cores;
+ * the projected problem is solved in FP64 on the host. Writes k eigenvalues
+ * in `which` order, their residuals ||Mx - lambda x||_2 for the unit
+ * eigenvectors as stored, and (if vectors is not NULL) the eigenvectors as
+ * the columns of an id space n x width k vector of the precision's dtype
+ * (zero at inactive nodes, the entry of largest magnitude positive).
+ * Errors: WG_EIGENPAIR_COUNT (k = 0 or above the active nodes),
+ * WG_NOT_SYMMETRIC, WG_NON_POSITIVE_DEGREE (normalised operators),
+ * WG_NOT_FINITE, WG_MEMORY_LIMIT (the working set is checked before the
+ * first step), WG_INVALID_ARGUMENT. */
+int wg_eigen(wg_context* ctx, const wg_graph* g, const wg_eigen_opts* opts, double* values,
+             double* residuals, wg_vector* vectors, wg_eigen_info* info);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference: a streaming copy kernel (16-byte loads and

@@ -23,7 +23,8 @@ STATUS_NAMES = {
     7: "division_by_zero", 8: "float_overflow", 9: "float_underflow",
     10: "bigint_out_of_range", 11: "invalid_argument",
     12: "node_outside_id_space", 13: "node_filter_id_space", 14: "invalid_csr",
-    15: "unsupported", 16: "device_error",
+    15: "unsupported", 16: "device_error", 17: "not_symmetric", 18: "non_positive_degree",
+    19: "eigenpair_count", 20: "not_finite",
 }
 DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
 NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
@@ -389,6 +390,35 @@ def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_dep
                                     _p(flt), -1 if max_depth is None else max_depth,
                                     int(pulling), depth.h, via.h, C.byref(stats)))
     return depth, via, _stats(stats)
+
+
+MATRICES = {"adjacency": 0, "laplacian": 1, "normalized_laplacian": 2, "random_walk": 3}
+WHICH = {"largest": 0, "smallest": 1}
+
+
+def eigen(ctx, graph, k, *, matrix="adjacency", which="largest", weighted=None, symmetrize=True,
+          tol=1e-6, max_iter=None, precision="f32", seed=0, basis_size=None, vectors=True):
+    """eigen (spectral/lanczos.rs:206-224): the k eigenpairs of `matrix` at the
+    `which` end by thick-restart Lanczos with full reorthogonalisation.
+    Returns (values, residuals, eigenvectors Vector n x k or None, info dict)."""
+    if matrix not in MATRICES:
+        raise WarrenError(11, f'unknown matrix "{matrix}"')
+    if which not in WHICH:
+        raise WarrenError(11, f'unknown which "{which}"')
+    if precision not in ("f32", "f64"):
+        raise WarrenError(11, f'unknown precision "{precision}"')
+    opts = _lib.EigenOpts(MATRICES[matrix], WHICH[which], -1 if weighted is None else int(weighted),
+                          int(bool(symmetrize)), int(k), int(basis_size or 0), float(tol),
+                          int(max_iter or 0), DTYPES[precision], 0, int(seed))
+    vals = np.zeros(max(int(k), 1), np.float64)
+    res = np.zeros(max(int(k), 1), np.float64)
+    vec = Vector(ctx, dtype=precision, id_space=graph.n, width=int(k)) if vectors and k > 0 else None
+    info = _lib.EigenInfo()
+    ctx.check(ctx.lib.wg_eigen(ctx.h, graph.h, C.byref(opts), _p(vals), _p(res),
+                               None if vec is None else vec.h, C.byref(info)))
+    return vals[:k], res[:k], vec, {"iterations": info.iterations, "components": info.components,
+                                    "converged": bool(info.converged), "exact": info.exact,
+                                    "restarts": info.restarts, "basis_size": info.basis_size}
 
 
 def msbfs(ctx, graph, sources, *, direction="out", max_depth=None, depths=True, reach=True,

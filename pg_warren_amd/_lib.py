@@ -46,6 +46,22 @@ class TraversalStats(C.Structure):
                 ("truncated", C.c_uint32), ("levels_pulled", C.c_uint32)]
 
 
+class EigenOpts(C.Structure):
+    """wg_eigen_opts: EigenOptions (spectral/lanczos.rs:140-156) and
+    GraphMatrix (spectral/operator.rs:79-93)."""
+    _fields_ = [("matrix", C.c_int32), ("which", C.c_int32), ("weighted", C.c_int32),
+                ("symmetrize", C.c_int32), ("k", C.c_uint32), ("basis_size", C.c_uint32),
+                ("tol", C.c_double), ("max_iter", C.c_uint64), ("precision", C.c_int32),
+                ("pad", C.c_int32), ("seed", C.c_uint64)]
+
+
+class EigenInfo(C.Structure):
+    """wg_eigen_info: Eigen (spectral/lanczos.rs:174-198)."""
+    _fields_ = [("iterations", C.c_uint64), ("components", C.c_uint64),
+                ("converged", C.c_uint32), ("exact", C.c_uint32), ("restarts", C.c_uint32),
+                ("basis_size", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("wg_context_create", I32, [I32, U64, P]),
@@ -75,6 +91,7 @@ SIGNATURES = [
     ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
     ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
     ("wg_pagerank_backward", I32, [P, P, P, P, P, P, P, P, P, P]),
+    ("wg_eigen", I32, [P, P, P, P, P, P, P]),
     ("wg_bfs_levels", I32, [P, P, I32, P, U64, P, I64, I32, P, P, P]),
     ("wg_bfs", I32, [P, P, I32, P, U64, P, I64, I64, P, P, P]),
     ("wg_reachable_set", I32, [P, P, I32, P, U64, I64, P, P, P]),

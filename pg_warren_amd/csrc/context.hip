@@ -539,6 +539,10 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint
         if (w) rw[pos] = w[p];
       }
     ro.pop_back();
+    // A = A^T exactly when the reverse arrays equal the forward ones (an
+    // undirected graph's do; spectral/operator.rs:420-436).
+    g->rev_is_fwd = std::equal(ro.begin(), ro.end(), off) && std::equal(rt.begin(), rt.end(), tgt) &&
+                    (!w || std::equal(rw.begin(), rw.end(), w));
     s = upload_csr(ctx, g->rev, n, ro.data(), rt.data(), w ? rw.data() : nullptr);
   }
   if (s) {
@@ -623,6 +627,8 @@ int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges,
   dfree(ctx, de, count * 8);
   dfree(ctx, dv, count * 8);
   if (e != hipSuccess) return hip_fail(ctx, e, "update weights");
+  // A weight changed in one direction breaks A = A^T.
+  g->rev_is_fwd = false;
   // Weights cast to other element types are cast again on next use.
   for (DevCsr* c : {&g->fwd, &g->rev})
     for (int d = 0; d < 5; ++d) {

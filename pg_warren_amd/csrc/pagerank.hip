@@ -569,7 +569,8 @@ __global__ __launch_bounds__(256) void adj_gradient(const uint64_t* __restrict__
   }
 }
 
-int launch_weighted_f64(wg_context* ctx, wg_graph* g, const double* x, double* y);
+int launch_plus_times_f64(wg_context* ctx, wg_graph* g, bool transpose, bool weighted, const double* x,
+                          double* y);
 
 }  // namespace wg
 
@@ -659,7 +660,7 @@ extern "C" int wg_pagerank_backward(wg_context* ctx, const wg_graph* gc, const w
   while (it < o->max_iter) {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
     ++it;
-    if ((s = launch_weighted_f64(ctx, g, lt, u))) return s;
+    if ((s = launch_plus_times_f64(ctx, g, false, true, lt, u))) return s;
     adj_advance<<<grid, 256, 0, st>>>(u, out, g_in, tomb, p, start, n, o->damping, scal, lambda, lt, part);
     adj_reduce<<<1, 256, 0, st>>>(part, grid, scal);
     WG_HIP(ctx, hipGetLastError());
@@ -670,7 +671,7 @@ extern "C" int wg_pagerank_backward(wg_context* ctx, const wg_graph* gc, const w
     }
   }
   // The gradient from the final adjoint.
-  if ((s = launch_weighted_f64(ctx, g, lt, u))) return s;
+  if ((s = launch_plus_times_f64(ctx, g, false, true, lt, u))) return s;
   if (m) {
     const unsigned ggrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, 65536));
     adj_gradient<<<ggrid, 256, 0, st>>>(g->fwd.offsets, g->fwd.targets, n, (const double*)ranks->data, out, u,

@@ -554,10 +554,11 @@ int launch_out_weights(wg_context* ctx, wg_graph* g, bool weighted, const double
   return launch_product<double, PLUS, FIRST>(ctx, g->fwd, weighted, x, out, 1, wa);
 }
 
-// y = A (+.x) x over the forward CSR in f64, with the edge weights.
-int launch_weighted_f64(wg_context* ctx, wg_graph* g, const double* x, double* y) {
+// y = A (+.x) x (or A^T with transpose) in f64; unweighted entries read 1.
+int launch_plus_times_f64(wg_context* ctx, wg_graph* g, bool transpose, bool weighted, const double* x,
+                          double* y) {
   WriteArgs wa{nullptr, 0, 0, 0, -1};
-  return launch_product<double, PLUS, TIMES>(ctx, g->fwd, true, x, y, 1, wa);
+  return launch_product<double, PLUS, TIMES>(ctx, transpose ? g->rev : g->fwd, weighted, x, y, 1, wa);
 }
 
 // The checks of spmv / spmm / multiply (compute/product/mod.rs:77-153).

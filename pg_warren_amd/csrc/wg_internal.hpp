@@ -95,6 +95,7 @@ struct wg_graph {
   wg_context* ctx = nullptr;
   uint64_t n = 0, nnz = 0;
   bool weighted = false;
+  bool rev_is_fwd = false;  // the reverse CSR equals the forward one (A = A^T)
   wg::DevCsr fwd, rev;
   // Search workspace sized by the graph, allocated on first use and kept
   // (charged to the context) until the graph is destroyed.

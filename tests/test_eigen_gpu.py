@@ -1,0 +1,164 @@
+"""Parity of the HIP Lanczos (wg_eigen) with the dense spectral oracle:
+eigenvalues against numpy's eigvalsh of the operator matrix (the reference's
+spectrum oracle), the path graph's closed form (spectral/lanczos/tests.rs:
+75-101), explicit residuals, orthonormal eigenvectors of fixed sign, zero at
+inactive nodes, and the exact pairs the components give (deflation.rs)."""
+import numpy as np
+import pytest
+
+import pg_warren_amd as W
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = W.Context(0)
+    yield c
+    c.close()
+
+
+def upload(ctx, csr):
+    return W.Graph(ctx, csr.n, csr.offsets, csr.targets, csr.weights)
+
+
+def random_csr(oracle, rng, n, m, weighted=True, symmetric=False, isolated=0):
+    src = rng.integers(0, n - isolated, m)
+    dst = rng.integers(0, n - isolated, m)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    w = rng.uniform(0.5, 2.0, len(src)) if weighted else None
+    if symmetric:
+        src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
+        w = None if w is None else np.concatenate([w, w])
+    return oracle.csr_from_edges(n, src, dst, w)
+
+
+def check_vectors(vec, vals, csr, matrix, weighted=True, symmetrize=True, tol=1e-8):
+    m, active, scale = S.operator(csr, matrix, weighted, symmetrize)
+    x = vec.numpy().astype(np.float64)
+    inactive = np.setdiff1d(np.arange(csr.n), active)
+    assert np.all(x[inactive] == 0.0)
+    xa = x[active]
+    if matrix == "random_walk":
+        xa = xa / scale[:, None]  # back to the symmetric matrix's eigenvectors
+        xa = xa / np.linalg.norm(xa, axis=0)
+    for l, lam in enumerate(vals):
+        v = xa[:, l]
+        r = np.linalg.norm(m @ v - lam * v)
+        assert r <= tol * max(1.0, abs(lam)), (l, lam, r)
+        top = np.argmax(np.abs(x[:, l]))
+        assert x[top, l] > 0
+    g = xa.T @ xa
+    np.testing.assert_allclose(g, np.eye(len(vals)), atol=1e-7)
+
+
+def test_path_laplacian_closed_form(ctx, oracle):
+    n = 60
+    src = list(range(n - 1)) + list(range(1, n))
+    dst = list(range(1, n)) + list(range(n - 1))
+    csr = oracle.csr_from_edges(n, src, dst)
+    g = upload(ctx, csr)
+    vals, res, vec, info = W.eigen(ctx, g, 4, matrix="laplacian", which="smallest", precision="f64",
+                                   tol=1e-8)
+    assert info["converged"] and info["components"] == 1 and info["exact"] == 1
+    x = vec.numpy()
+    for j in range(4):
+        want = 2 - 2 * np.cos(np.pi * j / n)
+        assert abs(vals[j] - want) < 1e-9
+        assert res[j] < 1e-7
+        exact = np.cos(np.pi * j * (np.arange(n) + 0.5) / n)
+        exact /= np.linalg.norm(exact)
+        assert abs(abs(x[:, j] @ exact) - 1) < 1e-8
+
+
+@pytest.mark.parametrize("matrix,which", [("adjacency", "largest"), ("adjacency", "smallest"),
+                                          ("laplacian", "largest"), ("laplacian", "smallest"),
+                                          ("normalized_laplacian", "smallest"),
+                                          ("normalized_laplacian", "largest"),
+                                          ("random_walk", "largest")])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_matches_dense_spectrum(ctx, oracle, matrix, which, precision):
+    rng = np.random.default_rng(hash((matrix, which)) % 1000)
+    csr = random_csr(oracle, rng, 300, 1500, isolated=5)
+    g = upload(ctx, csr)
+    k = 6
+    tol = 1e-10 if precision == "f64" else 1e-6
+    vals, res, vec, info = W.eigen(ctx, g, k, matrix=matrix, which=which, precision=precision, tol=tol)
+    want = S.spectrum(csr, matrix, k, which)
+    scale = np.abs(np.linalg.eigvalsh(S.operator(csr, matrix)[0])).max()
+    atol = (1e-9 if precision == "f64" else 2e-5) * scale
+    np.testing.assert_allclose(vals, want, rtol=0, atol=atol)
+    assert info["converged"]
+    assert np.all(res <= (1e-7 if precision == "f64" else 1e-4) * scale)
+    check_vectors(vec, vals, csr, matrix, tol=1e-7 if precision == "f64" else 1e-4)
+
+
+def test_thick_restarts(ctx, oracle):
+    rng = np.random.default_rng(17)
+    csr = random_csr(oracle, rng, 2000, 12000, symmetric=True)
+    g = upload(ctx, csr)
+    vals, res, vec, info = W.eigen(ctx, g, 8, matrix="normalized_laplacian", which="smallest",
+                                   precision="f64", tol=1e-10, basis_size=24)
+    assert info["restarts"] > 0 and info["converged"]
+    np.testing.assert_allclose(vals, S.spectrum(csr, "normalized_laplacian", 8, "smallest"), atol=1e-9)
+    check_vectors(vec, vals, csr, "normalized_laplacian")
+
+
+def test_components_are_deflated(ctx, oracle):
+    # Four components (one bipartite 4-cycle, a triangle, an edge, a random
+    # block) and two isolated nodes: the normalised Laplacian's 0 four times
+    # from the components, Lanczos for the rest.
+    rng = np.random.default_rng(3)
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 4), (7, 8)]
+    blk = rng.integers(9, 60, (200, 2))
+    edges += [(a, b) for a, b in blk if a != b]
+    src = [a for a, b in edges] + [b for a, b in edges]
+    dst = [b for a, b in edges] + [a for a, b in edges]
+    csr = oracle.csr_from_edges(62, src, dst)
+    g = upload(ctx, csr)
+    vals, res, vec, info = W.eigen(ctx, g, 6, matrix="normalized_laplacian", which="smallest",
+                                   precision="f64", tol=1e-10)
+    assert info["components"] == 4 and info["exact"] == 4
+    np.testing.assert_allclose(vals, S.spectrum(csr, "normalized_laplacian", 6, "smallest"), atol=1e-9)
+    assert np.all(vals[:4] == 0.0)
+    check_vectors(vec, vals, csr, "normalized_laplacian")
+    vals, res, vec, info = W.eigen(ctx, g, 3, matrix="random_walk", which="largest", precision="f64",
+                                   tol=1e-10)
+    np.testing.assert_allclose(vals, S.spectrum(csr, "random_walk", 3, "largest"), atol=1e-9)
+
+
+def test_unsymmetrised_and_errors(ctx, oracle):
+    rng = np.random.default_rng(8)
+    sym = random_csr(oracle, rng, 100, 400, symmetric=True)
+    g = upload(ctx, sym)
+    vals, _, _, _ = W.eigen(ctx, g, 3, symmetrize=False, precision="f64", tol=1e-10, vectors=False)
+    np.testing.assert_allclose(vals, S.spectrum(sym, "adjacency", 3, "largest", symmetrize=False),
+                               atol=1e-9)
+    asym = random_csr(oracle, rng, 100, 400)
+    ga = upload(ctx, asym)
+    with pytest.raises(W.WarrenError) as e:
+        W.eigen(ctx, ga, 3, symmetrize=False)
+    assert e.value.code == 17
+    with pytest.raises(W.WarrenError) as e:
+        W.eigen(ctx, ga, 0)
+    assert e.value.code == 19
+    with pytest.raises(W.WarrenError) as e:
+        W.eigen(ctx, ga, 101)
+    assert e.value.code == 19
+    neg = oracle.csr_from_edges(3, [0, 1, 1, 2], [1, 0, 2, 1], [1.0, 1.0, -3.0, -3.0])
+    with pytest.raises(W.WarrenError) as e:
+        W.eigen(ctx, upload(ctx, neg), 1, matrix="normalized_laplacian")
+    assert e.value.code == 18
+
+
+def test_rmat_normalized_laplacian(ctx, oracle):
+    off, tgt = W.rmat(12, 16, seed=5, symmetrize=True)
+    csr = oracle.Csr(len(off) - 1, off, tgt)
+    g = upload(ctx, csr)
+    vals, res, vec, info = W.eigen(ctx, g, 12, matrix="normalized_laplacian", which="smallest",
+                                   precision="f64", tol=1e-10, basis_size=64)
+    want = S.spectrum(csr, "normalized_laplacian", 12, "smallest")
+    np.testing.assert_allclose(vals, want, atol=1e-9)
+    assert np.all(res < 1e-8)
