@@ -256,6 +256,76 @@ def bench_adjoint(W, ctx, scale=20, iters=200, fd_edges=200):
     return res, state
 
 
+def bench_lanczos(W, ctx, scale=21, k=32, basis=128):
+    """The k smallest normalised-Laplacian eigenpairs of symmetrised R-MAT
+    `scale` (f64 basis of `basis` vectors, tol 1e-12): on the whole graph,
+    whose components give all of them exactly (no Lanczos step), and on its
+    largest connected component, relabelled, where Lanczos computes all but
+    the one exact pair."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    off, tgt = W.rmat(scale, 16, seed=5, symmetrize=True)
+    n = len(off) - 1
+    g = W.Graph(ctx, n, off, tgt)
+    opts = dict(matrix="normalized_laplacian", which="smallest", precision="f64", tol=1e-12,
+                basis_size=basis)
+    t1 = time.perf_counter()
+    vals, res, _, info = W.eigen(ctx, g, k, vectors=False, **opts)
+    t_full = time.perf_counter() - t1
+    g.close()
+    a = csr_matrix((np.ones(len(tgt)), tgt.astype(np.int64), off.astype(np.int64)), shape=(n, n))
+    _, lab = connected_components(a, directed=False)
+    deg = np.diff(off.astype(np.int64))
+    big = np.argmax(np.bincount(lab[deg > 0]))
+    keep = np.flatnonzero((lab == big) & (deg > 0))
+    sub = a[keep][:, keep].tocsr()
+    sub.sort_indices()
+    off_c, tgt_c = sub.indptr.astype(np.uint64), sub.indices.astype(np.uint32)
+    gc = W.Graph(ctx, len(keep), off_c, tgt_c)
+    t1 = time.perf_counter()
+    cvals, cres, vec, cinfo = W.eigen(ctx, gc, k, **opts)
+    t_cc = time.perf_counter() - t1
+    rel = np.where(cvals != 0, cres / np.abs(np.where(cvals != 0, cvals, 1.0)), cres)
+    x = vec.numpy()
+    vec.close()
+    gc.close()
+    res_ = {"scale": scale, "n": n, "nnz": len(tgt), "k": k, "basis_size": basis,
+            "full_graph": {"seconds": round(t_full, 3), "components": info["components"],
+                           "exact_pairs": info["exact"], "lanczos_steps": info["iterations"]},
+            "largest_component": {"n": len(keep), "nnz": len(tgt_c), "seconds": round(t_cc, 3),
+                                  "steps": cinfo["iterations"], "restarts": cinfo["restarts"],
+                                  "converged": cinfo["converged"], "exact_pairs": cinfo["exact"],
+                                  "ms_per_step": round(t_cc * 1e3 / max(cinfo["iterations"], 1), 3),
+                                  "max_rel_residual": float(rel.max()),
+                                  "eigenvalues": [float(v) for v in cvals]}}
+    log(f"lanczos s{scale}: full graph {t_full:.2f} s ({info['exact']} exact pairs); largest "
+        f"component {t_cc:.2f} s, {cinfo['iterations']} steps, {cinfo['restarts']} restarts, "
+        f"max rel residual {rel.max():.2e}")
+    return res_, {"off": off_c, "tgt": tgt_c, "x": x, "vals": cvals}
+
+
+def lanczos_oracle_errors(st, threads):
+    """||L x - lambda x|| / |lambda| (absolute at lambda = 0) and ||X^T X - I||
+    for the largest-component eigenpairs, with the oracle's f64 products."""
+    import oracle as O
+
+    n = len(st["off"]) - 1
+    csr = O.Csr(n, st["off"], st["tgt"])
+    t1 = time.perf_counter()
+    deg = O.spmv(csr, np.ones(n), "plus_times", "f64", threads=threads)
+    s = 1.0 / np.sqrt(deg)
+    worst = 0.0
+    for l, lam in enumerate(st["vals"]):
+        x = st["x"][:, l]
+        lx = x - s * O.spmv(csr, s * x, "plus_times", "f64", threads=threads)
+        r = np.linalg.norm(lx - lam * x)
+        worst = max(worst, r / abs(lam) if lam != 0 else r)
+    orth = float(np.abs(st["x"].T @ st["x"] - np.eye(st["x"].shape[1])).max())
+    return {"max_rel_residual_f64_cpu": float(worst), "max_orthogonality_error": orth,
+            "oracle_seconds": round(time.perf_counter() - t1, 1), "threads": threads}
+
+
 def adjoint_oracle_errors(st, threads):
     """The adjoint leg against the oracle's FP64 forward and adjoint."""
     import oracle as O
@@ -311,6 +381,8 @@ def main():
                     help="leave out the batched PageRank and multi-source BFS legs")
     ap.add_argument("--skip-adjoint", action="store_true",
                     help="leave out the PageRank adjoint leg (R-MAT two scales below)")
+    ap.add_argument("--skip-lanczos", action="store_true",
+                    help="leave out the Lanczos leg (symmetrised R-MAT one scale below)")
     ap.add_argument("--only-spmv", action="store_true",
                     help="profiling runs: the three SpMV legs and the copy kernel only")
     args = ap.parse_args()
@@ -437,9 +509,11 @@ def main():
     if not args.skip_batched:
         legs["ppr_batched_256"] = bench_batched_ppr(W, ctx, gf, n, deg)
         legs["msbfs_512"] = bench_msbfs(W, ctx, gf, n, deg)
-    adj_state = None
+    adj_state = lz_state = None
     if not args.skip_adjoint:
         legs["pagerank_adjoint"], adj_state = bench_adjoint(W, ctx, scale=max(args.scale - 2, 8))
+    if not args.skip_lanczos:
+        legs["lanczos"], lz_state = bench_lanczos(W, ctx, scale=max(args.scale - 1, 8))
 
     head = legs["plus_times_f32"]
     tuned = head.get("tuned_variant", 0)
@@ -476,6 +550,8 @@ def main():
                                                    bfs_depths, pr_gpu, iters)
         if adj_state is not None:
             result["errors_vs_oracle"]["pagerank_adjoint"] = adjoint_oracle_errors(adj_state, 16)
+        if lz_state is not None:
+            result["errors_vs_oracle"]["lanczos"] = lanczos_oracle_errors(lz_state, 16)
         log(f"errors vs oracle: {result['errors_vs_oracle']}")
     print(json.dumps(result), flush=True)
 
