@@ -152,7 +152,9 @@ int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_ve
  * dtype and semiring, times the launch variants and keeps the fastest:
  * bit 0 = the LDS-free direct kernel (only where the monoid's result does not
  * depend on grouping), bit 1 = non-temporal row streams, bit 2 = XCD-contiguous
- * block order. wg_set_spmv_variant forces one variant (0..7) for the context;
+ * block order, bit 3 = gathers from x permuted into the hot-first column order
+ * (columns ranked by how many entries read them; same values, same fold
+ * order, same result bits). wg_set_spmv_variant forces one variant (0..15);
  * wg_spmv_variant reports the choice made (-1: not tuned yet). */
 void wg_set_spmv_variant(wg_context* ctx, int variant);
 int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted);

@@ -250,6 +250,13 @@ static void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.long_pbase, (c.nlong + 1) * 4);
   for (int d = 0; d < 5; ++d)
     if (c.wview[d]) dfree(ctx, c.wview[d], c.nnz * dtype_size(d));
+  if (c.hot_state == 1) {
+    dfree(ctx, c.hot_tgt, std::max<uint64_t>(c.nnz, 1) * 4);
+    dfree(ctx, c.hot_inv, std::max<uint64_t>(c.hot_n, 1) * 4);
+    dfree(ctx, c.hot_x, std::max<uint64_t>(c.hot_n, 1) * 8);
+    dfree(ctx, c.hot_rank, std::max<uint64_t>(c.n, 1) * 4);
+  }
+  for (auto& t : c.lds_tgt) dfree(ctx, t.second.first, std::max<uint64_t>(c.nnz, 1) * 4);
   if (c.spmm_ready) {
     dfree(ctx, c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8);
     dfree(ctx, c.spmm_ticket_of_row, std::max<uint64_t>(c.n, 1) * 4);
@@ -333,6 +340,76 @@ __global__ __launch_bounds__(256) void renumber_sources(const uint32_t* __restri
                                                         uint32_t* __restrict__ out, uint64_t nnz) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * 256)
     out[i] = index[src[i]];
+}
+
+__global__ __launch_bounds__(256) void column_counts(const uint32_t* __restrict__ tgt, uint64_t nnz,
+                                                     unsigned int* __restrict__ count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * 256)
+    atomicAdd(&count[tgt[i]], 1u);
+}
+
+int prepare_hot_columns(wg_context* ctx, DevCsr& c) {
+  if (c.hot_state) return WG_OK;
+  const uint64_t n = c.n, m = c.nnz;
+  hipStream_t st = ctx->stream;
+  // How many entries gather each column (a one-time histogram).
+  unsigned int* cnt_d;
+  int s;
+  if ((s = dalloc(ctx, (void**)&cnt_d, std::max<uint64_t>(n, 1) * 4))) {
+    c.hot_state = -1;
+    return ok(ctx);
+  }
+  WG_HIP(ctx, hipMemsetAsync(cnt_d, 0, std::max<uint64_t>(n, 1) * 4, st));
+  if (m) column_counts<<<(unsigned)std::min<uint64_t>((m + 255) / 256, 16384), 256, 0, st>>>(c.targets, m, cnt_d);
+  std::vector<uint32_t> cnt(n);
+  if (n) WG_HIP(ctx, hipMemcpyAsync(cnt.data(), cnt_d, n * 4, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  // Ranks by descending count, ascending id within a count.
+  uint32_t maxc = 0;
+  for (uint32_t v : cnt) maxc = std::max(maxc, v);
+  std::vector<uint64_t> next((size_t)maxc + 2, 0);
+  for (uint32_t v : cnt) next[v]++;
+  uint64_t pos = 0;
+  for (uint64_t d = (uint64_t)maxc + 1; d-- > 1;) {
+    const uint64_t k = next[d];
+    next[d] = pos;
+    pos += k;
+  }
+  std::vector<uint32_t> rank(n, ~0u), inv(pos);
+  for (uint64_t v = 0; v < n; ++v)
+    if (cnt[v]) {
+      rank[v] = (uint32_t)next[cnt[v]]++;
+      inv[rank[v]] = (uint32_t)v;
+    }
+  const uint64_t nh = pos;
+  if ((s = dalloc(ctx, (void**)&c.hot_tgt, std::max<uint64_t>(m, 1) * 4))) {
+    dfree(ctx, cnt_d, std::max<uint64_t>(n, 1) * 4);
+    c.hot_tgt = nullptr;
+    c.hot_state = -1;
+    return ok(ctx);
+  }
+  if ((s = dalloc(ctx, (void**)&c.hot_inv, std::max<uint64_t>(nh, 1) * 4)) ||
+      (s = dalloc(ctx, &c.hot_x, std::max<uint64_t>(nh, 1) * 8))) {
+    dfree(ctx, cnt_d, std::max<uint64_t>(n, 1) * 4);
+    dfree(ctx, c.hot_tgt, std::max<uint64_t>(m, 1) * 4);
+    if (c.hot_inv) dfree(ctx, c.hot_inv, std::max<uint64_t>(nh, 1) * 4);
+    c.hot_tgt = nullptr;
+    c.hot_inv = nullptr;
+    c.hot_x = nullptr;
+    c.hot_state = -1;
+    return ok(ctx);
+  }
+  // The rank table stays in the histogram's buffer.
+  if (n) WG_HIP(ctx, hipMemcpyAsync(cnt_d, rank.data(), n * 4, hipMemcpyHostToDevice, st));
+  if (nh) WG_HIP(ctx, hipMemcpyAsync(c.hot_inv, inv.data(), nh * 4, hipMemcpyHostToDevice, st));
+  if (m)
+    renumber_sources<<<2048, 256, 0, st>>>(c.targets, (const uint32_t*)cnt_d, c.hot_tgt, m);
+  WG_HIP(ctx, hipGetLastError());
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  c.hot_rank = (uint32_t*)cnt_d;
+  c.hot_n = nh;
+  c.hot_state = 1;
+  return WG_OK;
 }
 
 int prepare_hot_order(wg_context* ctx, wg_graph* g) {

@@ -55,7 +55,27 @@ enum : int {
   kVarDirect = 1,  // spmv_direct instead of the LDS-staged spmv_blocks
   kVarNT = 2,      // non-temporal loads of the target / weight streams
   kVarXcd = 4,     // XCD-contiguous block order
+  kVarHot = 8,     // gather from x permuted into the hot-first column order
+  kVarLds = 16,    // persistent workgroups with the hottest columns' x in LDS
 };
+
+// Targets tagged for the LDS cache: rank | bit 31 for the first nhot ranks.
+__global__ __launch_bounds__(256) void tag_targets(const uint32_t* __restrict__ tgt,
+                                                   const uint32_t* __restrict__ rank, uint32_t nhot,
+                                                   uint64_t nnz, uint32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t t = tgt[i], r = rank[t];
+    out[i] = r < nhot ? (0x80000000u | r) : t;
+  }
+}
+
+// x in the hot-first column order: xh[r] = x[inv[r]].
+template <class E>
+__global__ __launch_bounds__(256) void gather_hot(const E* __restrict__ x, const uint32_t* __restrict__ inv,
+                                                  uint64_t nh, E* __restrict__ xh) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nh; i += (uint64_t)gridDim.x * 256)
+    xh[i] = x[inv[i]];
+}
 
 // A streamed load, non-temporal when nt (the row streams are read once and
 // should not evict the gathered x from L2 and the infinity cache).
@@ -74,21 +94,30 @@ __device__ __forceinline__ uint32_t block_index(bool xcd) {
   return k * q + (k < r ? k : r) + j;
 }
 
-template <class E, int ADD, int MUL, bool HW>
-__global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
-    const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
-    E* __restrict__ y, WriteArgs wa, typename Fold<E, ADD>::A* __restrict__ partials,
-    unsigned long long* __restrict__ err, int variant) {
+// The LDS a row block stages through.
+template <class E, int ADD>
+struct BlockLds {
+  E prod[kBlockNnz];
+  uint32_t rs[kBlockRows + 1];
+  typename Fold<E, ADD>::A wpart[kBlockThreads / 64];
+  int wpres[kBlockThreads / 64];
+};
+
+// One row block: its products staged in LDS (x read through `gather`),
+// each row folded in order, split rows' pieces leaving partials.
+template <class E, int ADD, int MUL, bool HW, class Gather>
+__device__ __forceinline__ void block_body(const Block& b, const uint64_t* __restrict__ off,
+                                           const uint32_t* __restrict__ tgt, const E* __restrict__ w,
+                                           const Gather& gather, E* __restrict__ y, const WriteArgs& wa,
+                                           typename Fold<E, ADD>::A* __restrict__ partials,
+                                           unsigned long long* __restrict__ err, bool nt,
+                                           BlockLds<E, ADD>& L) {
   using F = Fold<E, ADD>;
   using A = typename F::A;
-  __shared__ E prod[kBlockNnz];
-  __shared__ uint32_t rs[kBlockRows + 1];
-  __shared__ A wpart[kBlockThreads / 64];
-  __shared__ int wpres[kBlockThreads / 64];
-
-  const bool nt = variant & kVarNT;
-  const Block b = blocks[block_index(variant & kVarXcd)];
+  E* prod = L.prod;
+  uint32_t* rs = L.rs;
+  A* wpart = L.wpart;
+  int* wpres = L.wpres;
   const uint32_t tid = threadIdx.x;
   uint32_t e_err = 0;
   uint64_t e0, e1;
@@ -121,7 +150,7 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
         E wk = E(1);
         if constexpr (HW) wk = wv[k];
         E xv = E();
-        if constexpr (reads_x<MUL>()) xv = x[col[k]];
+        if constexpr (reads_x<MUL>()) xv = gather(col[k]);
         prod[i] = multiply<E, ADD, MUL>(wk, xv, e_err);
       }
     }
@@ -206,6 +235,46 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
     }
   }
   if (e_err) raise_error(err, b.row0, e_err);
+}
+
+template <class E, int ADD, int MUL, bool HW>
+__global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
+    const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
+    E* __restrict__ y, WriteArgs wa, typename Fold<E, ADD>::A* __restrict__ partials,
+    unsigned long long* __restrict__ err, int variant) {
+  __shared__ BlockLds<E, ADD> L;
+  const Block b = blocks[block_index(variant & kVarXcd)];
+  block_body<E, ADD, MUL, HW>(b, off, tgt, w, [&](uint32_t c) { return x[c]; }, y, wa, partials, err,
+                              (variant & kVarNT) != 0, L);
+}
+
+// Hot-column cache: the persistent variant. Each workgroup first copies
+// the hottest columns' x (xhot, ranks [0, nhot)) into LDS, then takes row
+// blocks in turn; a tagged target (bit 31 set) names a hot rank read from
+// LDS, any other target a column of x read from memory. The gathers the
+// L2 serves (its request rate bounds this product) drop by the hot
+// columns' share of the entries.
+constexpr uint32_t kHotTag = 0x80000000u;
+constexpr uint32_t kHotLdsBytes = 48 * 1024;
+
+template <class E, int ADD, int MUL, bool HW>
+__global__ __launch_bounds__(kBlockThreads) void spmv_lds(
+    const Block* __restrict__ blocks, uint32_t nblocks, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
+    const E* __restrict__ xhot, uint32_t nhot, E* __restrict__ y, WriteArgs wa,
+    typename Fold<E, ADD>::A* __restrict__ partials, unsigned long long* __restrict__ err, int variant) {
+  __shared__ BlockLds<E, ADD> L;
+  __shared__ E hot[kHotLdsBytes / sizeof(E)];
+  for (uint32_t i = threadIdx.x; i < nhot; i += kBlockThreads) hot[i] = xhot[i];
+  __syncthreads();
+  const bool nt = (variant & kVarNT) != 0;
+  auto gather = [&](uint32_t t) { return (t & kHotTag) ? hot[t & ~kHotTag] : x[t]; };
+  for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    const Block b = blocks[bi];
+    block_body<E, ADD, MUL, HW>(b, off, tgt, w, gather, y, wa, partials, err, nt, L);
+    __syncthreads();
+  }
 }
 
 // Folds each split row's piece partials in piece order and writes the row.
@@ -440,27 +509,83 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       if (csr.nblocks == 0) return WG_OK;
       A* part = (A*)ctx->scratch;
       constexpr bool direct_ok = grouping_free<E, ADD>();
+      // The hot-first gathers change only where x is read from: the same
+      // values, folded in the same order, so the result is the same bits.
+      auto hot_ok = [&]() {
+        if (!reads_x<MUL>()) return false;
+        if (!csr.hot_state && prepare_hot_columns(ctx, csr)) return false;
+        return csr.hot_state == 1;
+      };
+      // The LDS variant's tagged targets for this element size (built once),
+      // and the resident workgroups it launches.
+      constexpr uint32_t kHotCap = kHotLdsBytes / sizeof(E);
+      auto lds_ok = [&]() -> bool {
+        if (!hot_ok() || csr.n >= (1ull << 31)) return false;
+        if (csr.lds_tgt.count(sizeof(E))) return true;
+        uint32_t* t = nullptr;
+        if (dalloc(ctx, (void**)&t, std::max<uint64_t>(csr.nnz, 1) * 4)) {
+          (void)ok(ctx);
+          return false;
+        }
+        const uint32_t nhot = (uint32_t)std::min<uint64_t>(csr.hot_n, kHotCap);
+        tag_targets<<<2048, 256, 0, ctx->stream>>>(csr.targets, csr.hot_rank, nhot, csr.nnz, t);
+        csr.lds_tgt[sizeof(E)] = {t, nhot};
+        return hipGetLastError() == hipSuccess;
+      };
+      auto lds_grid = [&]() -> unsigned {
+        int per_cu = 0, cus = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, wv ? (const void*)spmv_lds<E, ADD, MUL, true> : (const void*)spmv_lds<E, ADD, MUL, false>,
+            kBlockThreads, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(csr.nblocks, (uint64_t)std::max(per_cu, 1) * std::max(cus, 1)));
+      };
       auto launch = [&](int variant, E* out) {
+        const uint32_t* tg = csr.targets;
+        const E* xs = x;
+        if (variant & kVarLds) {
+          const auto& lt = csr.lds_tgt[sizeof(E)];
+          gather_hot<E><<<(unsigned)std::max<uint32_t>(1, (lt.second + 255) / 256), 256, 0, ctx->stream>>>(
+              x, csr.hot_inv, lt.second, (E*)csr.hot_x);
+          const unsigned grid = lds_grid();
+          if (wv)
+            spmv_lds<E, ADD, MUL, true><<<grid, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.nblocks, csr.offsets, lt.first, wv, x, (const E*)csr.hot_x, lt.second, out, wa,
+                part, ctx->d_err, variant);
+          else
+            spmv_lds<E, ADD, MUL, false><<<grid, kBlockThreads, 0, ctx->stream>>>(
+                csr.blocks, csr.nblocks, csr.offsets, lt.first, nullptr, x, (const E*)csr.hot_x, lt.second, out,
+                wa, part, ctx->d_err, variant);
+          return;
+        }
+        if (variant & kVarHot) {
+          gather_hot<E><<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((csr.hot_n + 255) / 256, 16384)),
+                          256, 0, ctx->stream>>>(x, csr.hot_inv, csr.hot_n, (E*)csr.hot_x);
+          tg = csr.hot_tgt;
+          xs = (const E*)csr.hot_x;
+        }
         if (direct_ok && (variant & kVarDirect)) {
           if (wv)
             spmv_direct<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-                csr.blocks, csr.offsets, csr.targets, wv, x, out, wa, part, ctx->d_err, variant);
+                csr.blocks, csr.offsets, tg, wv, xs, out, wa, part, ctx->d_err, variant);
           else
             spmv_direct<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-                csr.blocks, csr.offsets, csr.targets, nullptr, x, out, wa, part, ctx->d_err, variant);
+                csr.blocks, csr.offsets, tg, nullptr, xs, out, wa, part, ctx->d_err, variant);
         } else {
           if (wv)
             spmv_blocks<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-                csr.blocks, csr.offsets, csr.targets, wv, x, out, wa, part, ctx->d_err, variant);
+                csr.blocks, csr.offsets, tg, wv, xs, out, wa, part, ctx->d_err, variant);
           else
             spmv_blocks<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-                csr.blocks, csr.offsets, csr.targets, nullptr, x, out, wa, part, ctx->d_err, variant);
+                csr.blocks, csr.offsets, tg, nullptr, xs, out, wa, part, ctx->d_err, variant);
         }
       };
       int variant = 0;
       const int key = ((DtypeOf<E>::v * 16 + ADD) * 16 + MUL) * 2 + (wv ? 1 : 0);
       if (ctx->spmv_variant >= 0) {
         variant = ctx->spmv_variant;
+        if ((variant & kVarHot) && !hot_ok()) variant &= ~kVarHot;
+        if ((variant & kVarLds) && !lds_ok()) variant &= ~kVarLds;
       } else if (csr.tuned.count(key)) {
         variant = csr.tuned[key];
       } else if (csr.nnz >= kTuneMinNnz) {
@@ -472,8 +597,13 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
         WG_HIP(ctx, hipEventCreate(&e0));
         WG_HIP(ctx, hipEventCreate(&e1));
         float best = 1e30f;
-        for (int v = 0; v < 8; ++v) {
+        const bool hot = hot_ok();
+        const bool lds = hot && lds_ok();
+        for (int v = 0; v < 32; ++v) {
           if ((v & kVarDirect) && !direct_ok) continue;
+          if ((v & kVarHot) && !hot) continue;
+          // The LDS kernel is persistent, LDS-staged and reads x as given.
+          if ((v & kVarLds) && (!lds || (v & (kVarDirect | kVarXcd | kVarHot)))) continue;
           launch(v, (E*)ys);
           WG_HIP(ctx, hipEventRecord(e0, ctx->stream));
           for (int r = 0; r < 3; ++r) launch(v, (E*)ys);
@@ -612,7 +742,7 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
 extern "C" {
 
 void wg_set_spmv_variant(wg_context* ctx, int variant) {
-  if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 7);
+  if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 31);
 }
 
 int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted) {

@@ -61,6 +61,20 @@ struct DevCsr {
   bool spmm_ready = false;
   // Autotuned SpMV launch variant per (dtype, monoid, multiply, weighted).
   std::map<int, int> tuned;
+  // Hot-first column numbering for the gathers (built on first use): the
+  // gathered columns ranked by how many entries read them (descending, ties
+  // by id), the targets in that numbering, the original id of each rank,
+  // and room for x permuted into it. hot_state: 0 not built, 1 ready,
+  // -1 not available (the memory budget refused it).
+  uint32_t* hot_tgt = nullptr;
+  uint32_t* hot_inv = nullptr;
+  uint32_t* hot_rank = nullptr;  // n: each column's rank (~0: never gathered)
+  uint64_t hot_n = 0;
+  void* hot_x = nullptr;
+  int hot_state = 0;
+  // Targets tagged for the LDS hot-column cache, per element size: bit 31
+  // and the rank for the first `nhot` ranks, the column id otherwise.
+  std::map<int, std::pair<uint32_t*, uint32_t>> lds_tgt;
 };
 
 constexpr uint32_t kSpmmPiece = 2048;
@@ -149,6 +163,9 @@ bool is_cancelled(wg_context* ctx);
 // The weights of `csr` cast to `dtype` (cached); returns the cast status.
 int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out);
 int prepare_spmm(wg_context* ctx, DevCsr& csr);
+// Builds csr's hot-first column numbering; returns WG_OK with hot_state -1
+// when the budget cannot hold it.
+int prepare_hot_columns(wg_context* ctx, DevCsr& csr);
 
 #define WG_HIP(ctx, expr)                                   \
   do {                                                      \

@@ -174,7 +174,7 @@ ADD_OF = {"plus_times": "plus", "min_plus": "min", "max_plus": "max", "max_times
           "max_second": "max"}
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", list(range(16)) + [16, 18])
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool", "u32"])
 def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype, variant):
     """Integer-valued inputs: every partial sum is exact, so even float plus
@@ -203,7 +203,7 @@ def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype, varia
     W.set_spmv_variant(ctx, -1)
 
 
-@pytest.mark.parametrize("variant", [0, 3, 7])
+@pytest.mark.parametrize("variant", [0, 3, 7, 8, 15, 16, 18])
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool"])
 def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype, variant):
     W.set_spmv_variant(ctx, variant)
@@ -230,7 +230,7 @@ def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype, variant):
     W.set_spmv_variant(ctx, -1)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 6, 7])
+@pytest.mark.parametrize("variant", [0, 1, 6, 7, 9, 14, 18])
 def test_float_plus_within_rounding_bound(ctx, oracle, variant):
     W.set_spmv_variant(ctx, variant)
     g = hub_graph(oracle, 5)
@@ -305,7 +305,7 @@ def test_autotuner_choice_is_a_valid_variant(ctx, oracle):
     xv = W.Vector(ctx, x)
     tuned = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
     choice = W.spmv_variant(g, "plus_times", "f32")
-    assert 0 <= choice < 8
+    assert 0 <= choice < 32
     W.set_spmv_variant(ctx, 0)
     plain = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
     W.set_spmv_variant(ctx, -1)
@@ -318,3 +318,27 @@ def test_repeated_launches_are_bit_identical(ctx, oracle):
     first = run_gpu(ctx, g, x, "plus_times", "f32", transpose=True)
     for _ in range(3):
         assert run_gpu(ctx, g, x, "plus_times", "f32", transpose=True).tobytes() == first.tobytes()
+
+
+@pytest.mark.parametrize("variant", [8, 16, 18])
+def test_hot_column_variants_on_a_graph_wider_than_the_cache(ctx, oracle, variant):
+    """R-MAT scale 16 has more gathered columns than the LDS cache holds, so
+    the cached variant reads hot columns from LDS and the rest from x; every
+    variant must give the default launch's bits on integer-valued input."""
+    off, tgt = W.rmat(16, 16, seed=7)
+    n = len(off) - 1
+    w = W.edge_weights(off, tgt, "int", seed=8, p0=1, p1=9)
+    g = W.Graph(ctx, n, off, tgt, w)
+    rng = np.random.default_rng(9)
+    for sr, dtype in [("plus_times", "f32"), ("plus_times", "f64"), ("min_plus", "u32"),
+                      ("min_plus", "i64"), ("lor_land", "bool"), ("max_times", "f64")]:
+        x = values(dtype, n, rng, ADD_OF[sr])
+        for transpose in (False, True):
+            prod = W.Product(sr, transpose=transpose)
+            xv = W.Vector(ctx, x, dtype=dtype)
+            W.set_spmv_variant(ctx, 0)
+            want = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+            W.set_spmv_variant(ctx, variant)
+            got = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+            W.set_spmv_variant(ctx, -1)
+            assert got.tobytes() == want.tobytes(), (sr, dtype, transpose)
