@@ -23,6 +23,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "semiring.hpp"
@@ -392,16 +393,21 @@ __global__ __launch_bounds__(256) void defl_pieces(const uint32_t* __restrict__ 
   }
 }
 
+// A workgroup a component: its pieces' sums in a fixed order.
 __global__ __launch_bounds__(256) void defl_coef(const double* __restrict__ part, const uint32_t* __restrict__ first_piece,
                                                  uint32_t ncomp, double* __restrict__ coef) {
-  for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < ncomp; c += gridDim.x * 256) {
-    double a = 0.0, b = 0.0;
-    for (uint32_t p = first_piece[c]; p < first_piece[c + 1]; ++p) {
-      a += part[2 * p];
-      b += part[2 * p + 1];
-    }
-    coef[2 * c] = a;
-    coef[2 * c + 1] = b;
+  __shared__ double sh[4];
+  const uint32_t c = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (uint32_t p = first_piece[c] + threadIdx.x; p < first_piece[c + 1]; p += 256) {
+    a += part[2 * p];
+    b += part[2 * p + 1];
+  }
+  const double ta = block_total(a, sh);
+  const double tb = block_total(b, sh);
+  if (threadIdx.x == 0) {
+    coef[2 * c] = ta;
+    coef[2 * c + 1] = tb;
   }
 }
 
@@ -466,13 +472,14 @@ __global__ __launch_bounds__(256) void store_column(const double* __restrict__ x
 // rotations accumulated into Q.
 static bool sym_eigen(std::vector<double> a, int m, std::vector<double>& values, std::vector<double>& vectors) {
   auto A = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
-  std::vector<double>& q = vectors;
-  q.assign((size_t)m * m, 0.0);
-  auto Q = [&](int i, int j) -> double& { return q[(size_t)i * m + j]; };
-  for (int i = 0; i < m; ++i) Q(i, i) = 1.0;
+  // The orthogonal factor is kept transposed (row k of qt = column k of
+  // Q), so the reflections and rotations sweep contiguous rows.
+  std::vector<double> qt((size_t)m * m, 0.0);
+  auto Qt = [&](int k, int r) -> double& { return qt[(size_t)k * m + r]; };
+  for (int i = 0; i < m; ++i) Qt(i, i) = 1.0;
   for (double v : a)
     if (!std::isfinite(v)) return false;
-  std::vector<double> v(m), p(m), w(m);
+  std::vector<double> v(m), p(m), w(m), t(m);
   for (int k = 0; k + 2 < m; ++k) {
     const int len = m - k - 1;
     double norm2 = 0.0;
@@ -490,23 +497,32 @@ static bool sym_eigen(std::vector<double> a, int m, std::vector<double>& values,
     // A22 -= v w^T + w v^T with p = beta A22 v, w = p - (beta p.v / 2) v.
     double pv = 0.0;
     for (int i = 0; i < len; ++i) {
-      double s = 0.0;
-      for (int j = 0; j < len; ++j) s += A(k + 1 + i, k + 1 + j) * v[j];
-      p[i] = beta * s;
+      const double* row = &A(k + 1 + i, k + 1);
+      double sum = 0.0;
+      for (int j = 0; j < len; ++j) sum += row[j] * v[j];
+      p[i] = beta * sum;
       pv += p[i] * v[i];
     }
     const double kk = beta * pv / 2.0;
     for (int i = 0; i < len; ++i) w[i] = p[i] - kk * v[i];
-    for (int i = 0; i < len; ++i)
-      for (int j = 0; j < len; ++j) A(k + 1 + i, k + 1 + j) -= v[i] * w[j] + w[i] * v[j];
+    for (int i = 0; i < len; ++i) {
+      double* row = &A(k + 1 + i, k + 1);
+      const double vi = v[i], wi = w[i];
+      for (int j = 0; j < len; ++j) row[j] -= vi * w[j] + wi * v[j];
+    }
     A(k + 1, k) = A(k, k + 1) = alpha;
     for (int i = 1; i < len; ++i) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
-    // Q[:, k+1:] (I - beta v v^T).
-    for (int r = 0; r < m; ++r) {
-      double t = 0.0;
-      for (int j = 0; j < len; ++j) t += Q(r, k + 1 + j) * v[j];
-      t *= beta;
-      for (int j = 0; j < len; ++j) Q(r, k + 1 + j) -= t * v[j];
+    // Q[:, k+1:] (I - beta v v^T), on the rows of qt.
+    std::fill(t.begin(), t.end(), 0.0);
+    for (int j = 0; j < len; ++j) {
+      const double* row = &Qt(k + 1 + j, 0);
+      const double vj = v[j];
+      for (int r = 0; r < m; ++r) t[r] += row[r] * vj;
+    }
+    for (int j = 0; j < len; ++j) {
+      double* row = &Qt(k + 1 + j, 0);
+      const double bv = beta * v[j];
+      for (int r = 0; r < m; ++r) row[r] -= bv * t[r];
     }
   }
   std::vector<double> d(m), e(std::max(m - 1, 1), 0.0);
@@ -541,16 +557,21 @@ static bool sym_eigen(std::vector<double> a, int m, std::vector<double>& values,
         e[k + 1] = c * e[k + 1];
         x = e[k];
       }
+      double* qa = &Qt(k, 0);
+      double* qb = &Qt(k + 1, 0);
       for (int r2 = 0; r2 < m; ++r2) {
-        const double qa = Q(r2, k), qb = Q(r2, k + 1);
-        Q(r2, k) = c * qa + s * qb;
-        Q(r2, k + 1) = -s * qa + c * qb;
+        const double u0 = qa[r2], u1 = qb[r2];
+        qa[r2] = c * u0 + s * u1;
+        qb[r2] = -s * u0 + c * u1;
       }
     }
   }
   values = d;
   for (double val : values)
     if (!std::isfinite(val)) return false;
+  vectors.resize((size_t)m * m);
+  for (int i = 0; i < m; ++i)
+    for (int l = 0; l < m; ++l) vectors[(size_t)i * m + l] = Qt(l, i);
   return true;
 }
 
@@ -732,15 +753,18 @@ struct Eigen {
   }
 
   // output = M input (operator.rs:237-259).
-  int apply() {
-    const double* x = input;
+  // output = M src, src the input buffer unless given (an f64 basis
+  // column is read in place).
+  int apply(const double* src = nullptr) {
+    if (!src) src = input;
+    const double* x = src;
     if (normalized) {
-      op_scale<<<L.grid(n), 256, 0, st>>>(input, scale, scaled, n);
+      op_scale<<<L.grid(n), 256, 0, st>>>(src, scale, scaled, n);
       x = scaled;
     }
     int s;
     if ((s = multiply(x, output, weighted))) return s;
-    op_combine<<<L.grid(n), 256, 0, st>>>(output, transpose ? scratch : nullptr, input, deg, scale, o.matrix, n,
+    op_combine<<<L.grid(n), 256, 0, st>>>(output, transpose ? scratch : nullptr, src, deg, scale, o.matrix, n,
                                           output);
     WG_HIP(ctx, hipGetLastError());
     return WG_OK;
@@ -899,15 +923,15 @@ struct Eigen {
     return WG_OK;
   }
 
-  // x onto the complement of the components' vectors; returns ||x||.
-  double deflate(double* x) {
+  // x onto the complement of the components' vectors; ||x||^2 into the
+  // device scalar *sq (enqueued only).
+  void deflate_enqueue(double* x, double* sq) {
     if (npieces) defl_pieces<<<npieces, 256, 0, st>>>(d_order, d_pstart, d_unit, d_colour, x, d_dpart);
-    defl_coef<<<std::max<uint32_t>(1, (comps.count + 255) / 256), 256, 0, st>>>(d_dpart, d_firstp, comps.count,
-                                                                               d_dcoef);
+    if (comps.count) defl_coef<<<comps.count, 256, 0, st>>>(d_dpart, d_firstp, comps.count, d_dcoef);
     defl_apply<<<nblocks, 256, 0, st>>>(x, d_comp, d_colour, d_unit, d_bip, d_dcoef,
                                         std::isnan(known_connected) ? 0.0 : 1.0,
                                         std::isnan(known_bipartite) ? 0.0 : 1.0, n, part);
-    return std::sqrt(f64_sum_blocks(nblocks));
+    reduce_cols<<<1, 256, 0, st>>>(part, nblocks, 1, sq);
   }
 
   double norm(const double* x) {
@@ -1004,13 +1028,16 @@ struct Solver {
     reduce_cols<<<1, 256, 0, st>>>(op.part, op.nblocks, 1, out);
   }
 
-  // basis.rs:130-156: the coefficients Q^T w and ||w|| after.
-  int orthogonalize(double* w, bool use_prev, std::vector<double>& coefs, double& nrm) {
+  // basis.rs:130-156: the coefficients Q^T w and ||w|| after; with the
+  // deflation, also ||w|| after projecting out the components' vectors
+  // (read in the same round trip).
+  int orthogonalize(double* w, bool use_prev, std::vector<double>& coefs, double& nrm, double* deflated = nullptr) {
     coefs.assign(len, 0.0);
     double* first = op.coef;             // len
     double* corr = op.coef + kMaxBasis;  // len
     double* alpha = op.scal + 1;
     double* nd = op.scal + 2;
+    double* dd = op.scal + 3;
     int32_t pj = -1;
     const bool recurrence = use_prev && len >= 2;
     if (recurrence) {
@@ -1028,16 +1055,19 @@ struct Solver {
       dot_partial<<<op.nblocks, 256, 0, st>>>(w, w, n, op.part);
       reduce_cols<<<1, 256, 0, st>>>(op.part, op.nblocks, 1, nd);
     }
+    if (deflated) op.deflate_enqueue(w, dd);
     WG_HIP(ctx, hipGetLastError());
     std::vector<double> hf(len), hc(len);
-    double ha = 0.0, hn = 0.0;
+    double ha = 0.0, hn = 0.0, hd = 0.0;
     if (len) {
       if (!recurrence) WG_HIP(ctx, hipMemcpyAsync(hf.data(), first, len * 8, hipMemcpyDeviceToHost, st));
       WG_HIP(ctx, hipMemcpyAsync(hc.data(), corr, len * 8, hipMemcpyDeviceToHost, st));
     }
     if (recurrence) WG_HIP(ctx, hipMemcpyAsync(&ha, alpha, 8, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipMemcpyAsync(&hn, nd, 8, hipMemcpyDeviceToHost, st));
+    if (deflated) WG_HIP(ctx, hipMemcpyAsync(&hd, dd, 8, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipStreamSynchronize(st));
+    if (deflated) *deflated = std::sqrt(hd);
     if (recurrence) {
       coefs[len - 2] = prev;
       coefs[len - 1] = ha;
@@ -1057,14 +1087,18 @@ struct Solver {
   // lanczos.rs:546-570. beta < 0: breakdown.
   int step(double& beta) {
     const uint32_t j = len - 1;
-    bas_column<E><<<op.L.grid(n), 256, 0, st>>>(Q, ld, j, n, op.input);
     int s;
-    if ((s = op.apply())) return s;
+    if constexpr (std::is_same<E, double>::value) {
+      if ((s = op.apply(Q + (uint64_t)j * ld))) return s;
+    } else {
+      bas_column<E><<<op.L.grid(n), 256, 0, st>>>(Q, ld, j, n, op.input);
+      if ((s = op.apply())) return s;
+    }
     ++iterations;
     std::vector<double> coefs;
-    double nrm;
-    if ((s = orthogonalize(op.output, has_prev, coefs, nrm))) return s;
-    double b = op.deflated ? op.deflate(op.output) : nrm;
+    double nrm, dn = 0.0;
+    if ((s = orthogonalize(op.output, has_prev, coefs, nrm, op.deflated ? &dn : nullptr))) return s;
+    double b = op.deflated ? dn : nrm;
     for (uint32_t i = 0; i < len; ++i) {
       h(i, j) = coefs[i];
       h(j, i) = coefs[i];
@@ -1082,10 +1116,10 @@ struct Solver {
     op_random<<<op.L.grid(n), 256, 0, st>>>(op.active, op.o.seed, streams - 1, n, op.input);
     const double before = op.norm(op.input);
     std::vector<double> coefs;
-    double nrm;
+    double nrm, dn = 0.0;
     int s;
-    if ((s = orthogonalize(op.input, false, coefs, nrm))) return s;
-    if (op.deflated) nrm = op.deflate(op.input);
+    if ((s = orthogonalize(op.input, false, coefs, nrm, op.deflated ? &dn : nullptr))) return s;
+    if (op.deflated) nrm = dn;
     ok = nrm > breakdown * before;
     if (!ok) return WG_OK;
     push(op.input, nrm);
