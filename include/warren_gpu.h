@@ -154,7 +154,10 @@ int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_ve
  * depend on grouping), bit 1 = non-temporal row streams, bit 2 = XCD-contiguous
  * block order, bit 3 = gathers from x permuted into the hot-first column order
  * (columns ranked by how many entries read them; same values, same fold
- * order, same result bits). wg_set_spmv_variant forces one variant (0..15);
+ * order, same result bits), bit 4 = a persistent kernel that keeps the
+ * hottest columns' x values in LDS and reads them there (bits 0, 2 and 3 are
+ * then ignored). wg_set_spmv_variant forces one variant (0..31; a bit the
+ * graph, dtype or monoid cannot take is dropped);
  * wg_spmv_variant reports the choice made (-1: not tuned yet). */
 void wg_set_spmv_variant(wg_context* ctx, int variant);
 int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted);
