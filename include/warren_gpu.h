@@ -43,7 +43,9 @@ enum wg_status {
   WG_NOT_SYMMETRIC = 17,       /* SpectralError::NotSymmetric */
   WG_NON_POSITIVE_DEGREE = 18, /* SpectralError::NonPositiveDegree */
   WG_EIGENPAIR_COUNT = 19,     /* SpectralError::EigenpairCount */
-  WG_NOT_FINITE = 20           /* SpectralError::NotFinite */
+  WG_NOT_FINITE = 20,          /* SpectralError::NotFinite */
+  WG_EDGE_OUTSIDE_ID_SPACE = 21,       /* GraphError::EdgeOutsideIdSpace (graph/error.rs:91-98) */
+  WG_RELATION_OUTSIDE_DICTIONARY = 22  /* GraphError::RelationOutsideDictionary (:99-104) */
 };
 
 /* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
@@ -105,6 +107,26 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* offsets,
 void wg_graph_destroy(wg_graph* g);
 /* n, nnz and whether the graph carries weights. */
 int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted);
+/* GraphCsr::build_as (graph/csr/graph_csr.rs:210-221) on the device: the
+ * CSR pair of an edge list of m edges sources[i] -> targets[i] with
+ * optional weights[i] and relations[i] (< relation_count; NULL: every edge
+ * is relation 0). Rows come out sorted by target; parallel edges of one
+ * relation fold into one, weights summed in input order, and a target
+ * reached through several relations appears once, the relations' sums
+ * added in relation order (build/deduplication.rs:83-152); self loops stay.
+ * The first edge outside the id space or the dictionary is the error
+ * (build/mod.rs:329-352). *deduplicated (may be NULL) receives the edges
+ * folded into another edge of the same relation. The arrays are host
+ * memory, or device memory when device_pointers is non-zero. Unweighted
+ * when weights is NULL. */
+int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* sources,
+                   const uint32_t* targets, const double* weights, const uint32_t* relations,
+                   uint32_t relation_count, int device_pointers, wg_graph** out,
+                   uint64_t* deduplicated);
+/* The forward CSR (host buffers of n+1, nnz, nnz; targets and weights may
+ * be NULL). */
+int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* offsets,
+                          uint32_t* targets, double* weights);
 /* The reverse CSR as built on the device (host buffers of n+1, nnz, nnz;
  * weights may be NULL). */
 int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* offsets,

@@ -77,6 +77,8 @@ def lib():
         L.or_csr_validate.argtypes = [P]
         L.or_csr_build.argtypes = [C.c_uint64, C.c_uint64, P, P, P, P, P, P, P]
         L.or_csr_transpose.argtypes = [P, P, P, P]
+        L.or_csr_build_merged.argtypes = [C.c_uint64, C.c_uint64, P, P, P, P, C.c_uint32, P, P, P,
+                                          P, P, P]
         L.or_spmv.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int,
                               C.c_int, C.c_int, C.c_uint64, P, P, C.c_int]
         L.or_pagerank.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int, C.c_uint32, P, P, P,
@@ -145,6 +147,30 @@ def csr_from_edges(n, src, dst, weights=None):
     _check(lib().or_csr_build(n, m, _p(src), _p(dst), _p(w), _p(off), _p(tgt), _p(wo), _p(nnz)))
     k = int(nnz[0])
     return Csr(n, off, tgt[:k].copy(), None if wo is None else wo[:k].copy())
+
+
+def csr_from_edge_list(n, src, dst, weights=None, relations=None, relation_count=1):
+    """GraphCsr::build's merged all-relations CSR (graph/csr/build/mod.rs:248-288):
+    returns (Csr, folded). Raises OracleError 21 / 22 for the first edge outside
+    the id space / the dictionary, with ``.edge`` set to its index."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    m = len(src)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    rel = None if relations is None else np.ascontiguousarray(relations, np.uint32)
+    off = np.zeros(n + 1, np.uint64)
+    tgt = np.zeros(max(m, 1), np.uint32)
+    wo = np.zeros(max(m, 1), np.float64) if w is not None else None
+    out = np.zeros(3, np.uint64)
+    code = lib().or_csr_build_merged(n, m, _p(src), _p(dst), _p(w), _p(rel), relation_count,
+                                     _p(off), _p(tgt), _p(wo), _p(out[0:]), _p(out[1:]),
+                                     _p(out[2:]))
+    if code != OK:
+        err = OracleError(code)
+        err.edge = int(out[2])
+        raise err
+    k = int(out[0])
+    return Csr(n, off, tgt[:k].copy(), None if wo is None else wo[:k].copy()), int(out[1])
 
 
 def spmv(fwd, x, semiring="plus_times", dtype=None, *, rev=None, transpose=False,

@@ -461,6 +461,67 @@ int or_csr_build(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
   return OR_OK;
 }
 
+int or_csr_build_merged(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* weights, const uint32_t* rel, uint32_t nrel,
+                        uint64_t* offsets_out, uint32_t* targets_out, double* weights_out,
+                        uint64_t* nnz_out, uint64_t* folded_out, uint64_t* bad_edge_out) {
+  // counted(), build/mod.rs:329-352: per edge, the id space, then the
+  // relation.
+  for (uint64_t i = 0; i < m; ++i) {
+    if (src[i] >= n || dst[i] >= n) {
+      *bad_edge_out = i;
+      return OR_EDGE_OUTSIDE_ID_SPACE;
+    }
+    if (rel && rel[i] >= nrel) {
+      *bad_edge_out = i;
+      return OR_RELATION_OUTSIDE_DICTIONARY;
+    }
+  }
+  auto relation = [&](uint64_t e) -> uint32_t { return rel ? rel[e] : 0u; };
+  // dense_rows (columns.rs:263-303): counting sort by source, input order
+  // kept within a row.
+  std::vector<uint64_t> start(n + 1, 0);
+  for (uint64_t i = 0; i < m; ++i) start[src[i] + 1]++;
+  for (uint64_t r = 0; r < n; ++r) start[r + 1] += start[r];
+  std::vector<uint64_t> order(m), cursor(start.begin(), start.end() - 1);
+  for (uint64_t i = 0; i < m; ++i) order[cursor[src[i]]++] = i;
+  uint64_t out = 0, folded = 0;
+  offsets_out[0] = 0;
+  std::vector<uint64_t> row;
+  for (uint64_t r = 0; r < n; ++r) {
+    row.assign(order.begin() + start[r], order.begin() + start[r + 1]);
+    std::stable_sort(row.begin(), row.end(), [&](uint64_t a, uint64_t b) {
+      return dst[a] != dst[b] ? dst[a] < dst[b] : relation(a) < relation(b);
+    });
+    size_t k = 0;
+    while (k < row.size()) {
+      const uint32_t t = dst[row[k]];
+      double total = 0.0;
+      bool first = true;
+      while (k < row.size() && dst[row[k]] == t) {
+        // relation_sum (deduplication.rs:196-206).
+        const uint32_t rr = relation(row[k]);
+        double sum = weights ? weights[row[k]] : 1.0;
+        ++k;
+        while (k < row.size() && dst[row[k]] == t && relation(row[k]) == rr) {
+          sum += weights ? weights[row[k]] : 1.0;
+          ++folded;
+          ++k;
+        }
+        total = first ? sum : total + sum;
+        first = false;
+      }
+      targets_out[out] = t;
+      if (weights_out) weights_out[out] = total;
+      ++out;
+    }
+    offsets_out[r + 1] = out;
+  }
+  *nnz_out = out;
+  *folded_out = folded;
+  return OR_OK;
+}
+
 int or_csr_transpose(const or_csr* f, uint64_t* off, uint32_t* tgt, double* wts) {
   uint64_t n = f->n, m = f->offsets[n];
   // Count in-degrees into off[t + 2], prefix, then scatter through off[t + 1]

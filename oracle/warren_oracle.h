@@ -38,7 +38,9 @@ enum {
   OR_NODE_OUTSIDE_ID_SPACE = 12,
   OR_NODE_FILTER_ID_SPACE = 13,
   OR_INVALID_CSR = 14,
-  OR_UNSUPPORTED = 15
+  OR_UNSUPPORTED = 15,
+  OR_EDGE_OUTSIDE_ID_SPACE = 21,
+  OR_RELATION_OUTSIDE_DICTIONARY = 22
 };
 
 /* Element types (compute/dtype.rs:12-21) plus u32, the image of i64 on
@@ -65,6 +67,20 @@ int or_csr_validate(const or_csr* csr);
 int or_csr_build(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                  const double* weights, uint64_t* offsets_out, uint32_t* targets_out,
                  double* weights_out, uint64_t* nnz_out);
+
+/* The merged all-relations CSR of an edge list as GraphCsr::build does
+ * (graph/csr/build/mod.rs:248-288, 329-352; columns.rs:263-343;
+ * deduplication.rs:83-152, 196-206): edges checked in order (the first
+ * edge outside the id space, else outside the dictionary, is the error and
+ * its index goes to *bad_edge_out), counted by source keeping input order,
+ * each row sorted stably by (target, relation), weights summed per
+ * relation in input order and the relation sums added in relation order.
+ * relations may be NULL (every edge relation 0). *nnz_out = entries kept,
+ * *folded_out = edges folded into an edge of the same relation. */
+int or_csr_build_merged(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* weights, const uint32_t* relations, uint32_t relation_count,
+                        uint64_t* offsets_out, uint32_t* targets_out, double* weights_out,
+                        uint64_t* nnz_out, uint64_t* folded_out, uint64_t* bad_edge_out);
 
 /* The reverse CSR by counting-sort transpose (graph/csr/transpose.rs:16-44).*/
 int or_csr_transpose(const or_csr* fwd, uint64_t* offsets_out, uint32_t* targets_out,

@@ -24,7 +24,8 @@ STATUS_NAMES = {
     10: "bigint_out_of_range", 11: "invalid_argument",
     12: "node_outside_id_space", 13: "node_filter_id_space", 14: "invalid_csr",
     15: "unsupported", 16: "device_error", 17: "not_symmetric", 18: "non_positive_degree",
-    19: "eigenpair_count", 20: "not_finite",
+    19: "eigenpair_count", 20: "not_finite", 21: "edge_outside_id_space",
+    22: "relation_outside_dictionary",
 }
 DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
 NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
@@ -184,6 +185,58 @@ class Graph:
         ctx._children.add(self)
         self.nnz = int(offsets[-1])
         self.weighted = weights is not None
+        self.deduplicated = 0
+
+    @classmethod
+    def build(cls, ctx, n, sources, targets, weights=None, relations=None, relation_count=1):
+        """GraphCsr::build_as (graph/csr/graph_csr.rs:210-221) on the device from
+        an edge list: rows sorted, parallel edges folded (weights summed in input
+        order, then across relations in relation order), self loops kept. Returns
+        the graph; ``graph.deduplicated`` counts the edges folded into another of
+        the same relation. Arrays are numpy (host) arrays."""
+        sources, targets = np.asarray(sources), np.asarray(targets)
+        for col in (sources, targets):
+            if col.dtype != np.uint32 and col.size and (col.min() < 0 or col.max() >= 2**32):
+                bad = int(np.flatnonzero((sources < 0) | (sources >= 2**32) | (targets < 0)
+                                         | (targets >= 2**32))[0])
+                raise WarrenError(21, f"edge {bad} from {sources[bad]} to {targets[bad]} lies "
+                                      "outside the id space")
+        src = np.ascontiguousarray(sources, np.uint32)
+        dst = np.ascontiguousarray(targets, np.uint32)
+        if src.shape != dst.shape:
+            raise ValueError("sources and targets differ in length")
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        rel = None if relations is None else np.ascontiguousarray(relations, np.uint32)
+        for a in (w, rel):
+            if a is not None and a.shape != src.shape:
+                raise ValueError("edge columns differ in length")
+        h = C.c_void_p()
+        folded = C.c_uint64()
+        ctx.check(ctx.lib.wg_graph_build(ctx.h, int(n), len(src), _p(src), _p(dst), _p(w), _p(rel),
+                                         int(relation_count), 0, C.byref(h), C.byref(folded)))
+        return cls._adopt(ctx, h, int(n), w is not None, int(folded.value))
+
+    @classmethod
+    def _adopt(cls, ctx, h, n, weighted, folded=0):
+        g = cls.__new__(cls)
+        g.ctx = ctx
+        g.n = n
+        g.h = h
+        ctx._children.add(g)
+        nnz = C.c_uint64()
+        ctx.check(ctx.lib.wg_graph_info(h, None, C.byref(nnz), None))
+        g.nnz = int(nnz.value)
+        g.weighted = weighted
+        g.deduplicated = folded
+        return g
+
+    def forward(self):
+        off = np.zeros(self.n + 1, np.uint64)
+        tgt = np.zeros(max(self.nnz, 1), np.uint32)
+        w = np.zeros(max(self.nnz, 1), np.float64) if self.weighted else None
+        self.ctx.check(self.ctx.lib.wg_graph_read_forward(self.ctx.h, self.h, _p(off), _p(tgt),
+                                                          _p(w)))
+        return off, tgt[:self.nnz], None if w is None else w[:self.nnz]
 
     def reverse(self):
         off = np.zeros(self.n + 1, np.uint64)

@@ -77,6 +77,8 @@ SIGNATURES = [
     ("wg_graph_destroy", None, [P]),
     ("wg_graph_info", I32, [P, P, P, P]),
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
+    ("wg_graph_read_forward", I32, [P, P, P, P, P]),
+    ("wg_graph_build", I32, [P, U64, U64, P, P, P, P, U32, I32, P, P]),
     ("wg_graph_update_weights", I32, [P, P, P, P, U64]),
     ("wg_vector_create", I32, [P, I32, U64, U64, P, P]),
     ("wg_vector_destroy", None, [P]),

@@ -200,22 +200,11 @@ static void partition_rows(const std::vector<uint64_t>& off, std::vector<Block>&
   }
 }
 
-static int upload_csr(wg_context* ctx, DevCsr& c, uint64_t n, const uint64_t* off,
-                      const uint32_t* tgt, const double* w) {
-  c.n = n;
-  c.nnz = off[n];
-  c.h_offsets.assign(off, off + n + 1);
+// The SpMV partition of a CSR whose offsets, targets and weights are on the
+// device and whose h_offsets are set: its blocks, split rows and their
+// partial slots.
+static int partition_csr(wg_context* ctx, DevCsr& c) {
   int s;
-  if ((s = dalloc(ctx, (void**)&c.offsets, (n + 1) * 8))) return s;
-  if ((s = dalloc(ctx, (void**)&c.targets, c.nnz * 4))) return s;
-  WG_HIP(ctx, hipMemcpyAsync(c.offsets, off, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-  if (c.nnz)
-    WG_HIP(ctx, hipMemcpyAsync(c.targets, tgt, c.nnz * 4, hipMemcpyHostToDevice, ctx->stream));
-  if (w) {
-    if ((s = dalloc(ctx, (void**)&c.weights, c.nnz * 8))) return s;
-    if (c.nnz)
-      WG_HIP(ctx, hipMemcpyAsync(c.weights, w, c.nnz * 8, hipMemcpyHostToDevice, ctx->stream));
-  }
   std::vector<Block> blocks;
   partition_rows(c.h_offsets, blocks, c.nlong, c.npartials, c.max_degree);
   c.nblocks = (uint32_t)blocks.size();
@@ -236,12 +225,38 @@ static int upload_csr(wg_context* ctx, DevCsr& c, uint64_t n, const uint64_t* of
   if (!lrows.empty())
     WG_HIP(ctx, hipMemcpyAsync(c.long_rows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   WG_HIP(ctx, hipMemcpyAsync(c.long_pbase, lbase.data(), lbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  // The copies read pageable host memory the caller may free on return.
+  // The copies read host memory the caller may free on return.
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return WG_OK;
 }
 
-static void free_csr(wg_context* ctx, DevCsr& c) {
+static int upload_csr(wg_context* ctx, DevCsr& c, uint64_t n, const uint64_t* off,
+                      const uint32_t* tgt, const double* w) {
+  c.n = n;
+  c.nnz = off[n];
+  c.h_offsets.assign(off, off + n + 1);
+  int s;
+  if ((s = dalloc(ctx, (void**)&c.offsets, (n + 1) * 8))) return s;
+  if ((s = dalloc(ctx, (void**)&c.targets, c.nnz * 4))) return s;
+  WG_HIP(ctx, hipMemcpyAsync(c.offsets, off, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (c.nnz)
+    WG_HIP(ctx, hipMemcpyAsync(c.targets, tgt, c.nnz * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (w) {
+    if ((s = dalloc(ctx, (void**)&c.weights, c.nnz * 8))) return s;
+    if (c.nnz)
+      WG_HIP(ctx, hipMemcpyAsync(c.weights, w, c.nnz * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  return partition_csr(ctx, c);
+}
+
+int adopt_device_csr(wg_context* ctx, DevCsr& c) {
+  c.h_offsets.resize(c.n + 1);
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  WG_HIP(ctx, hipMemcpy(c.h_offsets.data(), c.offsets, (c.n + 1) * 8, hipMemcpyDeviceToHost));
+  return partition_csr(ctx, c);
+}
+
+void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.offsets, (c.n + 1) * 8);
   dfree(ctx, c.targets, c.nnz * 4);
   if (c.weights) dfree(ctx, c.weights, c.nnz * 8);
@@ -600,28 +615,8 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint
   g->nnz = off[n];
   g->weighted = w != nullptr;
   s = upload_csr(ctx, g->fwd, n, off, tgt, w);
-  if (!s) {
-    // The reverse CSR by counting-sort transpose, scanning sources in order
-    // so every reverse row ascends (graph/csr/transpose.rs:24-44).
-    uint64_t m = off[n];
-    std::vector<uint64_t> ro(n + 2, 0);
-    std::vector<uint32_t> rt(m);
-    std::vector<double> rw(w ? m : 0);
-    for (uint64_t p = 0; p < m; ++p) ro[tgt[p] + 2]++;
-    for (uint64_t r = 1; r < n + 2; ++r) ro[r] += ro[r - 1];
-    for (uint64_t src = 0; src < n; ++src)
-      for (uint64_t p = off[src]; p < off[src + 1]; ++p) {
-        uint64_t pos = ro[tgt[p] + 1]++;
-        rt[pos] = (uint32_t)src;
-        if (w) rw[pos] = w[p];
-      }
-    ro.pop_back();
-    // A = A^T exactly when the reverse arrays equal the forward ones (an
-    // undirected graph's do; spectral/operator.rs:420-436).
-    g->rev_is_fwd = std::equal(ro.begin(), ro.end(), off) && std::equal(rt.begin(), rt.end(), tgt) &&
-                    (!w || std::equal(rw.begin(), rw.end(), w));
-    s = upload_csr(ctx, g->rev, n, ro.data(), rt.data(), w ? rw.data() : nullptr);
-  }
+  // The reverse CSR, transposed on the device (graph/csr/transpose.rs:24-44).
+  if (!s) s = build_reverse(ctx, g);
   if (s) {
     free_csr(ctx, g->fwd);
     free_csr(ctx, g->rev);
@@ -650,6 +645,18 @@ int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted) 
   return WG_OK;
 }
 
+int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* off, uint32_t* tgt, double* w) {
+  if (!ctx || !g || !off) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  WG_HIP(ctx, hipMemcpy(off, g->fwd.offsets, (g->n + 1) * 8, hipMemcpyDeviceToHost));
+  if (g->nnz) {
+    if (tgt) WG_HIP(ctx, hipMemcpy(tgt, g->fwd.targets, g->nnz * 4, hipMemcpyDeviceToHost));
+    if (w && g->fwd.weights)
+      WG_HIP(ctx, hipMemcpy(w, g->fwd.weights, g->nnz * 8, hipMemcpyDeviceToHost));
+  }
+  return ok(ctx);
+}
+
 int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* off, uint32_t* tgt,
                           double* w) {
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -676,16 +683,8 @@ int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges,
   int s;
   const bool built = g->cached.count(kSlotRevPos) != 0;
   if ((s = graph_buffer(ctx, g, kSlotRevPos, m * 8, &rp))) return s;
-  if (!built) {
-    // Each forward entry's place in the reverse CSR, as the transpose at
-    // upload placed it (sources scanned in order).
-    std::vector<uint32_t> tgt(m);
-    WG_HIP(ctx, hipStreamSynchronize(st));
-    WG_HIP(ctx, hipMemcpy(tgt.data(), g->fwd.targets, m * 4, hipMemcpyDeviceToHost));
-    std::vector<uint64_t> next(g->rev.h_offsets.begin(), g->rev.h_offsets.end() - 1), pos(m);
-    for (uint64_t p = 0; p < m; ++p) pos[p] = next[tgt[p]]++;
-    WG_HIP(ctx, hipMemcpy(rp, pos.data(), m * 8, hipMemcpyHostToDevice));
-  }
+  // Each forward entry's place in the reverse CSR, computed once.
+  if (!built && (s = reverse_position_map(ctx, g, (uint64_t*)rp))) return s;
   void *de, *dv;
   if ((s = dalloc(ctx, &de, count * 8))) return s;
   if ((s = dalloc(ctx, &dv, count * 8))) {

@@ -142,6 +142,14 @@ constexpr int kSlotHotIndex = 190, kSlotHotSrc = 191;
 constexpr int kSlotPprItems = 192, kSlotPprLrows = 193, kSlotPprLbase = 194;
 constexpr int kSlotRevPos = 195;  // forward entry -> reverse position
 int prepare_hot_order(wg_context* ctx, wg_graph* g);
+// A CSR whose offsets, targets and weights were built on the device: reads
+// its offsets back and partitions it for the SpMV.
+int adopt_device_csr(wg_context* ctx, DevCsr& c);
+void free_csr(wg_context* ctx, DevCsr& c);
+// Transposes g->fwd into g->rev on the device and sets g->rev_is_fwd.
+int build_reverse(wg_context* ctx, wg_graph* g);
+// pos[p] = the reverse-CSR position of forward entry p.
+int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos);
 }  // namespace wg
 
 namespace wg {
