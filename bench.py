@@ -256,6 +256,67 @@ def bench_adjoint(W, ctx, scale=20, iters=200, fd_edges=200):
     return res, state
 
 
+def bench_build(W, ctx, scale=22, reps=3):
+    """The device CSR build (wg_graph_build) from the raw R-MAT edge list of the
+    given scale, parallel edges and self loops included, f64 weights, the edge
+    list already in device memory: forward CSR (sort, fold, offsets), reverse
+    CSR (transpose), both partitioned for the SpMV. Minimal bytes: the edge
+    list read once, both CSRs written once."""
+    t0 = time.time()
+    src, dst = W.rmat_edges(scale, 16, seed=1)
+    w = np.random.default_rng(9).random(len(src))
+    n, m = 1 << scale, len(src)
+    log(f"build: raw edges {m} ({time.time() - t0:.1f}s)")
+    dsrc, ddst, dw = W.Vector(ctx, src), W.Vector(ctx, dst), W.Vector(ctx, w)
+    lib = ctx.lib
+    times, h = [], None
+    folded = C.c_uint64()
+    for r in range(reps + 1):
+        if h is not None:
+            lib.wg_graph_destroy(h)
+        h = C.c_void_p()
+        t1 = time.perf_counter()
+        ctx.check(lib.wg_graph_build(ctx.h, n, m, dsrc.device_ptr, ddst.device_ptr, dw.device_ptr,
+                                     None, 1, 1, C.byref(h), C.byref(folded)))
+        if r:  # the first build pays one-time library set-up
+            times.append(time.perf_counter() - t1)
+    g = W.Graph._adopt(ctx, h, n, True, int(folded.value))
+    fwd, rev = g.forward(), g.reverse()
+    nnz = g.nnz
+    g.close()
+    for v in (dsrc, ddst, dw):
+        v.close()
+    ms = min(times) * 1e3
+    byt = m * (4 + 4 + 8) + 2 * ((n + 1) * 8 + nnz * (4 + 8))
+    res = {"scale": scale, "edges_in": m, "nnz": nnz, "deduplicated": int(folded.value),
+           "ms_best_of_%d" % reps: round(ms, 2), "ms_mean": round(float(np.mean(times)) * 1e3, 2),
+           "medges_per_s": round(m / ms / 1e3, 1), "min_bytes": byt,
+           "gbs_min_bytes": round(byt / ms / 1e6, 1)}
+    log(f"build: {m} edges -> {nnz} entries in {ms:.1f} ms")
+    return res, {"n": n, "src": src, "dst": dst, "w": w, "fwd": fwd, "rev": rev}
+
+
+def build_oracle_errors(st):
+    """The oracle's build and transpose of the same edge list (one thread),
+    compared bit for bit; its time is the CPU side of the build leg."""
+    import oracle as O
+    O.build()
+    t = time.perf_counter()
+    ref, folded = O.csr_from_edge_list(st["n"], st["src"], st["dst"], st["w"])
+    t_build = time.perf_counter() - t
+    t = time.perf_counter()
+    rref = ref.transpose()
+    t_tr = time.perf_counter() - t
+    off, tgt, w = st["fwd"]
+    roff, rtgt, rw = st["rev"]
+    same = (np.array_equal(off, ref.offsets) and np.array_equal(tgt, ref.targets)
+            and np.array_equal(w.view(np.uint64), ref.weights.view(np.uint64))
+            and np.array_equal(roff, rref.offsets) and np.array_equal(rtgt, rref.targets)
+            and np.array_equal(rw.view(np.uint64), rref.weights.view(np.uint64)))
+    return {"bit_exact_forward_and_reverse": bool(same), "oracle_build_s_1_thread": round(t_build, 2),
+            "oracle_transpose_s_1_thread": round(t_tr, 2)}
+
+
 def bench_lanczos(W, ctx, scale=21, k=32, basis=128):
     """The k smallest normalised-Laplacian eigenpairs of symmetrised R-MAT
     `scale` (f64 basis of `basis` vectors, tol 1e-12): on the whole graph,
@@ -381,6 +442,8 @@ def main():
                     help="leave out the batched PageRank and multi-source BFS legs")
     ap.add_argument("--skip-adjoint", action="store_true",
                     help="leave out the PageRank adjoint leg (R-MAT two scales below)")
+    ap.add_argument("--skip-build", action="store_true",
+                    help="leave out the device CSR build leg")
     ap.add_argument("--skip-lanczos", action="store_true",
                     help="leave out the Lanczos leg (symmetrised R-MAT one scale below)")
     ap.add_argument("--only-spmv", action="store_true",
@@ -407,7 +470,9 @@ def main():
     xb = (rng.random(n) < 0.5).astype(np.uint8)
 
     ctx = W.Context(0)
+    t_up = time.perf_counter()
     gf = W.Graph(ctx, n, off, tgt, wf)
+    t_up = time.perf_counter() - t_up
     gi = W.Graph(ctx, n, off, tgt, wi)
     log(f"uploaded ({time.time() - t0:.1f}s)")
     vx = {"f32": W.Vector(ctx, xf), "u32": W.Vector(ctx, xu), "bool": W.Vector(ctx, xb)}
@@ -509,7 +574,16 @@ def main():
     if not args.skip_batched:
         legs["ppr_batched_256"] = bench_batched_ppr(W, ctx, gf, n, deg)
         legs["msbfs_512"] = bench_msbfs(W, ctx, gf, n, deg)
-    adj_state = lz_state = None
+    # The first upload also pays the library's one-time set-up: time a second.
+    t_up2 = time.perf_counter()
+    W.Graph(ctx, n, off, tgt, wf).close()
+    t_up2 = time.perf_counter() - t_up2
+    legs["graph_upload"] = {"ms": round(t_up2 * 1e3, 1), "ms_first": round(t_up * 1e3, 1),
+                            "what": "host CSR (u64 offsets, u32 targets, f64 weights) copied in, "
+                                    "reverse CSR transposed on the device, both partitioned"}
+    build_state = adj_state = lz_state = None
+    if not args.skip_build:
+        legs["csr_build"], build_state = bench_build(W, ctx, scale=args.scale)
     if not args.skip_adjoint:
         legs["pagerank_adjoint"], adj_state = bench_adjoint(W, ctx, scale=max(args.scale - 2, 8))
     if not args.skip_lanczos:
@@ -548,6 +622,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(off, tgt, wf, xf)
         result["errors_vs_oracle"] = oracle_errors(off, tgt, wf, wi, xf, xu, xb, outputs,
                                                    bfs_depths, pr_gpu, iters)
+        if build_state is not None:
+            result["errors_vs_oracle"]["csr_build"] = build_oracle_errors(build_state)
         if adj_state is not None:
             result["errors_vs_oracle"]["pagerank_adjoint"] = adjoint_oracle_errors(adj_state, 16)
         if lz_state is not None:

@@ -25,6 +25,11 @@ int wg_rmat_csr(uint32_t scale, uint32_t edge_factor, double a, double b, double
                 uint64_t seed, int symmetrize, uint64_t* offsets, uint32_t* targets,
                 uint64_t capacity, uint64_t* nnz_out, int threads);
 
+/* The raw edge list wg_rmat_csr starts from: edge_factor * 2^scale edges
+ * in generation order, self loops and parallel edges included. */
+int wg_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                  uint64_t seed, uint32_t* sources, uint32_t* targets, int threads);
+
 /* Edge weights as a function of (seed, source, target) alone:
  * kind 0: uniform in (0, 1], k * 2^-24 for k in 1..2^24 (exact in f32);
  * kind 1: integers uniform in [p0, p1];

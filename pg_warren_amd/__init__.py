@@ -520,6 +520,19 @@ def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetrize=False
     return off, tgt[:int(nnz[0])].copy()
 
 
+def rmat_edges(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, threads=0):
+    """The raw R-MAT edge list ``rmat`` starts from, in generation order: self
+    loops and parallel edges included. Returns (sources, targets), u32."""
+    lib = _lib.load()
+    m = edge_factor << scale
+    src = np.zeros(m, np.uint32)
+    dst = np.zeros(m, np.uint32)
+    code = lib.wg_rmat_edges(scale, edge_factor, a, b, c, seed, _p(src), _p(dst), threads)
+    if code:
+        raise WarrenError(code, "rmat_edges")
+    return src, dst
+
+
 WEIGHT_KINDS = {"uniform01": 0, "int": 1, "lognormal": 2}
 
 

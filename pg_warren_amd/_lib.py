@@ -100,6 +100,7 @@ SIGNATURES = [
     ("wg_msbfs", I32, [P, P, I32, P, U32, I64, P, P, P, P]),
     ("wg_device_copy", I32, [P, P, P, U64]),
     ("wg_rmat_csr", I32, [U32, U32, D, D, D, U64, I32, P, P, U64, P, I32]),
+    ("wg_rmat_edges", I32, [U32, U32, D, D, D, U64, P, P, I32]),
     ("wg_edge_weights", I32, [U64, P, P, U64, I32, D, D, P, I32]),
 ]
 

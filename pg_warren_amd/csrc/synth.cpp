@@ -45,16 +45,10 @@ void parallel_for(uint64_t n, int threads, F f) {
 
 extern "C" {
 
-int wg_rmat_csr(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
-                int symmetrize, uint64_t* offsets, uint32_t* targets, uint64_t capacity,
-                uint64_t* nnz_out, int threads) {
-  if (scale == 0 || scale > 31) return WG_INVALID_ARGUMENT;
-  const uint64_t n = 1ull << scale;
-  const uint64_t m = (uint64_t)ef << scale;
-  const uint64_t total = symmetrize ? 2 * m : m;
-  if (capacity < total) return WG_INVALID_ARGUMENT;
+// The raw R-MAT edge list: edge e's endpoints from its own SplitMix64 stream.
+static void rmat_edges(uint32_t scale, uint64_t m, double a, double b, double c, uint64_t seed,
+                       uint32_t* src, uint32_t* dst, int threads) {
   const double ab = a + b, abc = a + b + c;
-  std::vector<uint32_t> src(m), dst(m);
   const uint64_t key = mix64(seed ^ 0x5eed5eed5eed5eedull);
   parallel_for(m, threads, [&](uint64_t lo, uint64_t hi) {
     for (uint64_t e = lo; e < hi; ++e) {
@@ -79,6 +73,25 @@ int wg_rmat_csr(uint32_t scale, uint32_t ef, double a, double b, double c, uint6
       dst[e] = t;
     }
   });
+}
+
+int wg_rmat_edges(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+                  uint32_t* sources, uint32_t* targets, int threads) {
+  if (scale == 0 || scale > 31 || !sources || !targets) return WG_INVALID_ARGUMENT;
+  rmat_edges(scale, (uint64_t)ef << scale, a, b, c, seed, sources, targets, threads);
+  return WG_OK;
+}
+
+int wg_rmat_csr(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+                int symmetrize, uint64_t* offsets, uint32_t* targets, uint64_t capacity,
+                uint64_t* nnz_out, int threads) {
+  if (scale == 0 || scale > 31) return WG_INVALID_ARGUMENT;
+  const uint64_t n = 1ull << scale;
+  const uint64_t m = (uint64_t)ef << scale;
+  const uint64_t total = symmetrize ? 2 * m : m;
+  if (capacity < total) return WG_INVALID_ARGUMENT;
+  std::vector<uint32_t> src(m), dst(m);
+  rmat_edges(scale, m, a, b, c, seed, src.data(), dst.data(), threads);
   // Counting sort by source (optionally with reversed copies), then sort
   // and deduplicate each row, dropping self loops.
   std::vector<uint64_t> cnt(n + 1, 0);
