@@ -45,7 +45,8 @@ enum wg_status {
   WG_EIGENPAIR_COUNT = 19,     /* SpectralError::EigenpairCount */
   WG_NOT_FINITE = 20,          /* SpectralError::NotFinite */
   WG_EDGE_OUTSIDE_ID_SPACE = 21,       /* GraphError::EdgeOutsideIdSpace (graph/error.rs:91-98) */
-  WG_RELATION_OUTSIDE_DICTIONARY = 22  /* GraphError::RelationOutsideDictionary (:99-104) */
+  WG_RELATION_OUTSIDE_DICTIONARY = 22, /* GraphError::RelationOutsideDictionary (:99-104) */
+  WG_UNKNOWN_RELATION = 23             /* ComputeError::UnknownRelation (compute/error.rs:82-87) */
 };
 
 /* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
@@ -107,10 +108,12 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* offsets,
 void wg_graph_destroy(wg_graph* g);
 /* n, nnz and whether the graph carries weights. */
 int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted);
-/* GraphCsr::build_as (graph/csr/graph_csr.rs:210-221) on the device: the
+/* GraphCsr::build (graph/csr/graph_csr.rs:194-221) on the device: the
  * CSR pair of an edge list of m edges sources[i] -> targets[i] with
  * optional weights[i] and relations[i] (< relation_count; NULL: every edge
- * is relation 0). Rows come out sorted by target; parallel edges of one
+ * is relation 0). With relations and relation_count >= 2 each relation's
+ * CSR pair is built too (for wg_graph_select) and the graph's own pair is
+ * the merged all-relations CSR. Rows come out sorted by target; parallel edges of one
  * relation fold into one, weights summed in input order, and a target
  * reached through several relations appears once, the relations' sums
  * added in relation order (build/deduplication.rs:83-152); self loops stay.
@@ -123,6 +126,22 @@ int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* sour
                    const uint32_t* targets, const double* weights, const uint32_t* relations,
                    uint32_t relation_count, int device_pointers, wg_graph** out,
                    uint64_t* deduplicated);
+/* Relation selection (GraphCsr::selected, graph/csr/graph_csr.rs:320-340,
+ * and the matrix a product reads, compute/product/matrix.rs:126-157,
+ * 247-294): the graph whose CSR pair holds the edges of `relations` (NULL:
+ * every relation), for any entry point (wg_spmv, wg_pagerank, wg_bfs,
+ * wg_eigen, ...). One relation: its CSR pair as built. Several: their rows
+ * merged by column, an edge in several relations one entry whose weight is
+ * their sum in relation order (1 for an unweighted relation's entry), or
+ * with per_relation one entry per relation, in relation order. Built on
+ * first use and kept with g (destroyed with it; wg_graph_destroy on a
+ * selection does nothing). An index not below the relation count is
+ * WG_UNKNOWN_RELATION (the smallest such). */
+int wg_graph_select(wg_context* ctx, wg_graph* g, const uint32_t* relations, uint32_t nrelations,
+                    int per_relation, wg_graph** out);
+/* The relations a graph holds: relation_count of wg_graph_build when 2 or
+ * more, else 1 (the graph is its only relation). */
+int wg_graph_relation_count(const wg_graph* g, uint32_t* count);
 /* The forward CSR (host buffers of n+1, nnz, nnz; targets and weights may
  * be NULL). */
 int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* offsets,
@@ -159,6 +178,11 @@ typedef struct {
   const wg_vector* mask;   /* width-1 value mask of any dtype, or NULL */
   int32_t replace;
   int32_t accumulate;      /* wg_binary_op, or WG_OP_NONE */
+  const uint32_t* relations; /* the relations whose edges make up A; NULL: all
+                                (RelationSelection::All), else nrelations indexes */
+  uint32_t nrelations;
+  int32_t per_relation;    /* an edge in several selected relations is an entry
+                              in each rather than one entry (Product::per_relation) */
 } wg_product;
 
 /* spmv (compute/product/mod.rs:77-92): width-1 x and y of one dtype. */

@@ -77,8 +77,8 @@ def lib():
         L.or_csr_validate.argtypes = [P]
         L.or_csr_build.argtypes = [C.c_uint64, C.c_uint64, P, P, P, P, P, P, P]
         L.or_csr_transpose.argtypes = [P, P, P, P]
-        L.or_csr_build_merged.argtypes = [C.c_uint64, C.c_uint64, P, P, P, P, C.c_uint32, P, P, P,
-                                          P, P, P]
+        L.or_csr_build_merged.argtypes = [C.c_uint64, C.c_uint64, P, P, P, P, C.c_uint32, C.c_int,
+                                          P, P, P, P, P, P]
         L.or_spmv.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int,
                               C.c_int, C.c_int, C.c_uint64, P, P, C.c_int]
         L.or_pagerank.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int, C.c_uint32, P, P, P,
@@ -149,7 +149,8 @@ def csr_from_edges(n, src, dst, weights=None):
     return Csr(n, off, tgt[:k].copy(), None if wo is None else wo[:k].copy())
 
 
-def csr_from_edge_list(n, src, dst, weights=None, relations=None, relation_count=1):
+def csr_from_edge_list(n, src, dst, weights=None, relations=None, relation_count=1,
+                       keep_parallel=False):
     """GraphCsr::build's merged all-relations CSR (graph/csr/build/mod.rs:248-288):
     returns (Csr, folded). Raises OracleError 21 / 22 for the first edge outside
     the id space / the dictionary, with ``.edge`` set to its index."""
@@ -163,14 +164,55 @@ def csr_from_edge_list(n, src, dst, weights=None, relations=None, relation_count
     wo = np.zeros(max(m, 1), np.float64) if w is not None else None
     out = np.zeros(3, np.uint64)
     code = lib().or_csr_build_merged(n, m, _p(src), _p(dst), _p(w), _p(rel), relation_count,
-                                     _p(off), _p(tgt), _p(wo), _p(out[0:]), _p(out[1:]),
-                                     _p(out[2:]))
+                                     int(keep_parallel), _p(off), _p(tgt), _p(wo), _p(out[0:]),
+                                     _p(out[1:]), _p(out[2:]))
     if code != OK:
         err = OracleError(code)
         err.edge = int(out[2])
         raise err
     k = int(out[0])
     return Csr(n, off, tgt[:k].copy(), None if wo is None else wo[:k].copy()), int(out[1])
+
+
+def relation_csrs(n, src, dst, weights, relations, relation_count):
+    """Each relation's own CSR (GraphCsr's relation CSRs): the edges of
+    relation r built on their own."""
+    src, dst = np.asarray(src), np.asarray(dst)
+    relations = np.asarray(relations)
+    out = []
+    for r in range(relation_count):
+        sel = relations == r
+        out.append(csr_from_edge_list(n, src[sel], dst[sel],
+                                      None if weights is None else np.asarray(weights)[sel])[0])
+    return out
+
+
+def select(csrs, relations=None, per_relation=False):
+    """The matrix a product over `relations` reads (compute/product/matrix.rs:
+    247-294), from the relation CSRs `csrs` in relation order: one relation's
+    CSR as stored; several, each row's entries merged by column, ties in
+    relation order, an edge in several relations one entry (weights summed in
+    relation order, 1 for an unweighted relation's entry when any selected
+    relation has weights) or with per_relation one entry each."""
+    sel = sorted(set(range(len(csrs)) if relations is None else relations))
+    if len(sel) == 1:
+        return csrs[sel[0]]
+    n = csrs[0].n
+    weighted = any(csrs[r].weights is not None for r in sel)
+    src, dst, w, rel = [], [], [], []
+    for i, r in enumerate(sel):
+        c = csrs[r]
+        rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(c.offsets.astype(np.int64)))
+        src.append(rows)
+        dst.append(c.targets)
+        rel.append(np.full(len(c.targets), i, np.uint32))
+        if weighted:
+            w.append(c.weights if c.weights is not None else np.ones(len(c.targets)))
+    cat = lambda parts, dt: np.concatenate(parts).astype(dt) if parts else np.zeros(0, dt)
+    g, _ = csr_from_edge_list(n, cat(src, np.uint32), cat(dst, np.uint32),
+                              cat(w, np.float64) if weighted else None, cat(rel, np.uint32),
+                              max(len(sel), 1), keep_parallel=per_relation)
+    return g
 
 
 def spmv(fwd, x, semiring="plus_times", dtype=None, *, rev=None, transpose=False,

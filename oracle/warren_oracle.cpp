@@ -380,7 +380,19 @@ int spmv_typed(const or_csr* fwd, const or_csr* rev, int transpose, const Produc
   return push<E>(fwd, a, xs, ys);
 }
 
-bool valid_csr(const or_csr* c) { return or_csr_validate(c) == OR_OK; }
+// The rows a product reads: offsets monotone from 0, targets in the id space
+// and in column order. Unlike Csr::from_parts a column may repeat: the
+// per-relation matrix holds an entry per relation (compute/product/
+// matrix.rs:276-279).
+bool valid_csr(const or_csr* c) {
+  if (!c || !c->offsets || c->n >= (1ull << 32) || c->offsets[0] != 0) return false;
+  for (uint64_t r = 0; r < c->n; ++r) {
+    if (c->offsets[r + 1] < c->offsets[r]) return false;
+    for (uint64_t p = c->offsets[r]; p < c->offsets[r + 1]; ++p)
+      if (c->targets[p] >= c->n || (p > c->offsets[r] && c->targets[p - 1] > c->targets[p])) return false;
+  }
+  return true;
+}
 
 // ---------------------------------------------------------------------------
 // BFS (traversal/level.rs).
@@ -462,7 +474,7 @@ int or_csr_build(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
 }
 
 int or_csr_build_merged(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
-                        const double* weights, const uint32_t* rel, uint32_t nrel,
+                        const double* weights, const uint32_t* rel, uint32_t nrel, int keep_parallel,
                         uint64_t* offsets_out, uint32_t* targets_out, double* weights_out,
                         uint64_t* nnz_out, uint64_t* folded_out, uint64_t* bad_edge_out) {
   // counted(), build/mod.rs:329-352: per edge, the id space, then the
@@ -494,6 +506,14 @@ int or_csr_build_merged(uint64_t n, uint64_t m, const uint32_t* src, const uint3
       return dst[a] != dst[b] ? dst[a] < dst[b] : relation(a) < relation(b);
     });
     size_t k = 0;
+    if (keep_parallel) {
+      for (uint64_t e : row) {
+        targets_out[out] = dst[e];
+        if (weights_out) weights_out[out] = weights ? weights[e] : 1.0;
+        ++out;
+      }
+      k = row.size();
+    }
     while (k < row.size()) {
       const uint32_t t = dst[row[k]];
       double total = 0.0;

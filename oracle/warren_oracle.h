@@ -76,11 +76,15 @@ int or_csr_build(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
  * each row sorted stably by (target, relation), weights summed per
  * relation in input order and the relation sums added in relation order.
  * relations may be NULL (every edge relation 0). *nnz_out = entries kept,
- * *folded_out = edges folded into an edge of the same relation. */
+ * *folded_out = edges folded into an edge of the same relation.
+ * keep_parallel: no folding, every edge an entry, rows in (target,
+ * relation, input) order -- the per-relation multigraph a product reads
+ * with Product::per_relation (compute/product/matrix.rs:276-279). */
 int or_csr_build_merged(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                         const double* weights, const uint32_t* relations, uint32_t relation_count,
-                        uint64_t* offsets_out, uint32_t* targets_out, double* weights_out,
-                        uint64_t* nnz_out, uint64_t* folded_out, uint64_t* bad_edge_out);
+                        int keep_parallel, uint64_t* offsets_out, uint32_t* targets_out,
+                        double* weights_out, uint64_t* nnz_out, uint64_t* folded_out,
+                        uint64_t* bad_edge_out);
 
 /* The reverse CSR by counting-sort transpose (graph/csr/transpose.rs:16-44).*/
 int or_csr_transpose(const or_csr* fwd, uint64_t* offsets_out, uint32_t* targets_out,

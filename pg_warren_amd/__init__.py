@@ -25,7 +25,7 @@ STATUS_NAMES = {
     12: "node_outside_id_space", 13: "node_filter_id_space", 14: "invalid_csr",
     15: "unsupported", 16: "device_error", 17: "not_symmetric", 18: "non_positive_degree",
     19: "eigenpair_count", 20: "not_finite", 21: "edge_outside_id_space",
-    22: "relation_outside_dictionary",
+    22: "relation_outside_dictionary", 23: "unknown_relation",
 }
 DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
 NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
@@ -230,6 +230,33 @@ class Graph:
         g.deduplicated = folded
         return g
 
+    def select(self, relations=None, per_relation=False):
+        """The graph of a relation selection (GraphCsr::selected and the matrix a
+        product reads, compute/product/matrix.rs:126-157): kept with this graph,
+        usable wherever a graph is."""
+        rels, nrels = None, 0
+        if relations is not None:
+            arr = np.ascontiguousarray(relations, np.uint32).reshape(-1)
+            nrels = arr.size
+            arr = arr if nrels else np.zeros(1, np.uint32)
+            rels = arr.ctypes.data
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.wg_graph_select(self.ctx.h, self.h, rels, nrels, int(per_relation),
+                                                    C.byref(h)))
+        if h.value == self.h.value:
+            return self
+        weighted = C.c_int()
+        self.ctx.check(self.ctx.lib.wg_graph_info(h, None, None, C.byref(weighted)))
+        view = Graph._adopt(self.ctx, h, self.n, bool(weighted.value))
+        view._owner = self  # the view lives as long as this graph
+        return view
+
+    @property
+    def relation_count(self):
+        c = C.c_uint32()
+        self.ctx.check(self.ctx.lib.wg_graph_relation_count(self.h, C.byref(c)))
+        return int(c.value)
+
     def forward(self):
         off = np.zeros(self.n + 1, np.uint64)
         tgt = np.zeros(max(self.nnz, 1), np.uint32)
@@ -271,7 +298,8 @@ class Product:
     compute/product/mod.rs:33-56). weighted=None: if the graph has weights."""
 
     def __init__(self, semiring="plus_times", *, transpose=False, weighted=None, mask=None,
-                 mask_complement=False, replace=False, accumulate=None):
+                 mask_complement=False, replace=False, accumulate=None, relations=None,
+                 per_relation=False):
         self.semiring = semiring
         self.transpose = transpose
         self.weighted = weighted
@@ -279,6 +307,8 @@ class Product:
         self.mask_complement = mask_complement
         self.replace = replace
         self.accumulate = accumulate
+        self.relations = relations  # None: every relation (RelationSelection::All)
+        self.per_relation = per_relation
 
     def c(self):
         acc = -1
@@ -286,11 +316,21 @@ class Product:
             if self.accumulate not in BINARY_OPS:
                 raise WarrenError(11, f"unknown binary op {self.accumulate!r}")
             acc = BINARY_OPS.index(self.accumulate)
+        rels, nrels = None, 0
+        if self.relations is not None:
+            # Kept on the instance: the struct holds a raw pointer to it.
+            self._rels = np.ascontiguousarray(self.relations, np.uint32).reshape(-1)
+            if self._rels.size == 0:
+                self._rels = np.zeros(1, np.uint32)  # a valid pointer for an empty list
+                nrels = 0
+            else:
+                nrels = self._rels.size
+            rels = self._rels.ctypes.data
         return _lib.Product(semiring_index(self.semiring), int(self.transpose),
                             -1 if self.weighted is None else int(self.weighted),
                             int(self.mask_complement),
                             None if self.mask is None else self.mask.h.value,
-                            int(self.replace), acc)
+                            int(self.replace), acc, rels, nrels, int(self.per_relation))
 
 
 def spmv(ctx, graph, product, x, y):

@@ -25,7 +25,8 @@ class Product(C.Structure):
     """wg_product: compute/product/mod.rs:33-56."""
     _fields_ = [("semiring", C.c_int32), ("transpose", C.c_int32), ("weighted", C.c_int32),
                 ("mask_complement", C.c_int32), ("mask", P), ("replace", C.c_int32),
-                ("accumulate", C.c_int32)]
+                ("accumulate", C.c_int32), ("relations", P), ("nrelations", C.c_uint32),
+                ("per_relation", C.c_int32)]
 
 
 class PagerankOpts(C.Structure):
@@ -78,6 +79,8 @@ SIGNATURES = [
     ("wg_graph_info", I32, [P, P, P, P]),
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
     ("wg_graph_read_forward", I32, [P, P, P, P, P]),
+    ("wg_graph_select", I32, [P, P, P, U32, I32, P]),
+    ("wg_graph_relation_count", I32, [P, P]),
     ("wg_graph_build", I32, [P, U64, U64, P, P, P, P, U32, I32, P, P]),
     ("wg_graph_update_weights", I32, [P, P, P, P, U64]),
     ("wg_vector_create", I32, [P, I32, U64, U64, P, P]),

@@ -94,12 +94,12 @@ template <class V>
 __global__ __launch_bounds__(kThreads) void mark_runs(const uint64_t* __restrict__ skeys,
                                                       const V* __restrict__ perm,
                                                       const uint32_t* __restrict__ rel, uint64_t m,
-                                                      V* __restrict__ heads,
+                                                      bool keep, V* __restrict__ heads,
                                                       unsigned long long* __restrict__ runs) {
   __shared__ unsigned long long part[kThreads / 64];
   unsigned long long cnt = 0;
   for (uint64_t k = first_index(); k < m; k += stride()) {
-    const bool h = k == 0 || skeys[k] != skeys[k - 1];
+    const bool h = keep || k == 0 || skeys[k] != skeys[k - 1];
     const bool hr = h || (rel && rel[perm[k]] != rel[perm[k - 1]]);
     heads[k] = h ? 1 : 0;
     cnt += hr ? 1 : 0;
@@ -118,24 +118,27 @@ __global__ __launch_bounds__(kThreads) void mark_runs(const uint64_t* __restrict
 // pos[k] - 1 (pos = inclusive count of run heads): its target, its row, and
 // its weight, each relation's weights summed in input order and those
 // sums added in relation order (relation_sum and merge_weighted,
-// build/deduplication.rs:122-152, 196-206).
+// build/deduplication.rs:122-152, 196-206). With keep, every edge is an
+// entry of its own (a multigraph: Product::per_relation).
 template <class V>
 __global__ __launch_bounds__(kThreads) void fold_runs(const uint64_t* __restrict__ skeys,
                                                       const V* __restrict__ perm,
                                                       const V* __restrict__ pos,
                                                       const uint32_t* __restrict__ rel,
                                                       const double* __restrict__ w, uint64_t m,
-                                                      unsigned b, uint32_t* __restrict__ tgt,
+                                                      unsigned b, bool keep, uint32_t* __restrict__ tgt,
                                                       uint32_t* __restrict__ rowkey,
                                                       double* __restrict__ wout) {
   const uint64_t mask = (1ull << b) - 1;
   for (uint64_t k = first_index(); k < m; k += stride()) {
     const uint64_t key = skeys[k];
-    if (k > 0 && skeys[k - 1] == key) continue;
+    if (!keep && k > 0 && skeys[k - 1] == key) continue;
     const uint64_t u = (uint64_t)pos[k] - 1;
     tgt[u] = (uint32_t)(key & mask);
     rowkey[u] = (uint32_t)(key >> b);
-    if (wout) {
+    if (wout && keep) {
+      wout[u] = w[perm[k]];
+    } else if (wout) {
       double total = 0.0;
       bool first = true;
       uint64_t j = k;
@@ -227,6 +230,29 @@ __global__ __launch_bounds__(kThreads) void reverse_positions(const uint32_t* __
   }
 }
 
+// The edges regrouped by relation: out[k] = in[perm[k]].
+template <class V>
+__global__ __launch_bounds__(kThreads) void gather_edges(const V* __restrict__ perm,
+                                                         const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ dst,
+                                                         const double* __restrict__ w, uint64_t m,
+                                                         uint32_t* __restrict__ osrc,
+                                                         uint32_t* __restrict__ odst,
+                                                         double* __restrict__ ow) {
+  for (uint64_t k = first_index(); k < m; k += stride()) {
+    const uint64_t e = perm[k];
+    osrc[k] = src[e];
+    odst[k] = dst[e];
+    if (ow) ow[k] = w[e];
+  }
+}
+
+// The weight of an entry of an unweighted relation read with weights
+// (compute/product/matrix.rs:270-272).
+__global__ __launch_bounds__(kThreads) void fill_ones(double* __restrict__ w, uint64_t m) {
+  for (uint64_t k = first_index(); k < m; k += stride()) w[k] = 1.0;
+}
+
 // *flag |= 1 if a[i] != b[i] for some i (compared with ==).
 template <class T>
 __global__ __launch_bounds__(kThreads) void differs(const T* __restrict__ a, const T* __restrict__ b,
@@ -279,10 +305,11 @@ int inclusive_count(wg_context* ctx, const V* in, V* out, uint64_t m) {
 
 // The forward CSR of a device edge list into f (offsets, targets, weights;
 // not yet partitioned); *folded = edges folded into an edge of the same
-// relation.
+// relation. keep: no folding, parallel edges stay entries of their own.
 template <class V>
 int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
-                  const double* w, const uint32_t* rel, uint32_t nrel, DevCsr& f, uint64_t* folded) {
+                  const double* w, const uint32_t* rel, uint32_t nrel, DevCsr& f, uint64_t* folded,
+                  bool keep = false) {
   hipStream_t st = ctx->stream;
   const unsigned b = id_bits(n);
   const unsigned grid = grid_of(m);
@@ -337,7 +364,7 @@ int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, 
   {
     Scratch<V> heads;
     if ((s = heads.alloc(ctx, m))) return s;
-    mark_runs<V><<<grid, kThreads, 0, st>>>(skeys.p, perm.p, frel, m, heads.p, runs.p);
+    mark_runs<V><<<grid, kThreads, 0, st>>>(skeys.p, perm.p, frel, m, keep, heads.p, runs.p);
     WG_HIP(ctx, hipGetLastError());
     if ((s = inclusive_count(ctx, (const V*)heads.p, pos.p, m))) return s;
   }
@@ -354,8 +381,8 @@ int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, 
   if ((s = dalloc(ctx, (void**)&f.offsets, (n + 1) * 8))) return s;
   Scratch<uint32_t> rowkey;
   if ((s = rowkey.alloc(ctx, nnz))) return s;
-  fold_runs<V><<<grid, kThreads, 0, st>>>(skeys.p, perm.p, pos.p, frel, w, m, b, f.targets, rowkey.p,
-                                          w ? f.weights : nullptr);
+  fold_runs<V><<<grid, kThreads, 0, st>>>(skeys.p, perm.p, pos.p, frel, w, m, b, keep, f.targets,
+                                          rowkey.p, w ? f.weights : nullptr);
   WG_HIP(ctx, hipGetLastError());
   row_starts<<<grid_of(n + 1), kThreads, 0, st>>>(rowkey.p, nnz, n, f.offsets);
   WG_HIP(ctx, hipGetLastError());
@@ -405,6 +432,62 @@ int transpose_into(wg_context* ctx, const DevCsr& f, DevCsr& r) {
   return WG_OK;
 }
 
+// Each relation's CSR pair (GraphCsr's relation CSRs, graph/csr/build/
+// mod.rs:252-279) as a graph owned by g: the edges grouped by relation with
+// a stable sort (input order kept within a relation), then one build per
+// relation.
+template <class V>
+int build_relations(wg_context* ctx, wg_graph* g, uint64_t n, uint64_t m, const uint32_t* src,
+                    const uint32_t* dst, const double* w, const uint32_t* rel, uint32_t R) {
+  hipStream_t st = ctx->stream;
+  int s;
+  Scratch<uint32_t> bsrc, bdst;
+  Scratch<double> bw;
+  std::vector<uint64_t> starts(R + 1, 0);
+  if ((s = bsrc.alloc(ctx, m)) || (s = bdst.alloc(ctx, m))) return s;
+  if (w && (s = bw.alloc(ctx, m))) return s;
+  if (m) {
+    Scratch<uint32_t> rsorted;
+    Scratch<V> idx, perm;
+    Scratch<uint64_t> dstarts;
+    if ((s = rsorted.alloc(ctx, m)) || (s = idx.alloc(ctx, m)) || (s = perm.alloc(ctx, m)) ||
+        (s = dstarts.alloc(ctx, R + 1)))
+      return s;
+    iota<V><<<grid_of(m), kThreads, 0, st>>>(idx.p, m);
+    WG_HIP(ctx, hipGetLastError());
+    if ((s = sort_pairs(ctx, rel, rsorted.p, (const V*)idx.p, perm.p, m, id_bits(R)))) return s;
+    row_starts<<<grid_of(R + 1), kThreads, 0, st>>>(rsorted.p, m, R, dstarts.p);
+    gather_edges<V><<<grid_of(m), kThreads, 0, st>>>(perm.p, src, dst, w, m, bsrc.p, bdst.p,
+                                                     w ? bw.p : nullptr);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipMemcpyAsync(starts.data(), dstarts.p, (R + 1) * 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipStreamSynchronize(st));
+  }
+  for (uint32_t r = 0; r < R; ++r) {
+    wg_graph* c = new (std::nothrow) wg_graph();
+    if (!c) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+    c->ctx = ctx;
+    c->n = n;
+    c->weighted = w != nullptr;
+    c->owner = g;
+    g->relations.push_back(c);
+    const uint64_t b0 = starts[r], cnt = starts[r + 1] - starts[r];
+    uint64_t folded = 0;
+    s = cnt < (1ull << 32)
+            ? build_forward<uint32_t>(ctx, n, cnt, bsrc.p + b0, bdst.p + b0, w ? bw.p + b0 : nullptr,
+                                      nullptr, 0, c->fwd, &folded)
+            : build_forward<uint64_t>(ctx, n, cnt, bsrc.p + b0, bdst.p + b0, w ? bw.p + b0 : nullptr,
+                                      nullptr, 0, c->fwd, &folded);
+    if (!s) {
+      c->nnz = c->fwd.nnz;
+      s = adopt_device_csr(ctx, c->fwd);
+    }
+    if (!s) s = build_reverse(ctx, c);
+    if (s) return s;
+  }
+  return WG_OK;
+}
+
 }  // namespace
 
 // Transposes g->fwd (partitioned) into g->rev on the device, partitions it
@@ -450,9 +533,118 @@ int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos) {
   return WG_OK;
 }
 
+// The matrix a product over relations reads (compute/product/matrix.rs:
+// 126-157, 247-294): one selected relation's CSR pair as stored; several
+// relations' rows merged by column, an edge in several of them one entry
+// whose weight is their weights' sum in relation order (f64 sums start
+// from the first term: Rust's f64 Sum folds from -0.0), or with
+// per_relation an entry in each, in relation order. An unweighted relation
+// read with weights contributes 1 per entry. The selection is built once
+// and kept with the graph.
+int select_graph(wg_context* ctx, wg_graph* g, const uint32_t* rels, uint32_t k, int per_relation,
+                 wg_graph** out) {
+  *out = g;
+  std::vector<uint32_t> sel;
+  if (rels) {
+    sel.assign(rels, rels + k);
+    std::sort(sel.begin(), sel.end());
+    sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
+    // GraphCsr::selected (graph_csr.rs:320-340): the smallest unknown one.
+    for (uint32_t r : sel)
+      if (r >= g->relation_count)
+        return fail(ctx, WG_UNKNOWN_RELATION, "relation " + std::to_string(r) + " is not in the graph");
+  } else {
+    // Every relation: the merged CSR, unless read per relation.
+    if (!per_relation || g->relations.empty()) return WG_OK;
+    for (uint32_t r = 0; r < g->relation_count; ++r) sel.push_back(r);
+  }
+  if (sel.size() == 1) {
+    if (!g->relations.empty()) *out = g->relations[sel[0]];
+    return WG_OK;
+  }
+  std::string key = per_relation ? "p" : "m";
+  for (uint32_t r : sel) key += ":" + std::to_string(r);
+  auto it = g->selections.find(key);
+  if (it != g->selections.end()) {
+    *out = it->second;
+    return WG_OK;
+  }
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  uint64_t total = 0;
+  bool weighted = false;
+  for (uint32_t r : sel) {
+    total += g->relations[r]->nnz;
+    weighted |= g->relations[r]->weighted;
+  }
+  int s;
+  Scratch<uint32_t> src, dst;
+  Scratch<double> w;
+  if ((s = src.alloc(ctx, total)) || (s = dst.alloc(ctx, total))) return s;
+  if (weighted && (s = w.alloc(ctx, total))) return s;
+  uint64_t base = 0;
+  for (uint32_t r : sel) {
+    const DevCsr& c = g->relations[r]->fwd;
+    if (c.nnz) {
+      if ((s = expand_rows(ctx, c, src.p + base))) return s;
+      WG_HIP(ctx, hipMemcpyAsync(dst.p + base, c.targets, c.nnz * 4, hipMemcpyDeviceToDevice, st));
+      if (weighted && c.weights)
+        WG_HIP(ctx, hipMemcpyAsync(w.p + base, c.weights, c.nnz * 8, hipMemcpyDeviceToDevice, st));
+      else if (weighted)
+        fill_ones<<<grid_of(c.nnz), kThreads, 0, st>>>(w.p + base, c.nnz);
+      WG_HIP(ctx, hipGetLastError());
+    }
+    base += c.nnz;
+  }
+  wg_graph* v = new (std::nothrow) wg_graph();
+  if (!v) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  v->ctx = ctx;
+  v->n = g->n;
+  v->weighted = weighted;
+  v->owner = g;
+  uint64_t folded = 0;
+  // Relation order is input order here, and each relation holds a pair at
+  // most once: the build's runs are the selection's merged entries.
+  s = total < (1ull << 32)
+          ? build_forward<uint32_t>(ctx, g->n, total, src.p, dst.p, weighted ? w.p : nullptr, nullptr, 0,
+                                    v->fwd, &folded, per_relation != 0)
+          : build_forward<uint64_t>(ctx, g->n, total, src.p, dst.p, weighted ? w.p : nullptr, nullptr, 0,
+                                    v->fwd, &folded, per_relation != 0);
+  src.reset();
+  dst.reset();
+  w.reset();
+  if (!s) {
+    v->nnz = v->fwd.nnz;
+    s = adopt_device_csr(ctx, v->fwd);
+  }
+  if (!s) s = build_reverse(ctx, v);
+  if (s) {
+    destroy_graph(v);
+    return s;
+  }
+  g->selections[key] = v;
+  *out = v;
+  return WG_OK;
+}
+
 }  // namespace wg
 
 using namespace wg;
+
+extern "C" int wg_graph_select(wg_context* ctx, wg_graph* g, const uint32_t* relations,
+                               uint32_t nrelations, int per_relation, wg_graph** out) {
+  if (!ctx || !g || !out) return WG_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (nrelations && !relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
+  int s = select_graph(ctx, g, relations, nrelations, per_relation, out);
+  return s ? s : ok(ctx);
+}
+
+extern "C" int wg_graph_relation_count(const wg_graph* g, uint32_t* count) {
+  if (!g || !count) return WG_INVALID_ARGUMENT;
+  *count = g->relation_count;
+  return WG_OK;
+}
 
 extern "C" int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* sources,
                               const uint32_t* targets, const double* weights, const uint32_t* relations,
@@ -516,6 +708,17 @@ extern "C" int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uin
   g->n = n;
   g->weighted = weights != nullptr;
   uint64_t folded = 0;
+  // Two relations or more: each relation's CSR pair, then the merged one
+  // (GraphCsr::build stores both, graph_csr.rs:194-202).
+  if (rel && relation_count >= 2) {
+    g->relation_count = relation_count;
+    s = m < (1ull << 32) ? build_relations<uint32_t>(ctx, g, n, m, src, dst, w, rel, relation_count)
+                         : build_relations<uint64_t>(ctx, g, n, m, src, dst, w, rel, relation_count);
+    if (s) {
+      destroy_graph(g);
+      return s;
+    }
+  }
   s = m < (1ull << 32) ? build_forward<uint32_t>(ctx, n, m, src, dst, w, rel, relation_count, g->fwd, &folded)
                        : build_forward<uint64_t>(ctx, n, m, src, dst, w, rel, relation_count, g->fwd, &folded);
   // The edge list copies are no longer needed.
@@ -529,9 +732,7 @@ extern "C" int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uin
   }
   if (!s) s = build_reverse(ctx, g);
   if (s) {
-    free_csr(ctx, g->fwd);
-    free_csr(ctx, g->rev);
-    delete g;
+    destroy_graph(g);
     return s;
   }
   if (deduplicated) *deduplicated = folded;

@@ -256,6 +256,17 @@ int adopt_device_csr(wg_context* ctx, DevCsr& c) {
   return partition_csr(ctx, c);
 }
 
+void destroy_graph(wg_graph* g) {
+  (void)hipStreamSynchronize(g->ctx->stream);
+  for (wg_graph* r : g->relations) destroy_graph(r);
+  for (auto& v : g->selections) destroy_graph(v.second);
+  free_csr(g->ctx, g->fwd);
+  free_csr(g->ctx, g->rev);
+  for (auto& w : g->workspace) dfree(g->ctx, w.first, w.second);
+  for (auto& c : g->cached) dfree(g->ctx, c.second.first, c.second.second);
+  delete g;
+}
+
 void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.offsets, (c.n + 1) * 8);
   dfree(ctx, c.targets, c.nnz * 4);
@@ -628,13 +639,9 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint
 }
 
 void wg_graph_destroy(wg_graph* g) {
-  if (!g) return;
-  (void)hipStreamSynchronize(g->ctx->stream);
-  free_csr(g->ctx, g->fwd);
-  free_csr(g->ctx, g->rev);
-  for (auto& w : g->workspace) dfree(g->ctx, w.first, w.second);
-  for (auto& c : g->cached) dfree(g->ctx, c.second.first, c.second.second);
-  delete g;
+  // A relation or selection goes with the graph that owns it.
+  if (!g || g->owner) return;
+  destroy_graph(g);
 }
 
 int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted) {

@@ -716,12 +716,18 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
   if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   wg_graph* gm = const_cast<wg_graph*>(g);
+  int s;
+  if (p->nrelations && !p->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
+  if ((p->relations || p->per_relation) &&
+      (s = select_graph(ctx, gm, p->relations, p->nrelations, p->per_relation, &gm)))
+    return s;
   DevCsr& csr = p->transpose ? gm->rev : gm->fwd;
-  const bool weighted = p->weighted < 0 ? g->weighted : p->weighted != 0;
+  // Product::weighted None means "if the graph has weights"; a graph without
+  // them reads every weight as 1 either way (matrix.rs:140-141).
+  const bool weighted = (p->weighted < 0 || p->weighted != 0) && gm->weighted;
   WriteArgs wa{p->mask ? p->mask->data : nullptr, p->mask ? p->mask->dtype : 0,
                p->mask_complement, p->replace, p->accumulate};
   const bool can_fail = x->dtype == WG_I64 || x->dtype == WG_U32;
-  int s;
   if (can_fail && (s = clear_error(ctx))) return s;
   switch (x->dtype) {
     case WG_F32: s = dispatch<float>(ctx, csr, p->semiring, weighted, (const float*)x->data, (float*)y->data, x->width, wa); break;

@@ -132,6 +132,16 @@ struct wg_graph {
   uint32_t* ppr_lrows = nullptr;
   uint32_t* ppr_lbase = nullptr;
   uint32_t ppr_nitems = 0, ppr_npieces = 0, ppr_nlong = 0;
+  // Relations (GraphCsr's relation CSRs, graph/csr/graph_csr.rs:15-153):
+  // with two or more, each relation's CSR pair is a graph of its own and
+  // fwd/rev above are the merged all-relations pair; with one, fwd/rev are
+  // that relation's.
+  uint32_t relation_count = 1;
+  std::vector<wg_graph*> relations;
+  // Relation selections built for products and searches, by key.
+  std::map<std::string, wg_graph*> selections;
+  // Relations and selections are owned by their graph and go with it.
+  wg_graph* owner = nullptr;
 };
 
 namespace wg {
@@ -150,6 +160,12 @@ void free_csr(wg_context* ctx, DevCsr& c);
 int build_reverse(wg_context* ctx, wg_graph* g);
 // pos[p] = the reverse-CSR position of forward entry p.
 int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos);
+// The graph a selection of relations reads (g itself, one of its
+// relations, or a merged selection kept with g); rels NULL: every relation.
+int select_graph(wg_context* ctx, wg_graph* g, const uint32_t* rels, uint32_t k, int per_relation,
+                 wg_graph** out);
+// Frees a graph, its relations and selections.
+void destroy_graph(wg_graph* g);
 }  // namespace wg
 
 namespace wg {

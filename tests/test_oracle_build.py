@@ -100,3 +100,92 @@ def test_brute_force_build_on_integer_weights(seed):
     # Unweighted: the same pattern.
     u, _ = O.csr_from_edge_list(n, src, dst)
     assert set(pairs(u)) == set(naive)
+
+
+def test_raw_rmat_edges_build_to_the_rmat_csr():
+    """The raw generator output, self loops dropped, built by the oracle, is the
+    CSR the R-MAT generator returns (host code of the library; no GPU)."""
+    import pg_warren_amd as W
+    src, dst = W.rmat_edges(12, 16, seed=4)
+    off, tgt = W.rmat(12, 16, seed=4)
+    keep = src != dst
+    g, folded = O.csr_from_edge_list(1 << 12, src[keep], dst[keep])
+    assert np.array_equal(g.offsets, off) and np.array_equal(g.targets, tgt)
+    assert folded == int(keep.sum()) - len(tgt)
+
+
+# ---------------------------------------------------------------- relations
+# compute/product/tests.rs:32-38: relation 0: 0 -> 1 (2), 0 -> 2 (3),
+# 1 -> 2 (4), 2 -> 0 (5); relation 1: 0 -> 1 (10). Node 3 is isolated.
+EX_EDGES = [(0, 0, 1, 2.0), (0, 0, 2, 3.0), (0, 1, 2, 4.0), (0, 2, 0, 5.0), (1, 0, 1, 10.0)]
+EX_X = np.array([1.0, 2.0, 3.0, 4.0])
+
+
+def example_relations():
+    rel, src, dst, w = (np.array(c) for c in zip(*EX_EDGES))
+    return O.relation_csrs(4, src, dst, w, rel, 2)
+
+
+def test_relation_known_answers():
+    csrs = example_relations()
+    # compute/product/tests.rs:60-69: relation 1 alone; every relation
+    # unweighted.
+    assert list(O.spmv(O.select(csrs, [1]), EX_X)) == [20.0, 0.0, 0.0, 0.0]
+    merged = O.select(csrs)
+    assert list(O.spmv(merged, EX_X)) == [33.0, 12.0, 5.0, 0.0]
+    assert list(O.spmv(merged, EX_X, weighted=False)) == [5.0, 3.0, 1.0, 0.0]
+    assert list(O.spmv(merged, EX_X, rev=merged.transpose(), transpose=True)) == [15.0, 12.0, 11.0, 0.0]
+    # Per relation the 0 -> 1 edge counts in both relations: unweighted row 0
+    # is 2 + 2 + 3, and min_plus sees the lighter edge's own path.
+    per = O.select(csrs, per_relation=True)
+    assert list(O.spmv(per, EX_X, weighted=False)) == [7.0, 3.0, 1.0, 0.0]
+    assert O.spmv(per, EX_X, "min_plus")[0] == 4.0
+    assert O.spmv(merged, EX_X, "min_plus")[0] == 6.0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_per_relation_is_each_relations_product_added(seed):
+    """compute/product/tests.rs:611-650 and :651-662, on integer weights and
+    values (every semiring exact)."""
+    rng = np.random.default_rng(seed)
+    n, m, R = 30, 200, 3
+    rel = rng.integers(0, R, m)
+    src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+    w = rng.integers(1, 5, m).astype(np.float64)
+    x = rng.integers(-3, 4, n).astype(np.float64)
+    csrs = O.relation_csrs(n, src, dst, w, rel, R)
+    identity = {"plus_times": 0.0, "min_plus": np.inf, "max_times": -np.inf}
+    for sr in ("plus_times", "min_plus", "max_times"):
+        found = O.spmv(O.select(csrs, per_relation=True), x, sr)
+        parts = [O.spmv(c, x, sr) for c in csrs]
+        has = [np.diff(c.offsets.astype(np.int64)) > 0 for c in csrs]
+        expected = []
+        for row in range(n):
+            vals = [p[row] for p, h in zip(parts, has) if h[row]]
+            if not vals:
+                expected.append(identity[sr])
+                continue
+            acc = vals[0]
+            for v in vals[1:]:
+                acc = acc + v if sr == "plus_times" else (min(acc, v) if sr == "min_plus" else max(acc, v))
+            expected.append(acc)
+        assert list(found) == expected, sr
+        # By default: the product of the graph with every edge in one relation.
+        dedup, _ = O.csr_from_edge_list(n, src, dst, w)
+        assert list(O.spmv(O.select(csrs), x, sr)) == list(O.spmv(dedup, x, sr)), sr
+
+
+def test_merged_csr_and_merged_rows_agree_bit_for_bit():
+    """compute/product/tests.rs:507-530: with fractional weights (-0.0 among
+    them) the stored merged CSR and rows merged on the fly give the same
+    bits, since f64 sums start from their first term."""
+    rng = np.random.default_rng(5)
+    n, m, R = 25, 400, 4
+    rel = rng.integers(0, R, m)
+    src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+    w = rng.choice([0.1, 0.7, 1e-3, 3.0, -0.0], m)
+    stored, _ = O.csr_from_edge_list(n, src, dst, w, relations=rel, relation_count=R)
+    on_the_fly = O.select(O.relation_csrs(n, src, dst, w, rel, R))
+    assert np.array_equal(stored.offsets, on_the_fly.offsets)
+    assert np.array_equal(stored.targets, on_the_fly.targets)
+    assert np.array_equal(stored.weights.view(np.uint64), on_the_fly.weights.view(np.uint64))
