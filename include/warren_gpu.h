@@ -304,6 +304,25 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const ui
                      uint64_t nstarts, int64_t max_depth, uint64_t* out, uint64_t* count,
                      wg_traversal_stats* stats);
 
+/* BfsCursor (traversal/bfs_cursor.rs:23-135): the search of wg_bfs (or,
+ * with reachable, of wg_reachable_set: no filter or result limit, a pulled
+ * level listed by node) handed out one level per wg_bfs_cursor_next call,
+ * the starts first. Each level's visits are sorted and written on the
+ * device and reach out (host, room for the graph's n visits) in one copy;
+ * *more = 0 once no level follows (the search is exhausted, max_depth
+ * levels are done, or max_results visits were handed out and another was
+ * found: stats.truncated). The cursor keeps its own workspace, so several
+ * may be open on one graph; a relation subset is the graph
+ * wg_graph_select returns. Arguments are checked at creation. */
+typedef struct wg_bfs_cursor wg_bfs_cursor;
+int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
+                         uint64_t nstarts, const uint8_t* filter, int64_t max_depth,
+                         int64_t max_results, int reachable, wg_bfs_cursor** out);
+int wg_bfs_cursor_next(wg_context* ctx, wg_bfs_cursor* cursor, wg_visit* out, uint64_t* count,
+                       int* more);
+int wg_bfs_cursor_stats(const wg_bfs_cursor* cursor, wg_traversal_stats* stats);
+void wg_bfs_cursor_destroy(wg_bfs_cursor* cursor);
+
 /* Multi-source BFS: SourceBatch::search (compute/centrality/multi_source.rs:
  * 34-193) widened from 64 sources to up to 512, one bit per source. Every
  * node's distance from every source along direction, level by level until no

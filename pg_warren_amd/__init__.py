@@ -468,6 +468,62 @@ def reachable_set(ctx, graph, starts, *, direction="out", max_depth=None):
     return out[:cnt.value].copy(), _stats(stats)
 
 
+class BfsCursor:
+    """BfsCursor (traversal/bfs_cursor.rs:23-135): the search of ``bfs`` (or,
+    with reachable=True, of ``reachable_set``) one level per ``next_level``
+    call, the starts first. next_level returns (visits, more): visits as
+    [(node, depth, via or None)] and more False once no level follows."""
+
+    def __init__(self, ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
+                 max_results=None, reachable=False):
+        self.ctx, self.graph = ctx, graph
+        st = _starts(starts)
+        flt = _filter(node_filter)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wg_bfs_cursor_create(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+                                               _p(flt), -1 if max_depth is None else max_depth,
+                                               -1 if max_results is None else max_results,
+                                               int(reachable), C.byref(h)))
+        self.h = h
+        self._out = (_lib.Visit * max(graph.n, 1))()
+        ctx._children.add(self)
+
+    def next_level(self):
+        cnt = C.c_uint64()
+        more = C.c_int()
+        self.ctx.check(self.ctx.lib.wg_bfs_cursor_next(self.ctx.h, self.h, C.addressof(self._out),
+                                                       C.byref(cnt), C.byref(more)))
+        arr = np.ctypeslib.as_array(self._out)[:cnt.value]
+        visits = [(int(v["node"]), int(v["depth"]), None if v["via"] == 2**64 - 1 else int(v["via"]))
+                  for v in arr]
+        return visits, bool(more.value)
+
+    def levels(self):
+        """Every remaining level, as lists of visits (empty levels left out)."""
+        out, more = [], True
+        while more:
+            level, more = self.next_level()
+            if level:
+                out.append(level)
+        return out
+
+    def stats(self):
+        stats = _lib.TraversalStats()
+        self.ctx.check(self.ctx.lib.wg_bfs_cursor_stats(self.h, C.byref(stats)))
+        return _stats(stats)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h is not None:
+            self.ctx.lib.wg_bfs_cursor_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
                pulling=True, depth=None, via=None):
     """The device-resident search: (depth, via) u32 Vectors (0xffffffff where

@@ -80,6 +80,10 @@ SIGNATURES = [
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
     ("wg_graph_read_forward", I32, [P, P, P, P, P]),
     ("wg_graph_select", I32, [P, P, P, U32, I32, P]),
+    ("wg_bfs_cursor_create", I32, [P, P, I32, P, U64, P, C.c_int64, C.c_int64, I32, P]),
+    ("wg_bfs_cursor_next", I32, [P, P, P, P, P]),
+    ("wg_bfs_cursor_stats", I32, [P, P]),
+    ("wg_bfs_cursor_destroy", None, [P]),
     ("wg_graph_relation_count", I32, [P, P]),
     ("wg_graph_build", I32, [P, U64, U64, P, P, P, P, U32, I32, P, P]),
     ("wg_graph_update_weights", I32, [P, P, P, P, U64]),

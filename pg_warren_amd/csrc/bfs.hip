@@ -347,8 +347,8 @@ struct BfsRun {
   uint32_t levels = 0;
 };
 
-// The search's workspace, kept with the graph: bitmaps, queues, candidate
-// bytes, split-row piece hits, compaction partials and counters.
+// The search's workspace: bitmaps, queues, candidate bytes, split-row piece
+// hits, compaction partials and counters.
 struct BfsWorkspace {
   uint32_t *visited, *bm_a, *bm_b, *q_a, *q_b, *piece_hit;
   uint8_t* cand;
@@ -358,114 +358,183 @@ struct BfsWorkspace {
 
 static uint32_t edge_parts(const DevCsr& c) { return c.nblocks + (c.nlong + 255) / 256; }
 
-static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorkspace& w) {
+// Allocates a workspace's eight buffers into slot[], recording them in
+// `owned` (charged to the context).
+static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, void* slot[8],
+                           std::vector<std::pair<void*, size_t>>& owned) {
   const uint64_t nbc = (words + kWordsPerCountBlock - 1) / kWordsPerCountBlock;
-  if (!g->bfs_ws[0]) {
-    const uint64_t n = std::max<uint64_t>(g->n, 1);
-    const size_t sizes[8] = {3 * words * 4,
-                             2 * n * 4,
-                             std::max<uint32_t>(std::max(g->fwd.npartials, g->rev.npartials), 1) * 4,
-                             n,
-                             std::max<uint64_t>(nbc, 1) * 3 * 8,
-                             std::max<uint64_t>(nbc, 1) * 8,
-                             std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8,
-                             sizeof(BfsCounters)};
-    for (int k = 0; k < 8; ++k) {
-      void* p = nullptr;
-      int s = dalloc(ctx, &p, sizes[k]);
-      if (s) return s;
-      g->workspace.push_back({p, sizes[k]});
-      g->bfs_ws[k] = p;
-    }
-    // Candidate bytes start clear and every resolve clears what it reads.
-    WG_HIP(ctx, hipMemsetAsync(g->bfs_ws[3], 0, n, ctx->stream));
+  const uint64_t n = std::max<uint64_t>(g->n, 1);
+  const size_t sizes[8] = {3 * words * 4,
+                           2 * n * 4,
+                           std::max<uint32_t>(std::max(g->fwd.npartials, g->rev.npartials), 1) * 4,
+                           n,
+                           std::max<uint64_t>(nbc, 1) * 3 * 8,
+                           std::max<uint64_t>(nbc, 1) * 8,
+                           std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8,
+                           sizeof(BfsCounters)};
+  for (int k = 0; k < 8; ++k) {
+    void* p = nullptr;
+    int s = dalloc(ctx, &p, sizes[k]);
+    if (s) return s;
+    owned.push_back({p, sizes[k]});
+    slot[k] = p;
   }
-  uint32_t* bms = (uint32_t*)g->bfs_ws[0];
-  uint32_t* qs = (uint32_t*)g->bfs_ws[1];
-  w = BfsWorkspace{bms, bms + words, bms + 2 * words, qs, qs + std::max<uint64_t>(g->n, 1),
-                   (uint32_t*)g->bfs_ws[2], (uint8_t*)g->bfs_ws[3],
-                   (unsigned long long*)g->bfs_ws[4], (unsigned long long*)g->bfs_ws[5],
-                   (unsigned long long*)g->bfs_ws[6], (BfsCounters*)g->bfs_ws[7]};
+  // Candidate bytes start clear and every resolve clears what it reads.
+  WG_HIP(ctx, hipMemsetAsync(slot[3], 0, n, ctx->stream));
   return WG_OK;
 }
 
-// Runs the search, leaving depth and via (kUnvisited where unreached) in
-// device buffers. starts must be in range and deduplicated, in order.
-// Stops once more than `stop_after` nodes are visited (to cut a search
-// whose output is truncated anyway) unless stop_after < 0.
-int bfs_run(wg_context* ctx, wg_graph* g, int direction, const std::vector<uint32_t>& starts,
-            const uint8_t* filter_d, int64_t max_depth, int64_t stop_after, bool pulling,
-            uint32_t* depth, uint32_t* via, BfsRun& run) {
-  const uint64_t n = g->n;
-  DevCsr& adj = direction == WG_IN ? g->rev : g->fwd;
-  DevCsr& rev = direction == WG_IN ? g->fwd : g->rev;
-  hipStream_t st = ctx->stream;
-  const uint64_t words = (n + 63) / 64 * 2;  // 32-bit words, whole 64-node waves
-  const uint32_t nbc = (uint32_t)((words + kWordsPerCountBlock - 1) / kWordsPerCountBlock);
-  BfsWorkspace w;
-  int s = bfs_workspace(ctx, g, words, w);
-  if (s) return s;
-  WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
-  WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
-  WG_HIP(ctx, hipMemsetAsync(w.visited, 0, 3 * words * 4, st));  // visited, bm_a, bm_b
-  uint64_t mf = 0, visited_rev = 0;
-  if (!starts.empty()) {
-    WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
-    bfs_seed<<<(unsigned)std::min<size_t>((starts.size() + 255) / 256, 1024), 256, 0, st>>>(
-        w.q_a, (uint32_t)starts.size(), depth, w.visited, w.bm_a);
-    for (uint32_t v : starts) {
-      mf += adj.h_offsets[v + 1] - adj.h_offsets[v];
-      visited_rev += rev.h_offsets[v + 1] - rev.h_offsets[v];
-    }
+static void view_workspace(const wg_graph* g, void* const slot[8], uint64_t words, BfsWorkspace& w) {
+  uint32_t* bms = (uint32_t*)slot[0];
+  uint32_t* qs = (uint32_t*)slot[1];
+  w = BfsWorkspace{bms, bms + words, bms + 2 * words, qs, qs + std::max<uint64_t>(g->n, 1),
+                   (uint32_t*)slot[2], (uint8_t*)slot[3], (unsigned long long*)slot[4],
+                   (unsigned long long*)slot[5], (unsigned long long*)slot[6], (BfsCounters*)slot[7]};
+}
+
+// The graph's own workspace, allocated on first use and kept with it.
+static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorkspace& w) {
+  if (!g->bfs_ws[0]) {
+    int s = alloc_workspace(ctx, g, words, g->bfs_ws, g->workspace);
+    if (s) return s;
   }
-  uint64_t qn = starts.size();
-  run.visited = qn;
-  run.level_size.assign(1, qn);
+  view_workspace(g, g->bfs_ws, words, w);
+  return WG_OK;
+}
+
+// The level loop of the search, one level per step, its state kept between
+// steps (LevelSearch, traversal/level.rs:88-337). After init the frontier
+// is the starts; each step expands it into the next level, whose nodes are
+// then the frontier bitmap (frontier_size of them) and whose depth is d.
+struct BfsSearch {
+  wg_context* ctx = nullptr;
+  wg_graph* g = nullptr;
+  DevCsr* adj = nullptr;
+  DevCsr* rev = nullptr;
+  uint64_t n = 0, words = 0;
+  uint32_t nbc = 0;
+  BfsWorkspace w{};
+  const uint8_t* filter = nullptr;
+  bool pulling = true;
+  int64_t max_depth = -1;
+  uint32_t *depth = nullptr, *via = nullptr;
+  uint64_t frontier_size = 0, mf = 0, visited_rev = 0;
   uint32_t d = 0;
-  uint32_t* frontier = w.bm_a;  // bitmap of the current frontier
-  uint32_t* claim = w.bm_b;     // bitmap of the nodes claimed this level
-  bool queue_ready = true;      // q_a holds the frontier as a list
-  BfsCounters hc{};
-  while (qn > 0 && !(max_depth >= 0 && (int64_t)d >= max_depth) &&
-         !(stop_after >= 0 && run.visited > (uint64_t)stop_after)) {
+  uint32_t *frontier = nullptr, *claim = nullptr;
+  bool queue_ready = true;  // q_a holds the frontier as an ascending list
+  BfsRun run;
+
+  int init(wg_context* c, wg_graph* gr, int direction, const std::vector<uint32_t>& starts,
+           const uint8_t* filter_d, int64_t maxd, bool pull_ok, uint32_t* depth_d, uint32_t* via_d,
+           const BfsWorkspace& ws) {
+    ctx = c;
+    g = gr;
+    n = g->n;
+    adj = direction == WG_IN ? &g->rev : &g->fwd;
+    rev = direction == WG_IN ? &g->fwd : &g->rev;
+    words = (n + 63) / 64 * 2;  // 32-bit words, whole 64-node waves
+    nbc = (uint32_t)((words + kWordsPerCountBlock - 1) / kWordsPerCountBlock);
+    w = ws;
+    filter = filter_d;
+    max_depth = maxd;
+    pulling = pull_ok;
+    depth = depth_d;
+    via = via_d;
+    hipStream_t st = ctx->stream;
+    WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
+    WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
+    WG_HIP(ctx, hipMemsetAsync(w.visited, 0, 3 * words * 4, st));  // visited, bm_a, bm_b
+    mf = visited_rev = 0;
+    if (!starts.empty()) {
+      WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
+      bfs_seed<<<(unsigned)std::min<size_t>((starts.size() + 255) / 256, 1024), 256, 0, st>>>(
+          w.q_a, (uint32_t)starts.size(), depth, w.visited, w.bm_a);
+      WG_HIP(ctx, hipGetLastError());
+      for (uint32_t v : starts) {
+        mf += adj->h_offsets[v + 1] - adj->h_offsets[v];
+        visited_rev += rev->h_offsets[v + 1] - rev->h_offsets[v];
+      }
+    }
+    frontier_size = starts.size();
+    run = BfsRun{};
+    run.visited = frontier_size;
+    run.level_size.assign(1, frontier_size);
+    d = 0;
+    frontier = w.bm_a;
+    claim = w.bm_b;
+    queue_ready = true;
+    return WG_OK;
+  }
+
+  // No further level: the frontier is empty or max_depth levels are done.
+  bool exhausted() const { return frontier_size == 0 || (max_depth >= 0 && (int64_t)d >= max_depth); }
+
+  int step() {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+    hipStream_t st = ctx->stream;
     // Beamer's switch (level.rs:184-211): pull iff |F| * 24 > n and the
     // unvisited nodes' reverse entries are fewer than alpha * m_f.
-    const uint64_t m_u = rev.nnz - visited_rev;
-    const bool pull = pulling && qn * 24 > n && m_u < mf;
+    const uint64_t m_u = rev->nnz - visited_rev;
+    const bool pull = pulling && frontier_size * 24 > n && m_u < mf;
     if (!pull) {
-      if (!queue_ready)
-        bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
-      bfs_mark<<<(unsigned)std::min<uint64_t>((qn + 3) / 4, 256 * 64), 256, 0, st>>>(
-          w.q_a, qn, adj.offsets, adj.targets, w.visited, filter_d, w.cand);
+      if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
+      bfs_mark<<<(unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64), 256, 0, st>>>(
+          w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter, w.cand);
     }
     const int cands = pull ? 0 : 1;
-    if (rev.nblocks)
-      bfs_pull_blocks<<<rev.nblocks, kBlockThreads, 0, st>>>(
-          rev.blocks, rev.offsets, rev.targets, w.visited, frontier, filter_d, w.cand, cands, depth,
-          via, claim, w.piece_hit, w.part_edges, d + 1);
-    if (rev.nlong)
-      bfs_pull_long<<<(rev.nlong + 255) / 256, 256, 0, st>>>(
-          rev.long_rows, rev.long_pbase, rev.nlong, rev.offsets, rev.targets, w.visited, filter_d,
-          w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev.nblocks, d + 1);
-    bfs_count<<<nbc, 256, 0, st>>>(words, claim, w.visited, frontier, adj.offsets, rev.offsets, w.part);
-    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(rev) : 0, w.cnt);
+    if (rev->nblocks)
+      bfs_pull_blocks<<<rev->nblocks, kBlockThreads, 0, st>>>(
+          rev->blocks, rev->offsets, rev->targets, w.visited, frontier, filter, w.cand, cands, depth, via,
+          claim, w.piece_hit, w.part_edges, d + 1);
+    if (rev->nlong)
+      bfs_pull_long<<<(rev->nlong + 255) / 256, 256, 0, st>>>(
+          rev->long_rows, rev->long_pbase, rev->nlong, rev->offsets, rev->targets, w.visited, filter,
+          w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev->nblocks, d + 1);
+    bfs_count<<<nbc, 256, 0, st>>>(words, claim, w.visited, frontier, adj->offsets, rev->offsets, w.part);
+    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(*rev) : 0, w.cnt);
     WG_HIP(ctx, hipGetLastError());
+    BfsCounters hc{};
     WG_HIP(ctx, hipMemcpyAsync(&hc, w.cnt, sizeof hc, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipStreamSynchronize(st));
     std::swap(frontier, claim);
     queue_ready = false;
     // A pushed level scans every entry of every frontier row.
     run.edges += pull ? hc.edges : mf;
-    qn = hc.next_size;
+    frontier_size = hc.next_size;
     mf = hc.next_entries;
     visited_rev += hc.visited_rev;
-    run.visited += qn;
+    run.visited += frontier_size;
     run.level_pulled.push_back(pull ? 1 : 0);
-    run.level_size.push_back(qn);
+    run.level_size.push_back(frontier_size);
     ++d;
+    run.levels = d;
+    return WG_OK;
   }
-  run.levels = d;
+
+  // The current level's nodes, ascending, into q_a (the next push's queue).
+  int list_frontier() {
+    if (!queue_ready && frontier_size)
+      bfs_write<<<nbc, 256, 0, ctx->stream>>>(words, frontier, w.offsets, w.q_a);
+    WG_HIP(ctx, hipGetLastError());
+    queue_ready = true;
+    return WG_OK;
+  }
+};
+
+// Runs the search, leaving depth and via (kUnvisited where unreached) in
+// device buffers. starts must be in range and deduplicated, in order.
+int bfs_run(wg_context* ctx, wg_graph* g, int direction, const std::vector<uint32_t>& starts,
+            const uint8_t* filter_d, int64_t max_depth, bool pulling, uint32_t* depth, uint32_t* via,
+            BfsRun& run) {
+  BfsWorkspace w;
+  const uint64_t words = (g->n + 63) / 64 * 2;
+  int s = bfs_workspace(ctx, g, words, w);
+  if (s) return s;
+  BfsSearch S;
+  if ((s = S.init(ctx, g, direction, starts, filter_d, max_depth, pulling, depth, via, w))) return s;
+  while (!S.exhausted())
+    if ((s = S.step())) return s;
+  run = S.run;
   return WG_OK;
 }
 
@@ -490,11 +559,159 @@ struct DevBytes {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBytes() { dfree(ctx, p, bytes); }
+  int alloc(size_t b) {
+    bytes = std::max<size_t>(b, 1);
+    return dalloc(ctx, &p, bytes);
+  }
+};
+
+// keys[i] = via[nodes[i]]: a level's nodes keyed by their via.
+__global__ __launch_bounds__(256) void via_keys(const uint32_t* __restrict__ nodes,
+                                                const uint32_t* __restrict__ via, uint64_t count,
+                                                uint32_t* __restrict__ keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] = via[nodes[i]];
+}
+
+// The level's visits in emission order.
+__global__ __launch_bounds__(256) void write_visits(const uint32_t* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ via, uint64_t count,
+                                                    uint32_t depth, wg_visit* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = nodes[i];
+    out[i] = wg_visit{v, via[v], depth, 0};
+  }
+}
+
+// A search that hands out its visits level by level, in the reference's
+// order: the starts in the order given; a pushed level by (via, node), the
+// order reached when frontier nodes expand in ascending order; a pulled
+// level by node when `ascending_pulls` (a reachable set), else also by
+// (via, node). Visits are sorted and written on the device; a level's rows
+// reach the host in one copy. At most max_results visits (< 0: no limit);
+// a visit beyond them sets truncated.
+struct LevelEmitter {
+  wg_context* ctx = nullptr;
+  wg_graph* g = nullptr;
+  std::vector<std::pair<void*, size_t>> owned;  // workspace and buffers
+  void* slot[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  BfsSearch S;
+  std::vector<uint32_t> starts;
+  uint32_t *depth = nullptr, *via = nullptr, *keys_in = nullptr, *keys_out = nullptr;
+  uint8_t* filter = nullptr;
+  wg_visit* visits = nullptr;
+  int64_t max_results = -1;
+  bool ascending_pulls = false, seeded = false, finished = false, truncated = false;
+  uint64_t emitted = 0;
+
+  ~LevelEmitter() {
+    if (ctx) {
+      (void)hipStreamSynchronize(ctx->stream);
+      for (auto& b : owned) dfree(ctx, b.first, b.second);
+    }
+  }
+
+  int buffer(size_t bytes, void** p) {
+    int s = dalloc(ctx, p, std::max<size_t>(bytes, 1));
+    if (!s) owned.push_back({*p, std::max<size_t>(bytes, 1)});
+    return s;
+  }
+
+  int init(wg_context* c, wg_graph* gr, int direction, const uint64_t* st, uint64_t ns,
+           const uint8_t* host_filter, int64_t max_depth, int64_t maxr, bool pulling, bool asc) {
+    ctx = c;
+    g = gr;
+    max_results = maxr;
+    ascending_pulls = asc;
+    int s = prepare_starts(ctx, g->n, st, ns, starts);
+    if (s) return s;
+    const uint64_t n = std::max<uint64_t>(g->n, 1);
+    const uint64_t words = (g->n + 63) / 64 * 2;
+    if ((s = alloc_workspace(ctx, g, words, slot, owned))) return s;
+    if ((s = buffer(n * 4, (void**)&depth)) || (s = buffer(n * 4, (void**)&via)) ||
+        (s = buffer(n * 4, (void**)&keys_in)) || (s = buffer(n * 4, (void**)&keys_out)) ||
+        (s = buffer(n * sizeof(wg_visit), (void**)&visits)))
+      return s;
+    if (host_filter) {
+      if ((s = buffer(n, (void**)&filter))) return s;
+      WG_HIP(ctx, hipMemcpy(filter, host_filter, g->n, hipMemcpyHostToDevice));
+    }
+    BfsWorkspace w;
+    view_workspace(g, slot, words, w);
+    return S.init(ctx, g, direction, starts, filter, max_depth, pulling, depth, via, w);
+  }
+
+  // The room left under max_results.
+  uint64_t room() const {
+    return max_results < 0 ? UINT64_MAX : (uint64_t)max_results - std::min<uint64_t>(emitted, max_results);
+  }
+
+  // The next level's visits into out (host, room for the graph's n);
+  // *count of them. Returns with *more = 0 once nothing follows.
+  int next(wg_visit* out, uint64_t* count, int* more) {
+    *count = 0;
+    if (finished) {
+      *more = 0;
+      return WG_OK;
+    }
+    int s;
+    uint64_t size = 0;
+    if (!seeded) {
+      seeded = true;
+      size = starts.size();
+      const uint64_t k = std::min<uint64_t>(size, room());
+      for (uint64_t i = 0; i < k; ++i) out[i] = wg_visit{starts[i], UINT64_MAX, 0, 0};
+      *count = k;
+    } else {
+      if ((s = S.step())) return s;
+      size = S.frontier_size;
+      const uint64_t k = std::min<uint64_t>(size, room());
+      if (k) {
+        if ((s = S.list_frontier())) return s;
+        const uint32_t* nodes = S.w.q_a;
+        const bool by_node = ascending_pulls && S.run.level_pulled.back();
+        const unsigned grid = (unsigned)std::min<uint64_t>((size + 255) / 256, 4096);
+        if (!by_node) {
+          via_keys<<<grid, 256, 0, ctx->stream>>>(S.w.q_a, via, size, keys_in);
+          WG_HIP(ctx, hipGetLastError());
+          // Stable: equal vias keep ascending node order.
+          unsigned bits = 1;
+          while (bits < 32 && (1ull << bits) < g->n) ++bits;
+          if ((s = sort_u32_pairs(ctx, keys_in, keys_out, S.w.q_a, S.w.q_b, size, bits))) return s;
+          nodes = S.w.q_b;
+        }
+        write_visits<<<grid, 256, 0, ctx->stream>>>(nodes, via, k, S.d, visits);
+        WG_HIP(ctx, hipGetLastError());
+        WG_HIP(ctx, hipMemcpyAsync(out, visits, k * sizeof(wg_visit), hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      }
+      *count = k;
+    }
+    emitted += *count;
+    if (*count < size) truncated = true;
+    // A level that fills max_results exactly is not yet a truncation: the
+    // next call finds out whether a visit follows (Results::admit).
+    finished = truncated || S.exhausted();
+    *more = finished ? 0 : 1;
+    return WG_OK;
+  }
+
+  wg_traversal_stats stats() const {
+    uint32_t pulled = 0;
+    for (uint32_t p : S.run.level_pulled) pulled += p;
+    return wg_traversal_stats{emitted, S.run.edges, truncated ? 1u : 0u, pulled};
+  }
 };
 
 }  // namespace wg
 
 using namespace wg;
+
+struct wg_bfs_cursor {
+  wg::LevelEmitter e;
+};
 
 extern "C" {
 
@@ -513,12 +730,11 @@ int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint
   if (s) return s;
   DevBytes fb{ctx};
   if (filter) {
-    fb.bytes = std::max<uint64_t>(g->n, 1);
-    if ((s = dalloc(ctx, &fb.p, fb.bytes))) return s;
+    if ((s = fb.alloc(g->n))) return s;
     WG_HIP(ctx, hipMemcpy(fb.p, filter, g->n, hipMemcpyHostToDevice));
   }
   BfsRun run;
-  s = bfs_run(ctx, g, direction, st, (const uint8_t*)fb.p, max_depth, -1, pulling != 0,
+  s = bfs_run(ctx, g, direction, st, (const uint8_t*)fb.p, max_depth, pulling != 0,
               (uint32_t*)depth->data, (uint32_t*)via->data, run);
   if (s) return s;
   if (stats) {
@@ -529,66 +745,17 @@ int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint
   return ok(ctx);
 }
 
-// The visits in the reference's order: by depth; the starts in the order
-// given; a pushed level by (via, node), the order reached when frontier
-// nodes expand in ascending order; a pulled level ascending by node.
-static int emit(wg_context* ctx, wg_graph* g, int direction, const uint64_t* starts, uint64_t ns,
-                const uint8_t* filter, int64_t max_depth, int64_t max_results, bool pulling,
-                bool pulled_order, std::vector<wg_visit>& visits, wg_traversal_stats* stats) {
-  const uint64_t n = g->n;
-  std::vector<uint32_t> st;
-  int s = prepare_starts(ctx, n, starts, ns, st);
-  if (s) return s;
-  DevBytes fb{ctx}, dd{ctx}, vd{ctx};
-  if (filter) {
-    fb.bytes = std::max<uint64_t>(n, 1);
-    if ((s = dalloc(ctx, &fb.p, fb.bytes))) return s;
-    WG_HIP(ctx, hipMemcpy(fb.p, filter, n, hipMemcpyHostToDevice));
+// Every level of an emitter into out.
+static int emit_all(wg_context* ctx, LevelEmitter& e, wg_visit* out, uint64_t* count) {
+  uint64_t total = 0;
+  int more = 1;
+  while (more) {
+    uint64_t k = 0;
+    int s = e.next(out + total, &k, &more);
+    if (s) return s;
+    total += k;
   }
-  dd.bytes = vd.bytes = std::max<uint64_t>(n, 1) * 4;
-  if ((s = dalloc(ctx, &dd.p, dd.bytes)) || (s = dalloc(ctx, &vd.p, vd.bytes))) return s;
-  BfsRun run;
-  s = bfs_run(ctx, g, direction, st, (const uint8_t*)fb.p, max_depth, max_results, pulling,
-              (uint32_t*)dd.p, (uint32_t*)vd.p, run);
-  if (s) return s;
-  std::vector<uint32_t> depth(n), via(n);
-  if (n) {
-    WG_HIP(ctx, hipMemcpy(depth.data(), dd.p, n * 4, hipMemcpyDeviceToHost));
-    WG_HIP(ctx, hipMemcpy(via.data(), vd.p, n * 4, hipMemcpyDeviceToHost));
-  }
-  // Counting sort by depth (nodes ascending within), then per pushed level
-  // a stable sort by via, which gives (via, node).
-  const uint32_t levels = run.levels;
-  std::vector<uint64_t> start_of(levels + 2, 0);
-  for (uint64_t v = 0; v < n; ++v)
-    if (depth[v] != kUnvisited && depth[v] >= 1 && depth[v] <= levels) start_of[depth[v] + 1]++;
-  for (uint32_t l = 1; l < levels + 2; ++l) start_of[l] += start_of[l - 1];
-  std::vector<uint32_t> by_depth(start_of[levels + 1]);
-  {
-    std::vector<uint64_t> cur(start_of);
-    for (uint64_t v = 0; v < n; ++v)
-      if (depth[v] != kUnvisited && depth[v] >= 1 && depth[v] <= levels) by_depth[cur[depth[v]]++] = (uint32_t)v;
-  }
-  visits.clear();
-  for (uint32_t sv : st) visits.push_back(wg_visit{sv, UINT64_MAX, 0, 0});
-  for (uint32_t l = 1; l <= levels; ++l) {
-    uint32_t* a = by_depth.data() + start_of[l];
-    uint32_t* b = by_depth.data() + start_of[l + 1];
-    const bool keep_ascending = pulled_order && run.level_pulled[l - 1];
-    if (!keep_ascending)
-      std::stable_sort(a, b, [&](uint32_t x, uint32_t y) { return via[x] < via[y]; });
-    for (uint32_t* p = a; p < b; ++p) visits.push_back(wg_visit{*p, via[*p], l, 0});
-  }
-  bool truncated = false;
-  if (max_results >= 0 && visits.size() > (uint64_t)max_results) {
-    visits.resize(max_results);
-    truncated = true;
-  }
-  if (stats) {
-    uint32_t pulled = 0;
-    for (uint32_t p : run.level_pulled) pulled += p;
-    *stats = wg_traversal_stats{visits.size(), run.edges, truncated ? 1u : 0u, pulled};
-  }
+  *count = total;
   return WG_OK;
 }
 
@@ -598,12 +765,12 @@ int wg_bfs(wg_context* ctx, const wg_graph* gc, int direction, const uint64_t* s
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
   WG_HIP(ctx, hipSetDevice(ctx->device));
-  std::vector<wg_visit> v;
-  int s = emit(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, filter, max_depth,
-               max_results, true, false, v, stats);
+  LevelEmitter e;
+  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, filter, max_depth,
+                 max_results, true, false);
+  if (!s) s = emit_all(ctx, e, out, count);
   if (s) return s;
-  std::copy(v.begin(), v.end(), out);
-  *count = v.size();
+  if (stats) *stats = e.stats();
   return ok(ctx);
 }
 
@@ -613,13 +780,53 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const u
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
   WG_HIP(ctx, hipSetDevice(ctx->device));
-  std::vector<wg_visit> v;
-  int s = emit(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, nullptr, max_depth, -1,
-               true, true, v, stats);
+  LevelEmitter e;
+  std::vector<wg_visit> v(std::max<uint64_t>(gc->n, 1));
+  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, nullptr, max_depth, -1,
+                 true, true);
+  if (!s) s = emit_all(ctx, e, v.data(), count);
   if (s) return s;
-  for (size_t i = 0; i < v.size(); ++i) out[i] = v[i].node;
-  *count = v.size();
+  for (uint64_t i = 0; i < *count; ++i) out[i] = v[i].node;
+  if (stats) *stats = e.stats();
   return ok(ctx);
 }
+
+int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
+                         uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
+                         int reachable, wg_bfs_cursor** out) {
+  if (!ctx || !g || !out) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
+  if (reachable && (filter || max_results >= 0))
+    return fail(ctx, WG_INVALID_ARGUMENT, "a reachable-set cursor takes no filter or result limit");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  wg_bfs_cursor* c = new (std::nothrow) wg_bfs_cursor();
+  if (!c) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  // Levels are pushed or pulled as wg_bfs does (same visits and stats); a
+  // reachable set lists a pulled level by node (bfs_cursor.rs:74-87).
+  int s = c->e.init(ctx, const_cast<wg_graph*>(g), direction, starts, nstarts, filter, max_depth,
+                    max_results, true, reachable != 0);
+  if (s) {
+    delete c;
+    return s;
+  }
+  *out = c;
+  return ok(ctx);
+}
+
+int wg_bfs_cursor_next(wg_context* ctx, wg_bfs_cursor* c, wg_visit* out, uint64_t* count, int* more) {
+  if (!ctx || !c || !out || !count || !more) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  int s = c->e.next(out, count, more);
+  return s ? s : ok(ctx);
+}
+
+int wg_bfs_cursor_stats(const wg_bfs_cursor* c, wg_traversal_stats* stats) {
+  if (!c || !stats) return WG_INVALID_ARGUMENT;
+  *stats = c->e.stats();
+  return WG_OK;
+}
+
+void wg_bfs_cursor_destroy(wg_bfs_cursor* c) { delete c; }
 
 }  // extern "C"

@@ -518,6 +518,11 @@ int build_reverse(wg_context* ctx, wg_graph* g) {
   return WG_OK;
 }
 
+int sort_u32_pairs(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout,
+                   uint64_t m, unsigned bits) {
+  return m ? sort_pairs(ctx, kin, kout, vin, vout, m, bits) : WG_OK;
+}
+
 // pos[p] = the reverse position of forward entry p, for every entry.
 int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos) {
   const uint64_t m = g->nnz;
