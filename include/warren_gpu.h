@@ -46,7 +46,8 @@ enum wg_status {
   WG_NOT_FINITE = 20,          /* SpectralError::NotFinite */
   WG_EDGE_OUTSIDE_ID_SPACE = 21,       /* GraphError::EdgeOutsideIdSpace (graph/error.rs:91-98) */
   WG_RELATION_OUTSIDE_DICTIONARY = 22, /* GraphError::RelationOutsideDictionary (:99-104) */
-  WG_UNKNOWN_RELATION = 23             /* ComputeError::UnknownRelation (compute/error.rs:82-87) */
+  WG_UNKNOWN_RELATION = 23,            /* ComputeError::UnknownRelation (compute/error.rs:82-87) */
+  WG_INVALID_SEGMENT = 24              /* GraphError::Segment* (graph/error.rs): a segment's framing */
 };
 
 /* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
@@ -142,6 +143,40 @@ int wg_graph_select(wg_context* ctx, wg_graph* g, const uint32_t* relations, uin
 /* The relations a graph holds: relation_count of wg_graph_build when 2 or
  * more, else 1 (the graph is its only relation). */
 int wg_graph_relation_count(const wg_graph* g, uint32_t* count);
+/* ---- segments: the WRSG codec of a CSR (graph/segment/mod.rs:1-16,
+ * graph/segment/csr.rs). Each segment is a 16-byte little-endian header
+ * (magic "WRSG", format 2, kind, element width, reserved 0, u64 count)
+ * and its payload: offsets (kind 1, u64), targets (kind 2, u32 below a
+ * 2^32 id space), weights (kind 3, f64) and, with sparse offsets, rows
+ * (kind 11, the stored rows' ids). */
+typedef struct {
+  const uint8_t* rows; uint64_t rows_len;       /* NULL: dense offsets */
+  const uint8_t* offsets; uint64_t offsets_len;
+  const uint8_t* targets; uint64_t targets_len;
+  const uint8_t* weights; uint64_t weights_len; /* NULL: unweighted */
+} wg_csr_segments;
+/* AnyCsr::decode (graph/segment/csr.rs:84-127) straight to the device: a
+ * graph from its forward CSR's segments (host memory), and its reverse
+ * CSR's if given (taken as the transpose, as the reference takes it), else
+ * transposed on the device. Every check of the decode and of
+ * Csr::from_segments (graph/csr/mod.rs:78-137, 239-258) in their order;
+ * the first violation is the error: WG_INVALID_SEGMENT for the framing
+ * (SegmentTruncated, SegmentMagic, SegmentFormat, SegmentKind,
+ * SegmentReserved, SegmentWidth, SegmentLength), WG_INVALID_CSR for the
+ * CSR (OffsetsLength, FirstOffsetNotZero, StoredRowsNotAscending,
+ * StoredRowOutsideIdSpace, StoredRowEmpty, OffsetsNotMonotone,
+ * FinalOffset, TargetOutsideIdSpace, RowNotSorted, WeightsLength), the
+ * variant and its fields in the message. */
+int wg_graph_load_segments(wg_context* ctx, uint64_t id_space, const wg_csr_segments* forward,
+                           const wg_csr_segments* reverse, wg_graph** out);
+/* The byte lengths of a CSR's dense segments (csr.rs:130-137). */
+int wg_csr_segment_lengths(uint64_t id_space, uint64_t nnz, uint64_t* offsets_len,
+                           uint64_t* targets_len, uint64_t* weights_len);
+/* AnyCsr::encode with dense offsets: the forward (reverse = 0) or reverse
+ * CSR as its segments, into host buffers of wg_csr_segment_lengths bytes
+ * (any may be NULL; weights only for a weighted graph). */
+int wg_graph_encode_segments(wg_context* ctx, const wg_graph* g, int reverse, uint8_t* offsets,
+                             uint8_t* targets, uint8_t* weights);
 /* The forward CSR (host buffers of n+1, nnz, nnz; targets and weights may
  * be NULL). */
 int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* offsets,

@@ -25,7 +25,7 @@ STATUS_NAMES = {
     12: "node_outside_id_space", 13: "node_filter_id_space", 14: "invalid_csr",
     15: "unsupported", 16: "device_error", 17: "not_symmetric", 18: "non_positive_degree",
     19: "eigenpair_count", 20: "not_finite", 21: "edge_outside_id_space",
-    22: "relation_outside_dictionary", 23: "unknown_relation",
+    22: "relation_outside_dictionary", 23: "unknown_relation", 24: "invalid_segment",
 }
 DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
 NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
@@ -215,6 +215,43 @@ class Graph:
         ctx.check(ctx.lib.wg_graph_build(ctx.h, int(n), len(src), _p(src), _p(dst), _p(w), _p(rel),
                                          int(relation_count), 0, C.byref(h), C.byref(folded)))
         return cls._adopt(ctx, h, int(n), w is not None, int(folded.value))
+
+    @classmethod
+    def load_segments(cls, ctx, id_space, forward, reverse=None):
+        """AnyCsr::decode (graph/segment/csr.rs:84-127) straight to the device:
+        forward (and optionally reverse) as dicts of segment bytes with keys
+        'offsets', 'targets' and optionally 'weights' and 'rows'."""
+        keep = []
+
+        def seg(d):
+            c = _lib.CsrSegments()
+            for k in ("rows", "offsets", "targets", "weights"):
+                b = d.get(k)
+                if b is not None:
+                    arr = np.frombuffer(bytes(b), np.uint8) if len(b) else np.zeros(1, np.uint8)
+                    keep.append(arr)
+                    setattr(c, k, arr.ctypes.data)
+                    setattr(c, k + "_len", len(b))
+            return c
+
+        fw = seg(forward)
+        rv = None if reverse is None else seg(reverse)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wg_graph_load_segments(ctx.h, int(id_space), C.byref(fw),
+                                                 None if rv is None else C.byref(rv), C.byref(h)))
+        return cls._adopt(ctx, h, int(id_space), forward.get("weights") is not None)
+
+    def encode_segments(self, reverse=False):
+        """AnyCsr::encode with dense offsets: {'offsets', 'targets', 'weights'}."""
+        lo, lt, lw = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.lib.wg_csr_segment_lengths(self.n, self.nnz, C.byref(lo), C.byref(lt), C.byref(lw))
+        o = np.zeros(lo.value, np.uint8)
+        t = np.zeros(lt.value, np.uint8)
+        w = np.zeros(lw.value, np.uint8) if self.weighted else None
+        self.ctx.check(self.ctx.lib.wg_graph_encode_segments(self.ctx.h, self.h, int(reverse), _p(o),
+                                                             _p(t), _p(w)))
+        return {"offsets": o.tobytes(), "targets": t.tobytes(),
+                "weights": None if w is None else w.tobytes()}
 
     @classmethod
     def _adopt(cls, ctx, h, n, weighted, folded=0):

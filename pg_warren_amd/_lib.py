@@ -29,6 +29,13 @@ class Product(C.Structure):
                 ("per_relation", C.c_int32)]
 
 
+class CsrSegments(C.Structure):
+    """wg_csr_segments: graph/segment/csr.rs:22-29."""
+    _fields_ = [("rows", P), ("rows_len", C.c_uint64), ("offsets", P), ("offsets_len", C.c_uint64),
+                ("targets", P), ("targets_len", C.c_uint64), ("weights", P),
+                ("weights_len", C.c_uint64)]
+
+
 class PagerankOpts(C.Structure):
     """wg_pagerank_opts: compute/pagerank/mod.rs:27-47."""
     _fields_ = [("damping", C.c_double), ("tol", C.c_double), ("has_tol", C.c_int32),
@@ -80,6 +87,9 @@ SIGNATURES = [
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
     ("wg_graph_read_forward", I32, [P, P, P, P, P]),
     ("wg_graph_select", I32, [P, P, P, U32, I32, P]),
+    ("wg_graph_load_segments", I32, [P, U64, P, P, P]),
+    ("wg_csr_segment_lengths", I32, [U64, U64, P, P, P]),
+    ("wg_graph_encode_segments", I32, [P, P, I32, P, P, P]),
     ("wg_bfs_cursor_create", I32, [P, P, I32, P, U64, P, C.c_int64, C.c_int64, I32, P]),
     ("wg_bfs_cursor_next", I32, [P, P, P, P, P]),
     ("wg_bfs_cursor_stats", I32, [P, P]),

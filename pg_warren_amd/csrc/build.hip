@@ -390,6 +390,8 @@ int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, 
   return WG_OK;
 }
 
+}  // namespace
+
 // row_of[p] for every entry of a partitioned CSR.
 int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of) {
   if (c.nnz == 0) return WG_OK;
@@ -398,6 +400,8 @@ int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of) {
   WG_HIP(ctx, hipGetLastError());
   return WG_OK;
 }
+
+namespace {
 
 // The reverse CSR of the partitioned forward CSR f into r (offsets,
 // targets, weights; not yet partitioned).
@@ -499,22 +503,33 @@ int build_reverse(wg_context* ctx, wg_graph* g) {
   if (s) return s;
   // A = A^T exactly when the reverse arrays equal the forward ones (an
   // undirected graph's do; spectral/operator.rs:420-436).
+  bool same = false;
+  if ((s = same_csr(ctx, g->fwd, g->rev, &same))) return s;
+  g->rev_is_fwd = same;
+  return WG_OK;
+}
+
+// Whether two CSRs over one id space hold equal arrays (weights compared
+// with ==; both weighted or neither).
+int same_csr(wg_context* ctx, const DevCsr& a, const DevCsr& b, bool* same) {
+  *same = false;
+  if (a.n != b.n || a.nnz != b.nnz || (a.weights == nullptr) != (b.weights == nullptr)) return WG_OK;
   Scratch<unsigned> flag;
+  int s;
   if ((s = flag.alloc(ctx, 1))) return s;
   hipStream_t st = ctx->stream;
   WG_HIP(ctx, hipMemsetAsync(flag.p, 0, 4, st));
-  const uint64_t n = g->n, m = g->nnz;
-  differs<uint64_t><<<grid_of(n + 1), kThreads, 0, st>>>(g->fwd.offsets, g->rev.offsets, n + 1, flag.p);
+  const uint64_t n = a.n, m = a.nnz;
+  differs<uint64_t><<<grid_of(n + 1), kThreads, 0, st>>>(a.offsets, b.offsets, n + 1, flag.p);
   if (m) {
-    differs<uint32_t><<<grid_of(m), kThreads, 0, st>>>(g->fwd.targets, g->rev.targets, m, flag.p);
-    if (g->fwd.weights)
-      differs<double><<<grid_of(m), kThreads, 0, st>>>(g->fwd.weights, g->rev.weights, m, flag.p);
+    differs<uint32_t><<<grid_of(m), kThreads, 0, st>>>(a.targets, b.targets, m, flag.p);
+    if (a.weights) differs<double><<<grid_of(m), kThreads, 0, st>>>(a.weights, b.weights, m, flag.p);
   }
   WG_HIP(ctx, hipGetLastError());
   unsigned h = 0;
   WG_HIP(ctx, hipMemcpyAsync(&h, flag.p, 4, hipMemcpyDeviceToHost, st));
   WG_HIP(ctx, hipStreamSynchronize(st));
-  g->rev_is_fwd = h == 0;
+  *same = h == 0;
   return WG_OK;
 }
 

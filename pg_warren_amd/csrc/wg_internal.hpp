@@ -166,6 +166,10 @@ int select_graph(wg_context* ctx, wg_graph* g, const uint32_t* rels, uint32_t k,
                  wg_graph** out);
 // Frees a graph, its relations and selections.
 void destroy_graph(wg_graph* g);
+// Whether two CSRs hold equal arrays (on the device).
+int same_csr(wg_context* ctx, const DevCsr& a, const DevCsr& b, bool* same);
+// row_of[p] = the row of entry p of a partitioned CSR.
+int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of);
 // Stable ascending radix sort of m (key, value) pairs on key bits [0, bits),
 // on the context stream.
 int sort_u32_pairs(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout,
