@@ -296,6 +296,31 @@ def bench_build(W, ctx, scale=22, reps=3):
     return res, {"n": n, "src": src, "dst": dst, "w": w, "fwd": fwd, "rev": rev}
 
 
+def bench_segments(W, ctx, off, tgt, w, reps=3):
+    """The segment loader (wg_graph_load_segments) on the bench graph's
+    forward CSR as WRSG segments with sparse offsets (the rows segment) and
+    f64 weights, in host memory: headers read, payloads copied in, every
+    CSR check on the device, sparse offsets made dense, the reverse
+    transposed. Beside it the oracle's decode of the same bytes (numpy)."""
+    from oracle import segments as S
+    n = len(off) - 1
+    seg = S.sparse_form(off, tgt, w)
+    times = []
+    for _ in range(reps + 1):
+        t1 = time.perf_counter()
+        g = W.Graph.load_segments(ctx, n, seg)
+        times.append(time.perf_counter() - t1)
+        g.close()
+    t1 = time.perf_counter()
+    S.decode_csr(n, seg["rows"], seg["offsets"], seg["targets"], seg["weights"])
+    t_oracle = time.perf_counter() - t1
+    byt = sum(len(b) for b in seg.values() if b is not None)
+    ms = min(times[1:]) * 1e3
+    log(f"segments: {byt / 1e6:.0f} MB loaded in {ms:.1f} ms (oracle decode {t_oracle:.2f} s)")
+    return {"bytes": byt, "ms_best_of_%d" % reps: round(ms, 1), "gbs": round(byt / ms / 1e6, 1),
+            "sparse_rows": True, "oracle_decode_s_numpy": round(t_oracle, 2)}
+
+
 def build_oracle_errors(st):
     """The oracle's build and transpose of the same edge list (one thread),
     compared bit for bit; its time is the CPU side of the build leg."""
@@ -615,6 +640,8 @@ def main():
     build_state = adj_state = lz_state = None
     if not args.skip_build:
         legs["csr_build"], build_state = bench_build(W, ctx, scale=args.scale)
+        if not args.no_cpu:
+            legs["segment_load"] = bench_segments(W, ctx, off, tgt, wf)
     if not args.skip_adjoint:
         legs["pagerank_adjoint"], adj_state = bench_adjoint(W, ctx, scale=max(args.scale - 2, 8))
     if not args.skip_lanczos:
