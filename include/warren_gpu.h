@@ -47,7 +47,9 @@ enum wg_status {
   WG_EDGE_OUTSIDE_ID_SPACE = 21,       /* GraphError::EdgeOutsideIdSpace (graph/error.rs:91-98) */
   WG_RELATION_OUTSIDE_DICTIONARY = 22, /* GraphError::RelationOutsideDictionary (:99-104) */
   WG_UNKNOWN_RELATION = 23,            /* ComputeError::UnknownRelation (compute/error.rs:82-87) */
-  WG_INVALID_SEGMENT = 24              /* GraphError::Segment* (graph/error.rs): a segment's framing */
+  WG_INVALID_SEGMENT = 24,             /* GraphError::Segment* (graph/error.rs): a segment's framing */
+  WG_COMBINER_TYPE = 25,               /* ComputeError::CombinerType (compute/error.rs:109-116) */
+  WG_SENDER_OUTSIDE_ID_SPACE = 26      /* ComputeError::SenderOutsideIdSpace (:117-124) */
 };
 
 /* ---- element types (compute/dtype.rs:12-21), plus u32: the image of i64 on
@@ -68,7 +70,7 @@ enum wg_binary_op {
 };
 
 /* ---- traversal direction (graph::Direction, traversal/adjacency.rs:20-60) */
-enum wg_direction { WG_OUT = 0, WG_IN = 1 };
+enum wg_direction { WG_OUT = 0, WG_IN = 1, WG_BOTH = 2 /* gather only */ };
 
 typedef struct wg_context wg_context;
 typedef struct wg_graph wg_graph;
@@ -370,6 +372,51 @@ void wg_bfs_cursor_destroy(wg_bfs_cursor* cursor);
 int wg_msbfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* sources,
              uint32_t nsources, int64_t max_depth, wg_vector* depths, wg_vector* reach,
              wg_vector* farness, uint32_t* levels);
+
+/* ---- gather: compute/gather/mod.rs:1-154 ---------------------------------
+ * Messages fanned out over the graph's edges and combined at each receiver
+ * in collect order: ascending sender dense id, a sender's messages in push
+ * order, the receiver's own first with include_self. A receiver's senders
+ * are its distinct neighbours in the direction (WG_IN: in-neighbours, the
+ * pull of a transposed product; WG_OUT; WG_BOTH: their union) over the
+ * selected relations (NULL: all), or with per_relation a sender once per
+ * relation linking it (both directions of one relation once). Combiners in
+ * Combiner::ALL order (compute/gather/combiner.rs:13-57). */
+enum wg_combiner {
+  WG_COMBINE_COLLECT = 0, WG_COMBINE_COUNT, WG_COMBINE_SUM, WG_COMBINE_MIN, WG_COMBINE_MAX,
+  WG_COMBINE_FIRST, WG_COMBINE_CONCAT, WG_COMBINE_BOOL_OR, WG_COMBINE_BOOL_AND, WG_COMBINE_MODE
+};
+typedef struct {
+  int32_t combiner;         /* wg_combiner */
+  int32_t direction;        /* wg_direction */
+  const uint32_t* relations;
+  uint32_t nrelations;
+  int32_t per_relation;
+  const wg_vector* mask;    /* receivers the mask selects (compute/mask.rs:35-45), or NULL */
+  int32_t mask_complement;
+  int32_t include_self;
+} wg_gather_opts;
+typedef struct wg_gathered wg_gathered;
+/* gather (mod.rs:96-145) over `count` messages of element type dtype
+ * (WG_F32, WG_F64, WG_I64 or WG_BOOL; byte messages are not gathered on
+ * the device, WG_UNSUPPORTED), senders[i] sending messages[i] (host arrays,
+ * push order). The result holds the receivers that got a message,
+ * ascending, each with its value: count as i64; sum, min, max, first,
+ * bool_or, bool_and and mode as one message (i64 sums that overflow are
+ * WG_BIGINT_OUT_OF_RANGE, the lowest receiver's); collect as its messages
+ * in collect order. A combiner the element type does not take is
+ * WG_COMBINER_TYPE (concat always); a sender outside the id space,
+ * WG_SENDER_OUTSIDE_ID_SPACE. */
+int wg_gather(wg_context* ctx, const wg_graph* g, const wg_gather_opts* opts, int dtype, uint64_t count,
+              const uint32_t* senders, const void* messages, wg_gathered** out);
+/* The receivers and the values held (collect: every collected message). */
+int wg_gathered_info(const wg_gathered* r, uint64_t* receivers, uint64_t* values);
+/* Host copies: receivers (u64 dense ids), values (i64 for count, else the
+ * messages' type), offsets (collect: receivers + 1 positions into values).
+ * Any may be NULL. */
+int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t* receivers, void* values,
+                     uint64_t* offsets);
+void wg_gathered_destroy(wg_gathered* r);
 
 /* ---- eigenpairs: eigen (spectral/lanczos.rs:69-224) ----------------------
  * The matrix (spectral/operator.rs:33-45): S = (A + A^T)/2 with symmetrize

@@ -26,6 +26,7 @@ STATUS_NAMES = {
     15: "unsupported", 16: "device_error", 17: "not_symmetric", 18: "non_positive_degree",
     19: "eigenpair_count", 20: "not_finite", 21: "edge_outside_id_space",
     22: "relation_outside_dictionary", 23: "unknown_relation", 24: "invalid_segment",
+    25: "combiner_type", 26: "sender_outside_id_space",
 }
 DTYPES = {"f32": 0, "f64": 1, "i64": 2, "bool": 3, "u32": 4}
 NP_DTYPES = {"f32": np.float32, "f64": np.float64, "i64": np.int64, "bool": np.uint8,
@@ -503,6 +504,61 @@ def reachable_set(ctx, graph, starts, *, direction="out", max_depth=None):
                                        -1 if max_depth is None else max_depth, _p(out),
                                        C.byref(cnt), C.byref(stats)))
     return out[:cnt.value].copy(), _stats(stats)
+
+
+COMBINERS = ["collect", "count", "sum", "min", "max", "first", "concat", "bool_or", "bool_and", "mode"]
+GATHER_DIRECTIONS = {"out": 0, "in": 1, "both": 2}
+
+
+def gather(ctx, graph, senders, messages, combiner, *, direction="in", relations=None,
+           per_relation=False, mask=None, mask_complement=False, include_self=False):
+    """gather (compute/gather/mod.rs:96-145): messages[i] sent by senders[i]
+    (push order), combined at each receiver in collect order. Returns
+    (receivers, values): values one per receiver (count: i64), or for
+    collect a list of arrays, one per receiver. messages: a numpy array of
+    f32, f64, i64 or bool; mask: a Vector or None."""
+    msg = np.asarray(messages)
+    dt = _dtype_of(msg)
+    msg = np.ascontiguousarray(msg, NP_DTYPES[dt])
+    snd = np.asarray(senders)
+    if snd.size and (snd.min() < 0 or snd.max() >= 2**32):
+        bad = int(np.flatnonzero((snd < 0) | (snd >= 2**32))[0])
+        raise WarrenError(26, f"sender {snd[bad]} outside the id space {graph.n}")
+    snd = np.ascontiguousarray(snd, np.uint32)
+    if snd.shape != msg.shape:
+        raise ValueError("senders and messages differ in length")
+    if combiner not in COMBINERS:
+        raise WarrenError(11, f"unknown combiner {combiner!r}")
+    rels, nrels, keep = None, 0, None
+    if relations is not None:
+        keep = np.ascontiguousarray(relations, np.uint32).reshape(-1)
+        nrels = keep.size
+        keep = keep if nrels else np.zeros(1, np.uint32)
+        rels = keep.ctypes.data
+    if mask is not None and not isinstance(mask, Vector):
+        mask = Vector(ctx, np.asarray(mask))
+    opts = _lib.GatherOpts(COMBINERS.index(combiner), GATHER_DIRECTIONS[direction], rels, nrels,
+                           int(per_relation), None if mask is None else mask.h.value,
+                           int(mask_complement), int(include_self))
+    h = C.c_void_p()
+    ctx.check(ctx.lib.wg_gather(ctx.h, graph.h, C.byref(opts), DTYPES[dt], len(snd), _p(snd), _p(msg),
+                                C.byref(h)))
+    try:
+        nr, nv = C.c_uint64(), C.c_uint64()
+        ctx.lib.wg_gathered_info(h, C.byref(nr), C.byref(nv))
+        recv = np.zeros(max(nr.value, 1), np.uint64)
+        vt = np.int64 if combiner == "count" else NP_DTYPES[dt]
+        vals = np.zeros(max(nv.value, 1), vt)
+        offs = np.zeros(nr.value + 1, np.uint64) if combiner == "collect" else None
+        ctx.check(ctx.lib.wg_gathered_read(ctx.h, h, _p(recv), _p(vals), _p(offs)))
+    finally:
+        ctx.lib.wg_gathered_destroy(h)
+    recv, vals = recv[:nr.value], vals[:nv.value]
+    if dt == "bool":
+        vals = vals.astype(bool) if combiner != "count" else vals
+    if combiner == "collect":
+        return recv, [vals[int(offs[i]):int(offs[i + 1])] for i in range(nr.value)]
+    return recv, vals
 
 
 class BfsCursor:

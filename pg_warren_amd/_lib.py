@@ -36,6 +36,13 @@ class CsrSegments(C.Structure):
                 ("weights_len", C.c_uint64)]
 
 
+class GatherOpts(C.Structure):
+    """wg_gather_opts: compute/gather/mod.rs:48-64."""
+    _fields_ = [("combiner", C.c_int32), ("direction", C.c_int32), ("relations", P),
+                ("nrelations", C.c_uint32), ("per_relation", C.c_int32), ("mask", P),
+                ("mask_complement", C.c_int32), ("include_self", C.c_int32)]
+
+
 class PagerankOpts(C.Structure):
     """wg_pagerank_opts: compute/pagerank/mod.rs:27-47."""
     _fields_ = [("damping", C.c_double), ("tol", C.c_double), ("has_tol", C.c_int32),
@@ -87,6 +94,10 @@ SIGNATURES = [
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),
     ("wg_graph_read_forward", I32, [P, P, P, P, P]),
     ("wg_graph_select", I32, [P, P, P, U32, I32, P]),
+    ("wg_gather", I32, [P, P, P, I32, U64, P, P, P]),
+    ("wg_gathered_info", I32, [P, P, P]),
+    ("wg_gathered_read", I32, [P, P, P, P, P]),
+    ("wg_gathered_destroy", None, [P]),
     ("wg_graph_load_segments", I32, [P, U64, P, P, P]),
     ("wg_csr_segment_lengths", I32, [U64, U64, P, P, P]),
     ("wg_graph_encode_segments", I32, [P, P, I32, P, P, P]),

@@ -538,6 +538,49 @@ int sort_u32_pairs(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const u
   return m ? sort_pairs(ctx, kin, kout, vin, vout, m, bits) : WG_OK;
 }
 
+int sort_u64_keyed(wg_context* ctx, const uint64_t* kin, uint64_t* kout, const uint64_t* vin, uint64_t* vout,
+                   uint64_t m, unsigned bits) {
+  return m ? sort_pairs(ctx, kin, kout, vin, vout, m, bits) : WG_OK;
+}
+
+int sort_u32_keyed(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint64_t* vin, uint64_t* vout,
+                   uint64_t m, unsigned bits) {
+  return m ? sort_pairs(ctx, kin, kout, vin, vout, m, bits) : WG_OK;
+}
+
+int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m) {
+  if (!m) return WG_OK;
+  size_t tb = 0;
+  WG_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, in, out, (uint64_t)0, (size_t)m, rocprim::plus<uint64_t>(),
+                                      ctx->stream));
+  Scratch<char> tmp;
+  if (int s = tmp.alloc(ctx, tb)) return s;
+  WG_HIP(ctx, rocprim::exclusive_scan((void*)tmp.p, tb, in, out, (uint64_t)0, (size_t)m,
+                                      rocprim::plus<uint64_t>(), ctx->stream));
+  return WG_OK;
+}
+
+// The edges of a CSR as (row, column) pairs, into src/dst at `base`.
+int csr_edges(wg_context* ctx, const DevCsr& c, uint32_t* src, uint32_t* dst) {
+  if (!c.nnz) return WG_OK;
+  int s = expand_rows(ctx, c, src);
+  if (s) return s;
+  WG_HIP(ctx, hipMemcpyAsync(dst, c.targets, c.nnz * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  return WG_OK;
+}
+
+// An unweighted CSR (forward only, partitioned) of an edge list on the
+// device: parallel edges folded (keep = false) or kept, rows sorted by
+// column, equal columns in input order.
+int csr_of_edges(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, bool keep,
+                 DevCsr& out) {
+  uint64_t folded = 0;
+  int s = m < (1ull << 32) ? build_forward<uint32_t>(ctx, n, m, src, dst, nullptr, nullptr, 0, out, &folded, keep)
+                           : build_forward<uint64_t>(ctx, n, m, src, dst, nullptr, nullptr, 0, out, &folded, keep);
+  if (!s) s = adopt_device_csr(ctx, out);
+  return s;
+}
+
 // pos[p] = the reverse position of forward entry p, for every entry.
 int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos) {
   const uint64_t m = g->nnz;

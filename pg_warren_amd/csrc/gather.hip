@@ -1,0 +1,616 @@
+// gather (compute/gather/mod.rs): per-node messages fanned out over the
+// graph's edges and combined at each receiver, on the device.
+//
+// A receiver's senders are its distinct neighbours in the gather's
+// direction over the selected relations (in: the reverse CSR's row, out:
+// the forward row, both: their union), once each, or with per_relation once
+// per relation linking them; each sender's messages arrive in push order,
+// the senders in ascending dense id, the receiver's own messages first with
+// include_self (mod.rs:1-23). The pieces:
+//   messages   a stable sort by sender (push order kept) gives each sender's
+//              messages as a contiguous run (the outbox, outbox.rs:41-75);
+//   senders    a CSR per receiver: a relation selection's forward or
+//              reverse CSR, or for both directions the union of the two
+//              built like a selection; kept with the graph;
+//   combine    a thread per receiver walks its senders' runs in collect
+//              order and folds with the reference's operator rules
+//              (SemiringElement::apply: i64 sums checked, min and max in
+//              SQL order, the first operand kept on ties), or counts, or
+//              keeps the first;
+//   collect    counts, an exclusive scan, then each receiver writes its
+//              messages at its offset;
+//   mode       the collected lists sorted by (receiver, value in SQL order)
+//              stably, so a run of equal values keeps collect order; a
+//              receiver's longest run wins, the smallest value on a tie,
+//              and its first message is the answer.
+// Receivers that got no message are left out; the rest come in ascending
+// dense-id order (gathered.rs:10-16).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "semiring.hpp"
+#include "wg_internal.hpp"
+
+namespace wg {
+namespace {
+
+constexpr unsigned kThreads = 256;
+// Combiner::ALL order (compute/gather/combiner.rs:46-57).
+enum { C_COLLECT = 0, C_COUNT, C_SUM, C_MIN, C_MAX, C_FIRST, C_CONCAT, C_BOOL_OR, C_BOOL_AND, C_MODE };
+
+unsigned grid_of(uint64_t items) {
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + kThreads - 1) / kThreads, 8192));
+}
+
+__device__ __forceinline__ uint64_t first_index() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint64_t stride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+__global__ __launch_bounds__(kThreads) void first_sender_outside(const uint32_t* __restrict__ senders, uint64_t k,
+                                                                 uint64_t n, unsigned long long* __restrict__ first) {
+  for (uint64_t i = first_index(); i < k; i += stride())
+    if (senders[i] >= n) {
+      atomicMin(first, (unsigned long long)i);
+      return;
+    }
+}
+
+template <class E>
+__global__ __launch_bounds__(kThreads) void permute(const uint32_t* __restrict__ perm, const E* __restrict__ in,
+                                                    uint64_t k, E* __restrict__ out) {
+  for (uint64_t i = first_index(); i < k; i += stride()) out[i] = in[perm[i]];
+}
+
+__global__ __launch_bounds__(kThreads) void iota32(uint32_t* __restrict__ a, uint64_t k) {
+  for (uint64_t i = first_index(); i < k; i += stride()) a[i] = (uint32_t)i;
+}
+
+// moff[s] = the first sorted message of sender s, s in [0, n].
+__global__ __launch_bounds__(kThreads) void sender_starts(const uint32_t* __restrict__ sorted, uint64_t k,
+                                                          uint64_t n, uint64_t* __restrict__ moff) {
+  for (uint64_t s = first_index(); s <= n; s += stride()) {
+    uint64_t lo = 0, hi = k;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if ((uint64_t)sorted[mid] < s)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    moff[s] = lo;
+  }
+}
+
+struct Receiving {
+  const uint64_t* soff;  // senders CSR (null: no senders)
+  const uint32_t* stgt;
+  const uint64_t* moff;  // each sender's run of messages
+  const void* mask;
+  int mask_dtype, mask_complement, self;
+};
+
+__device__ __forceinline__ bool selected(const Receiving& R, uint64_t r) {
+  return !R.mask || mask_selects(R.mask, R.mask_dtype, R.mask_complement, r);
+}
+
+// Calls visit(k) for receiver r's messages k in collect order.
+template <class F>
+__device__ __forceinline__ void each_message(const Receiving& R, uint64_t r, F&& visit) {
+  if (R.self)
+    for (uint64_t k = R.moff[r]; k < R.moff[r + 1]; ++k) visit(k);
+  if (R.soff)
+    for (uint64_t p = R.soff[r]; p < R.soff[r + 1]; ++p) {
+      const uint32_t s = R.stgt[p];
+      for (uint64_t k = R.moff[s]; k < R.moff[s + 1]; ++k) visit(k);
+    }
+}
+
+// Folds each receiver's messages with OP (a BinaryOp, or -1 to count only):
+// cnt[r] messages, val[r] their fold; the lowest failing receiver raises.
+template <class E, int OP>
+__global__ __launch_bounds__(kThreads) void fold_receivers(Receiving R, uint64_t n, const E* __restrict__ mval,
+                                                           E* __restrict__ val, uint64_t* __restrict__ cnt,
+                                                           unsigned long long* __restrict__ err) {
+  for (uint64_t r = first_index(); r < n; r += stride()) {
+    if (!selected(R, r)) {
+      cnt[r] = 0;
+      continue;
+    }
+    uint64_t c = 0;
+    E acc{};
+    uint32_t e_err = 0;
+    each_message(R, r, [&](uint64_t k) {
+      if constexpr (OP >= 0) {
+        const E v = mval[k];
+        if (c == 0)
+          acc = v;
+        else if (!e_err)
+          acc = apply<OP>(acc, v, e_err);
+      }
+      ++c;
+    });
+    cnt[r] = c;
+    if constexpr (OP >= 0) val[r] = acc;
+    if (e_err) raise_error(err, r, e_err);
+  }
+}
+
+// Each receiver's messages in collect order at its offset; ord[i] = the
+// receiver's rank among those that received (for mode).
+template <class E>
+__global__ __launch_bounds__(kThreads) void collect_fill(Receiving R, uint64_t n, const E* __restrict__ mval,
+                                                         const uint64_t* __restrict__ start,
+                                                         const uint64_t* __restrict__ rank, E* __restrict__ out,
+                                                         uint32_t* __restrict__ ord) {
+  for (uint64_t r = first_index(); r < n; r += stride()) {
+    if (!selected(R, r)) continue;
+    uint64_t at = start[r];
+    const uint32_t rk = (uint32_t)rank[r];
+    each_message(R, r, [&](uint64_t k) {
+      out[at] = mval[k];
+      if (ord) ord[at] = rk;
+      ++at;
+    });
+  }
+}
+
+// received[r] = 1 if r got a message.
+__global__ __launch_bounds__(kThreads) void received_flags(const uint64_t* __restrict__ cnt, uint64_t n,
+                                                           uint64_t* __restrict__ flags) {
+  for (uint64_t r = first_index(); r < n; r += stride()) flags[r] = cnt[r] ? 1 : 0;
+}
+
+// The receivers that got a message, ascending, their values and (collect)
+// their offsets into the collected messages.
+template <class E>
+__global__ __launch_bounds__(kThreads) void compact_receivers(const uint64_t* __restrict__ cnt,
+                                                              const uint64_t* __restrict__ rank, uint64_t n,
+                                                              const E* __restrict__ val,
+                                                              const uint64_t* __restrict__ start,
+                                                              uint32_t* __restrict__ recv, E* __restrict__ oval,
+                                                              int64_t* __restrict__ ocnt,
+                                                              uint64_t* __restrict__ ooff) {
+  for (uint64_t r = first_index(); r < n; r += stride()) {
+    if (!cnt[r]) continue;
+    const uint64_t i = rank[r];
+    recv[i] = (uint32_t)r;
+    if (oval) oval[i] = val[r];
+    if (ocnt) ocnt[i] = (int64_t)cnt[r];
+    if (ooff) ooff[i] = start[r];
+  }
+}
+
+// A key whose unsigned order is SQL order: NaN above every number and one
+// value, -0 equal to 0 (SemiringElement::sql_cmp).
+template <class E>
+__device__ __forceinline__ uint64_t sql_key(E v) {
+  if constexpr (std::is_same<E, double>::value || std::is_same<E, float>::value) {
+    const double d = (double)v;
+    if (d != d) return ~0ull;
+    if (d == 0.0) return 0x8000000000000000ull;
+    const uint64_t b = __double_as_longlong(d);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+  } else if constexpr (std::is_same<E, int64_t>::value) {
+    return (uint64_t)v ^ 0x8000000000000000ull;
+  } else {
+    return (uint64_t)(v != 0);
+  }
+}
+
+template <class E>
+__global__ __launch_bounds__(kThreads) void sql_keys(const E* __restrict__ vals, uint64_t k,
+                                                     uint64_t* __restrict__ keys, uint64_t* __restrict__ pos) {
+  for (uint64_t i = first_index(); i < k; i += stride()) {
+    keys[i] = sql_key(vals[i]);
+    pos[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ord_of(const uint32_t* __restrict__ ord, const uint64_t* __restrict__ pos,
+                                                   uint64_t k, uint32_t* __restrict__ out) {
+  for (uint64_t i = first_index(); i < k; i += stride()) out[i] = ord[pos[i]];
+}
+
+// The mode of receiver i's collected messages, now in (value, collect)
+// order at [off[i], off[i+1]): the longest run of one SQL value, the
+// first such (the smallest value) on a tie, answered by its first message.
+template <class E>
+__global__ __launch_bounds__(kThreads) void mode_pick(const uint64_t* __restrict__ off, uint64_t nrecv,
+                                                      const uint64_t* __restrict__ pos, const E* __restrict__ vals,
+                                                      E* __restrict__ out) {
+  for (uint64_t i = first_index(); i < nrecv; i += stride()) {
+    uint64_t best_at = off[i], best_len = 0;
+    uint64_t k = off[i];
+    while (k < off[i + 1]) {
+      const uint64_t key = sql_key(vals[pos[k]]);
+      uint64_t j = k + 1;
+      while (j < off[i + 1] && sql_key(vals[pos[j]]) == key) ++j;
+      if (j - k > best_len) {
+        best_len = j - k;
+        best_at = k;
+      }
+      k = j;
+    }
+    out[i] = vals[pos[best_at]];
+  }
+}
+
+// A device buffer charged to the context, freed on scope exit.
+template <class T>
+struct Buf {
+  wg_context* ctx = nullptr;
+  T* p = nullptr;
+  size_t bytes = 0;
+  int alloc(wg_context* c, uint64_t count) {
+    ctx = c;
+    bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+    return dalloc(ctx, (void**)&p, bytes);
+  }
+  T* release() {
+    T* q = p;
+    p = nullptr;
+    return q;
+  }
+  ~Buf() {
+    if (p) dfree(ctx, p, bytes);
+  }
+};
+
+int combiner_accepts(int combiner, int dtype) {
+  const bool numeric = dtype == WG_F32 || dtype == WG_F64 || dtype == WG_I64;
+  switch (combiner) {
+    case C_COLLECT: case C_COUNT: case C_FIRST: return 1;
+    case C_SUM: case C_MIN: case C_MAX: return numeric;
+    case C_BOOL_OR: case C_BOOL_AND: return dtype == WG_BOOL;
+    case C_CONCAT: return 0;  // bytes messages only
+    case C_MODE: return 1;
+  }
+  return -1;
+}
+
+}  // namespace
+
+// The senders CSR of a gather over g (Sources, neighbourhood.rs): rows are
+// receivers. In and out read a relation selection's reverse or forward
+// CSR; both builds the union, kept with g.
+static int senders_csr(wg_context* ctx, wg_graph* g, int direction, const uint32_t* rels, uint32_t k,
+                       int per_relation, const DevCsr** out) {
+  wg_graph* v;
+  int s;
+  if (direction != WG_BOTH) {
+    if ((s = select_graph(ctx, g, rels, k, per_relation, &v))) return s;
+    *out = direction == WG_IN ? &v->rev : &v->fwd;
+    return WG_OK;
+  }
+  // The relations linking: each once (merged), or each selected relation's
+  // own union, those kept side by side in relation order (per relation).
+  std::vector<uint32_t> sel;
+  if (rels) {
+    sel.assign(rels, rels + k);
+    std::sort(sel.begin(), sel.end());
+    sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
+    for (uint32_t r : sel)
+      if (r >= g->relation_count)
+        return fail(ctx, WG_UNKNOWN_RELATION, "relation " + std::to_string(r) + " is not in the graph");
+  } else {
+    for (uint32_t r = 0; r < g->relation_count; ++r) sel.push_back(r);
+  }
+  const bool per = per_relation && g->relations.size() > 1 && sel.size() > 1;
+  std::string key = std::string("b") + (per ? "p" : "m");
+  for (uint32_t r : sel) key += ":" + std::to_string(r);
+  auto it = g->selections.find(key);
+  if (it != g->selections.end()) {
+    *out = &it->second->fwd;
+    return WG_OK;
+  }
+  auto relation = [&](uint32_t r) -> wg_graph* { return g->relations.empty() ? g : g->relations[r]; };
+  // The merged view of all selected relations serves the merged union.
+  std::vector<wg_graph*> parts;
+  if (per) {
+    for (uint32_t r : sel) parts.push_back(relation(r));
+  } else {
+    wg_graph* merged;
+    if ((s = select_graph(ctx, g, rels, k, 0, &merged))) return s;
+    parts.push_back(merged);
+  }
+  // Each part's union of forward and reverse rows, folded.
+  std::vector<DevCsr> unions(parts.size());
+  uint64_t total = 0;
+  auto free_unions = [&]() {
+    for (DevCsr& c : unions) free_csr(ctx, c);
+  };
+  for (size_t i = 0; i < parts.size(); ++i) {
+    const wg_graph* p = parts[i];
+    const uint64_t m2 = p->fwd.nnz + p->rev.nnz;
+    Buf<uint32_t> src, dst;
+    if ((s = src.alloc(ctx, m2)) || (s = dst.alloc(ctx, m2)) || (s = csr_edges(ctx, p->fwd, src.p, dst.p)) ||
+        (s = csr_edges(ctx, p->rev, src.p + p->fwd.nnz, dst.p + p->fwd.nnz)) ||
+        (s = csr_of_edges(ctx, g->n, m2, src.p, dst.p, false, unions[i]))) {
+      free_unions();
+      return s;
+    }
+    total += unions[i].nnz;
+  }
+  wg_graph* view = new (std::nothrow) wg_graph();
+  if (!view) {
+    free_unions();
+    return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  }
+  view->ctx = ctx;
+  view->n = g->n;
+  view->owner = g;
+  if (parts.size() == 1) {
+    view->fwd = std::move(unions[0]);
+    unions.clear();
+  } else {
+    // Side by side in relation order: the union of each, kept as it is.
+    Buf<uint32_t> src, dst;
+    if ((s = src.alloc(ctx, total)) || (s = dst.alloc(ctx, total))) {
+      free_unions();
+      destroy_graph(view);
+      return s;
+    }
+    uint64_t base = 0;
+    for (DevCsr& c : unions) {
+      if ((s = csr_edges(ctx, c, src.p + base, dst.p + base))) break;
+      base += c.nnz;
+    }
+    if (!s) s = csr_of_edges(ctx, g->n, total, src.p, dst.p, true, view->fwd);
+    free_unions();
+    if (s) {
+      destroy_graph(view);
+      return s;
+    }
+  }
+  view->nnz = view->fwd.nnz;
+  g->selections[key] = view;
+  *out = &view->fwd;
+  return WG_OK;
+}
+
+}  // namespace wg
+
+using namespace wg;
+
+struct wg_gathered {
+  wg_context* ctx = nullptr;
+  int combiner = 0, dtype = 0;
+  uint64_t nrecv = 0, nvals = 0;
+  uint32_t* recv = nullptr;   // nrecv receivers, ascending
+  void* vals = nullptr;       // nvals values (count: i64)
+  uint64_t* offs = nullptr;   // collect: nrecv + 1 offsets into vals
+  size_t recv_bytes = 0, vals_bytes = 0, offs_bytes = 0;
+  ~wg_gathered() {
+    if (ctx) {
+      dfree(ctx, recv, recv_bytes);
+      dfree(ctx, vals, vals_bytes);
+      dfree(ctx, offs, offs_bytes);
+    }
+  }
+};
+
+namespace wg {
+
+template <class E>
+static int run_gather(wg_context* ctx, const Receiving& R0, uint64_t n, int combiner, const E* mval,
+                      wg_gathered* out) {
+  hipStream_t st = ctx->stream;
+  const unsigned grid = grid_of(n);
+  Receiving R = R0;
+  int s;
+  Buf<uint64_t> cnt, flags, rank;
+  Buf<E> val;
+  if ((s = cnt.alloc(ctx, n)) || (s = flags.alloc(ctx, n)) || (s = rank.alloc(ctx, n)) || (s = val.alloc(ctx, n)))
+    return s;
+  if ((s = clear_error(ctx))) return s;
+  switch (combiner) {
+    case C_SUM: fold_receivers<E, PLUS><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    case C_MIN: fold_receivers<E, MIN><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    case C_MAX: fold_receivers<E, MAX><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    case C_BOOL_OR: fold_receivers<E, LOR><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    case C_BOOL_AND: fold_receivers<E, LAND><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    case C_FIRST: fold_receivers<E, FIRST><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+    default: fold_receivers<E, -1><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
+  }
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = collect_error(ctx, "gather"))) return s;
+  // Ranks of the receivers that got a message.
+  received_flags<<<grid, kThreads, 0, st>>>(cnt.p, n, flags.p);
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = exclusive_scan_u64(ctx, flags.p, rank.p, n))) return s;
+  uint64_t last_rank = 0, last_flag = 0;
+  if (n) {
+    WG_HIP(ctx, hipMemcpyAsync(&last_rank, rank.p + n - 1, 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipMemcpyAsync(&last_flag, flags.p + n - 1, 8, hipMemcpyDeviceToHost, st));
+  }
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  const uint64_t nrecv = last_rank + last_flag;
+  out->nrecv = nrecv;
+  out->recv_bytes = std::max<uint64_t>(nrecv, 1) * 4;
+  if ((s = dalloc(ctx, (void**)&out->recv, out->recv_bytes))) return s;
+  const bool listing = combiner == C_COLLECT || combiner == C_MODE;
+  if (!listing) {
+    out->nvals = nrecv;
+    out->vals_bytes = std::max<uint64_t>(nrecv, 1) * (combiner == C_COUNT ? 8 : sizeof(E));
+    if ((s = dalloc(ctx, &out->vals, out->vals_bytes))) return s;
+    compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, val.p, nullptr, out->recv,
+                                                    combiner == C_COUNT ? nullptr : (E*)out->vals,
+                                                    combiner == C_COUNT ? (int64_t*)out->vals : nullptr, nullptr);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    return WG_OK;
+  }
+  // Collected lists: each receiver's start, the messages in collect order.
+  Buf<uint64_t> start;
+  if ((s = start.alloc(ctx, n + 1))) return s;
+  if ((s = exclusive_scan_u64(ctx, cnt.p, start.p, n))) return s;
+  uint64_t total = 0, tail = 0;
+  if (n) {
+    WG_HIP(ctx, hipMemcpyAsync(&total, start.p + n - 1, 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipMemcpyAsync(&tail, cnt.p + n - 1, 8, hipMemcpyDeviceToHost, st));
+  }
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  total += tail;
+  Buf<E> coll;
+  Buf<uint32_t> ord;
+  if ((s = coll.alloc(ctx, total))) return s;
+  if (combiner == C_MODE && (s = ord.alloc(ctx, total))) return s;
+  collect_fill<E><<<grid, kThreads, 0, st>>>(R, n, mval, start.p, rank.p, coll.p,
+                                             combiner == C_MODE ? ord.p : nullptr);
+  out->offs_bytes = (nrecv + 1) * 8;
+  if ((s = dalloc(ctx, (void**)&out->offs, out->offs_bytes))) return s;
+  compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, nullptr, start.p, out->recv, nullptr, nullptr,
+                                                  out->offs);
+  WG_HIP(ctx, hipGetLastError());
+  WG_HIP(ctx, hipMemcpyAsync(out->offs + nrecv, &total, 8, hipMemcpyHostToDevice, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  if (combiner == C_COLLECT) {
+    out->nvals = total;
+    out->vals_bytes = coll.bytes;
+    out->vals = coll.release();
+    return WG_OK;
+  }
+  // Mode: (receiver, SQL value) order, collect order kept within.
+  Buf<uint64_t> keys, keys2, pos, pos2;
+  Buf<uint32_t> ord2, ord3;
+  if ((s = keys.alloc(ctx, total)) || (s = keys2.alloc(ctx, total)) || (s = pos.alloc(ctx, total)) ||
+      (s = pos2.alloc(ctx, total)) || (s = ord2.alloc(ctx, total)) || (s = ord3.alloc(ctx, total)))
+    return s;
+  sql_keys<E><<<grid_of(total), kThreads, 0, st>>>(coll.p, total, keys.p, pos.p);
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = sort_u64_keyed(ctx, keys.p, keys2.p, pos.p, pos2.p, total, 64))) return s;
+  ord_of<<<grid_of(total), kThreads, 0, st>>>(ord.p, pos2.p, total, ord2.p);
+  WG_HIP(ctx, hipGetLastError());
+  unsigned bits = 1;
+  while (bits < 32 && (1ull << bits) < nrecv) ++bits;
+  if ((s = sort_u32_keyed(ctx, ord2.p, ord3.p, pos2.p, pos.p, total, bits))) return s;
+  out->nvals = nrecv;
+  out->vals_bytes = std::max<uint64_t>(nrecv, 1) * sizeof(E);
+  if ((s = dalloc(ctx, &out->vals, out->vals_bytes))) return s;
+  mode_pick<E><<<grid_of(nrecv), kThreads, 0, st>>>(out->offs, nrecv, pos.p, coll.p, (E*)out->vals);
+  WG_HIP(ctx, hipGetLastError());
+  // The lists were only the means: a mode result is one value a receiver.
+  dfree(ctx, out->offs, out->offs_bytes);
+  out->offs = nullptr;
+  out->offs_bytes = 0;
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  return WG_OK;
+}
+
+}  // namespace wg
+
+extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, int dtype,
+                         uint64_t count, const uint32_t* senders, const void* messages, wg_gathered** out) {
+  if (!ctx || !gc || !o || !out) return WG_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (count && (!senders || !messages)) return fail(ctx, WG_INVALID_ARGUMENT, "null messages");
+  if (o->direction != WG_OUT && o->direction != WG_IN && o->direction != WG_BOTH)
+    return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
+  if (o->nrelations && !o->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
+  const int acc = combiner_accepts(o->combiner, dtype);
+  if (acc < 0) return fail(ctx, WG_INVALID_ARGUMENT, "unknown combiner");
+  if (dtype != WG_F32 && dtype != WG_F64 && dtype != WG_I64 && dtype != WG_BOOL)
+    return fail(ctx, WG_UNSUPPORTED, "messages of this element type are not gathered on the device");
+  // Combiner::check (combiner.rs:82-94).
+  if (!acc) return fail(ctx, WG_COMBINER_TYPE, "the combiner does not accept messages of this type");
+  wg_graph* g = const_cast<wg_graph*>(gc);
+  const uint64_t n = g->n;
+  if (o->mask) {
+    if (o->mask->id_space != n) return fail(ctx, WG_ID_SPACE_MISMATCH, "mask id space mismatch");
+    if (o->mask->width != 1) return fail(ctx, WG_WIDTH_MISMATCH, "mask width must be 1");
+  }
+  if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int s;
+  const size_t es = dtype_size(dtype);
+  // The outbox: messages sorted by sender, push order kept (outbox.rs:41-75).
+  Buf<uint32_t> dsend, sorted, idx, perm;
+  Buf<uint8_t> dval, mval;
+  Buf<uint64_t> moff;
+  if ((s = dsend.alloc(ctx, count)) || (s = dval.alloc(ctx, count * es)) || (s = moff.alloc(ctx, n + 1)))
+    return s;
+  if (count) {
+    WG_HIP(ctx, hipMemcpyAsync(dsend.p, senders, count * 4, hipMemcpyHostToDevice, st));
+    WG_HIP(ctx, hipMemcpyAsync(dval.p, messages, count * es, hipMemcpyHostToDevice, st));
+    Buf<unsigned long long> first;
+    if ((s = first.alloc(ctx, 1))) return s;
+    WG_HIP(ctx, hipMemsetAsync(first.p, 0xff, 8, st));
+    first_sender_outside<<<grid_of(count), kThreads, 0, st>>>(dsend.p, count, n, first.p);
+    WG_HIP(ctx, hipGetLastError());
+    unsigned long long bad = 0;
+    WG_HIP(ctx, hipMemcpyAsync(&bad, first.p, 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    if (bad != ~0ull)
+      return fail(ctx, WG_SENDER_OUTSIDE_ID_SPACE,
+                  "sender " + std::to_string(senders[bad]) + " outside the id space " + std::to_string(n));
+  }
+  if ((s = sorted.alloc(ctx, count)) || (s = idx.alloc(ctx, count)) || (s = perm.alloc(ctx, count)) ||
+      (s = mval.alloc(ctx, count * es)))
+    return s;
+  if (count) {
+    iota32<<<grid_of(count), kThreads, 0, st>>>(idx.p, count);
+    unsigned bits = 1;
+    while (bits < 32 && (1ull << bits) < n) ++bits;
+    if ((s = sort_u32_pairs(ctx, dsend.p, sorted.p, idx.p, perm.p, count, bits))) return s;
+    switch (dtype) {
+      case WG_F32: permute<float><<<grid_of(count), kThreads, 0, st>>>(perm.p, (const float*)dval.p, count, (float*)mval.p); break;
+      case WG_F64: permute<double><<<grid_of(count), kThreads, 0, st>>>(perm.p, (const double*)dval.p, count, (double*)mval.p); break;
+      case WG_I64: permute<int64_t><<<grid_of(count), kThreads, 0, st>>>(perm.p, (const int64_t*)dval.p, count, (int64_t*)mval.p); break;
+      default: permute<uint8_t><<<grid_of(count), kThreads, 0, st>>>(perm.p, (const uint8_t*)dval.p, count, (uint8_t*)mval.p); break;
+    }
+    WG_HIP(ctx, hipGetLastError());
+  }
+  sender_starts<<<grid_of(n + 1), kThreads, 0, st>>>(sorted.p, count, n, moff.p);
+  WG_HIP(ctx, hipGetLastError());
+  // The senders of each receiver.
+  const DevCsr* S = nullptr;
+  if ((s = senders_csr(ctx, g, o->direction, o->relations, o->nrelations, o->per_relation, &S))) return s;
+  Receiving R{S->offsets, S->targets, moff.p, o->mask ? o->mask->data : nullptr, o->mask ? o->mask->dtype : 0,
+              o->mask_complement, o->include_self};
+  wg_gathered* res = new (std::nothrow) wg_gathered();
+  if (!res) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
+  res->ctx = ctx;
+  res->combiner = o->combiner;
+  res->dtype = dtype;
+  switch (dtype) {
+    case WG_F32: s = run_gather<float>(ctx, R, n, o->combiner, (const float*)mval.p, res); break;
+    case WG_F64: s = run_gather<double>(ctx, R, n, o->combiner, (const double*)mval.p, res); break;
+    case WG_I64: s = run_gather<int64_t>(ctx, R, n, o->combiner, (const int64_t*)mval.p, res); break;
+    default: s = run_gather<uint8_t>(ctx, R, n, o->combiner, (const uint8_t*)mval.p, res); break;
+  }
+  if (s) {
+    delete res;
+    return s;
+  }
+  *out = res;
+  return ok(ctx);
+}
+
+extern "C" int wg_gathered_info(const wg_gathered* r, uint64_t* receivers, uint64_t* values) {
+  if (!r) return WG_INVALID_ARGUMENT;
+  if (receivers) *receivers = r->nrecv;
+  if (values) *values = r->nvals;
+  return WG_OK;
+}
+
+extern "C" int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t* receivers, void* values,
+                                uint64_t* offsets) {
+  if (!ctx || !r) return WG_INVALID_ARGUMENT;
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (receivers && r->nrecv) {
+    std::vector<uint32_t> tmp(r->nrecv);
+    WG_HIP(ctx, hipMemcpy(tmp.data(), r->recv, r->nrecv * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < r->nrecv; ++i) receivers[i] = tmp[i];
+  }
+  const size_t es = r->combiner == 1 ? 8 : dtype_size(r->dtype);
+  if (values && r->nvals) WG_HIP(ctx, hipMemcpy(values, r->vals, r->nvals * es, hipMemcpyDeviceToHost));
+  if (offsets && r->offs) WG_HIP(ctx, hipMemcpy(offsets, r->offs, (r->nrecv + 1) * 8, hipMemcpyDeviceToHost));
+  return ok(ctx);
+}
+
+extern "C" void wg_gathered_destroy(wg_gathered* r) { delete r; }

@@ -174,6 +174,18 @@ int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of);
 // on the context stream.
 int sort_u32_pairs(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout,
                    uint64_t m, unsigned bits);
+int sort_u64_keyed(wg_context* ctx, const uint64_t* kin, uint64_t* kout, const uint64_t* vin, uint64_t* vout,
+                   uint64_t m, unsigned bits);
+int sort_u32_keyed(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint64_t* vin, uint64_t* vout,
+                   uint64_t m, unsigned bits);
+// out[i] = in[0] + ... + in[i-1] (out[0] = 0), on the context stream.
+int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m);
+// A CSR's entries as (row, column) pairs (nnz of each).
+int csr_edges(wg_context* ctx, const DevCsr& c, uint32_t* src, uint32_t* dst);
+// The unweighted, partitioned CSR of a device edge list: parallel edges
+// folded, or kept (in input order) with keep.
+int csr_of_edges(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, bool keep,
+                 DevCsr& out);
 }  // namespace wg
 
 namespace wg {
