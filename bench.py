@@ -296,6 +296,36 @@ def bench_build(W, ctx, scale=22, reps=3):
     return res, {"n": n, "src": src, "dst": dst, "w": w, "fwd": fwd, "rev": rev}
 
 
+def bench_gather(W, ctx, g, n, reps=3):
+    """gather over the bench graph: one f64 message from every node, summed at
+    each receiver from its in-neighbours (the message-passing form of a
+    transposed plus_second product), and the same messages collected."""
+    rng = np.random.default_rng(7)
+    senders = np.arange(n, dtype=np.uint32)
+    msgs = rng.random(n)
+    opts = W._lib.GatherOpts(W.COMBINERS.index("sum"), 1, None, 0, 0, None, 0, 0)
+    res = {}
+    for comb in ("sum", "collect"):
+        opts.combiner = W.COMBINERS.index(comb)
+        times = []
+        for _ in range(reps + 1):
+            h = C.c_void_p()
+            t1 = time.perf_counter()
+            ctx.check(ctx.lib.wg_gather(ctx.h, g.h, C.byref(opts), 1, n, senders.ctypes.data,
+                                        msgs.ctypes.data, C.byref(h)))
+            times.append(time.perf_counter() - t1)
+            nr, nv = C.c_uint64(), C.c_uint64()
+            ctx.lib.wg_gathered_info(h, C.byref(nr), C.byref(nv))
+            ctx.lib.wg_gathered_destroy(h)
+        ms = min(times[1:]) * 1e3
+        res[comb] = {"ms_best_of_%d" % reps: round(ms, 2), "receivers": int(nr.value),
+                     "values": int(nv.value)}
+        log(f"gather {comb}: {ms:.1f} ms, {nr.value} receivers, {nv.value} values")
+    res["messages"] = n
+    res["direction"] = "in"
+    return res
+
+
 def bench_segments(W, ctx, off, tgt, w, reps=3):
     """The segment loader (wg_graph_load_segments) on the bench graph's
     forward CSR as WRSG segments with sparse offsets (the rows segment) and
@@ -637,6 +667,7 @@ def main():
     legs["graph_upload"] = {"ms": round(t_up2 * 1e3, 1), "ms_first": round(t_up * 1e3, 1),
                             "what": "host CSR (u64 offsets, u32 targets, f64 weights) copied in, "
                                     "reverse CSR transposed on the device, both partitioned"}
+    legs["gather_in_f64"] = bench_gather(W, ctx, gf, n)
     build_state = adj_state = lz_state = None
     if not args.skip_build:
         legs["csr_build"], build_state = bench_build(W, ctx, scale=args.scale)

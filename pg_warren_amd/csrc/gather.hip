@@ -12,13 +12,18 @@
 //   senders    a CSR per receiver: a relation selection's forward or
 //              reverse CSR, or for both directions the union of the two
 //              built like a selection; kept with the graph;
-//   combine    a thread per receiver walks its senders' runs in collect
-//              order and folds with the reference's operator rules
-//              (SemiringElement::apply: i64 sums checked, min and max in
-//              SQL order, the first operand kept on ties), or counts, or
-//              keeps the first;
-//   collect    counts, an exclusive scan, then each receiver writes its
-//              messages at its offset;
+//   collect    each sender entry's message count, prefix sums over the
+//              entries and over the receivers give every (receiver, sender)
+//              run its place, so a thread per entry copies its sender's
+//              messages there: each receiver's messages contiguous, in
+//              collect order, written in parallel;
+//   combine    a thread per receiver (a 64-lane block for a segment of 512
+//              or more, staging 256 at a time in LDS) folds its contiguous
+//              messages left to right with the
+//              reference's operator rules (SemiringElement::apply: i64 sums
+//              checked, min and max in SQL order, the first operand kept on
+//              ties), so float sums are the reference's bit for bit; count
+//              needs only the counts;
 //   mode       the collected lists sorted by (receiver, value in SQL order)
 //              stably, so a run of equal values keeps collect order; a
 //              receiver's longest run wins, the smallest value on a tie,
@@ -31,6 +36,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "semiring.hpp"
@@ -97,64 +103,163 @@ __device__ __forceinline__ bool selected(const Receiving& R, uint64_t r) {
   return !R.mask || mask_selects(R.mask, R.mask_dtype, R.mask_complement, r);
 }
 
-// Calls visit(k) for receiver r's messages k in collect order.
-template <class F>
-__device__ __forceinline__ void each_message(const Receiving& R, uint64_t r, F&& visit) {
-  if (R.self)
-    for (uint64_t k = R.moff[r]; k < R.moff[r + 1]; ++k) visit(k);
-  if (R.soff)
-    for (uint64_t p = R.soff[r]; p < R.soff[r + 1]; ++p) {
-      const uint32_t s = R.stgt[p];
-      for (uint64_t k = R.moff[s]; k < R.moff[s + 1]; ++k) visit(k);
+// ecnt[p] = the messages of entry p's sender (ecnt[m] = 0 closes the scan).
+__global__ __launch_bounds__(kThreads) void entry_counts(const uint32_t* __restrict__ stgt, uint64_t m,
+                                                         const uint64_t* __restrict__ moff,
+                                                         uint64_t* __restrict__ ecnt) {
+  for (uint64_t p = first_index(); p <= m; p += stride()) {
+    if (p == m) {
+      ecnt[p] = 0;
+    } else {
+      const uint32_t s = stgt[p];
+      ecnt[p] = moff[s + 1] - moff[s];
     }
+  }
 }
 
-// Folds each receiver's messages with OP (a BinaryOp, or -1 to count only):
-// cnt[r] messages, val[r] their fold; the lowest failing receiver raises.
-template <class E, int OP>
-__global__ __launch_bounds__(kThreads) void fold_receivers(Receiving R, uint64_t n, const E* __restrict__ mval,
-                                                           E* __restrict__ val, uint64_t* __restrict__ cnt,
-                                                           unsigned long long* __restrict__ err) {
-  for (uint64_t r = first_index(); r < n; r += stride()) {
-    if (!selected(R, r)) {
+// cnt[r] = the messages receiver r gets: its own (include_self) and its
+// senders' (a difference of the entries' prefix sums); 0 outside the mask.
+// cnt[n] = 0 closes the scan.
+__global__ __launch_bounds__(kThreads) void receiver_counts(Receiving R, uint64_t n,
+                                                            const uint64_t* __restrict__ eprefix,
+                                                            uint64_t* __restrict__ cnt) {
+  for (uint64_t r = first_index(); r <= n; r += stride()) {
+    if (r == n || !selected(R, r)) {
       cnt[r] = 0;
       continue;
     }
-    uint64_t c = 0;
-    E acc{};
-    uint32_t e_err = 0;
-    each_message(R, r, [&](uint64_t k) {
-      if constexpr (OP >= 0) {
-        const E v = mval[k];
-        if (c == 0)
-          acc = v;
-        else if (!e_err)
-          acc = apply<OP>(acc, v, e_err);
-      }
-      ++c;
-    });
+    uint64_t c = R.self ? R.moff[r + 1] - R.moff[r] : 0;
+    if (R.soff) c += eprefix[R.soff[r + 1]] - eprefix[R.soff[r]];
     cnt[r] = c;
-    if constexpr (OP >= 0) val[r] = acc;
+  }
+}
+
+// The collected messages: a receiver's own first (thread per receiver),
+// then each sender entry's run at its prefix position (thread per entry),
+// so every receiver's messages lie in collect order at [start[r],
+// start[r + 1]). ord, if given, gets the receiver's rank at each message.
+template <class E>
+__global__ __launch_bounds__(kThreads) void fill_self(Receiving R, uint64_t n, const E* __restrict__ mval,
+                                                      const uint64_t* __restrict__ start,
+                                                      const uint64_t* __restrict__ rank, E* __restrict__ out,
+                                                      uint32_t* __restrict__ ord) {
+  for (uint64_t r = first_index(); r < n; r += stride()) {
+    if (!R.self || !selected(R, r)) continue;
+    uint64_t at = start[r];
+    for (uint64_t k = R.moff[r]; k < R.moff[r + 1]; ++k, ++at) {
+      out[at] = mval[k];
+      if (ord) ord[at] = (uint32_t)rank[r];
+    }
+  }
+}
+
+template <class E>
+__global__ __launch_bounds__(kThreads) void fill_entries(Receiving R, uint64_t m, const uint32_t* __restrict__ row_of,
+                                                         const uint64_t* __restrict__ eprefix,
+                                                         const E* __restrict__ mval,
+                                                         const uint64_t* __restrict__ start,
+                                                         const uint64_t* __restrict__ rank, E* __restrict__ out,
+                                                         uint32_t* __restrict__ ord) {
+  for (uint64_t p = first_index(); p < m; p += stride()) {
+    const uint32_t r = row_of[p];
+    if (!selected(R, r)) continue;
+    const uint32_t s = R.stgt[p];
+    uint64_t at = start[r] + (R.self ? R.moff[r + 1] - R.moff[r] : 0) + eprefix[p] - eprefix[R.soff[r]];
+    for (uint64_t k = R.moff[s]; k < R.moff[s + 1]; ++k, ++at) {
+      out[at] = mval[k];
+      if (ord) ord[at] = (uint32_t)rank[r];
+    }
+  }
+}
+
+// Left fold of each receiver's collected messages with OP (SemiringElement::
+// apply, in collect order); the lowest failing receiver raises. A thread
+// per receiver folds a segment shorter than kLong with sixteen loads ahead
+// of the fold; longer segments are listed (in any order: each is folded on
+// its own) for fold_long.
+constexpr uint64_t kLong = 512;
+
+template <class E, int OP>
+__device__ __forceinline__ void fold_into(E& acc, bool& first, uint32_t& e_err, E x) {
+  if (first) {
+    acc = x;
+    first = false;
+  } else if (!e_err) {
+    acc = apply<OP>(acc, x, e_err);
+  }
+}
+
+template <class E, int OP>
+__global__ __launch_bounds__(kThreads) void fold_segments(uint64_t n, const uint64_t* __restrict__ start,
+                                                          const E* __restrict__ coll, E* __restrict__ val,
+                                                          uint32_t* __restrict__ long_list,
+                                                          unsigned* __restrict__ nlong,
+                                                          unsigned long long* __restrict__ err) {
+  for (uint64_t r = first_index(); r < n; r += stride()) {
+    const uint64_t a = start[r], b = start[r + 1];
+    if (a == b) continue;
+    if (b - a >= kLong) {
+      long_list[atomicAdd(nlong, 1u)] = (uint32_t)r;
+      continue;
+    }
+    E acc{};
+    bool first = true;
+    uint32_t e_err = 0;
+    constexpr int kAhead = 16;
+    uint64_t k = a;
+    for (; k + kAhead <= b && !e_err; k += kAhead) {
+      E v[kAhead];
+#pragma unroll
+      for (int t = 0; t < kAhead; ++t) v[t] = coll[k + t];
+#pragma unroll
+      for (int t = 0; t < kAhead; ++t) fold_into<E, OP>(acc, first, e_err, v[t]);
+    }
+    for (; k < b && !e_err; ++k) fold_into<E, OP>(acc, first, e_err, coll[k]);
+    val[r] = acc;
     if (e_err) raise_error(err, r, e_err);
   }
 }
 
-// Each receiver's messages in collect order at its offset; ord[i] = the
-// receiver's rank among those that received (for mode).
-template <class E>
-__global__ __launch_bounds__(kThreads) void collect_fill(Receiving R, uint64_t n, const E* __restrict__ mval,
-                                                         const uint64_t* __restrict__ start,
-                                                         const uint64_t* __restrict__ rank, E* __restrict__ out,
-                                                         uint32_t* __restrict__ ord) {
-  for (uint64_t r = first_index(); r < n; r += stride()) {
-    if (!selected(R, r)) continue;
-    uint64_t at = start[r];
-    const uint32_t rk = (uint32_t)rank[r];
-    each_message(R, r, [&](uint64_t k) {
-      out[at] = mval[k];
-      if (ord) ord[at] = rk;
-      ++at;
-    });
+// A long segment, one 64-lane block each: the block stages 256 messages at
+// a time in LDS, the next 256 loading into registers while lane 0 folds
+// the staged ones in order.
+constexpr unsigned kLongThreads = 64, kStage = 256;
+
+template <class E, int OP>
+__global__ __launch_bounds__(kLongThreads) void fold_long(const uint32_t* __restrict__ long_list,
+                                                          const uint64_t* __restrict__ start,
+                                                          const E* __restrict__ coll, E* __restrict__ val,
+                                                          unsigned long long* __restrict__ err) {
+  __shared__ E stage[kStage];
+  const uint32_t r = long_list[blockIdx.x];
+  const uint64_t a = start[r], b = start[r + 1];
+  const unsigned lane = threadIdx.x;
+  E next[kStage / kLongThreads];
+  auto load = [&](uint64_t base) {
+#pragma unroll
+    for (unsigned c = 0; c < kStage / kLongThreads; ++c) {
+      const uint64_t k = base + c * kLongThreads + lane;
+      next[c] = k < b ? coll[k] : E{};
+    }
+  };
+  E acc{};
+  bool first = true;
+  uint32_t e_err = 0;
+  load(a);
+  for (uint64_t base = a; base < b; base += kStage) {
+#pragma unroll
+    for (unsigned c = 0; c < kStage / kLongThreads; ++c) stage[c * kLongThreads + lane] = next[c];
+    __syncthreads();
+    if (base + kStage < b) load(base + kStage);
+    if (lane == 0) {
+      const unsigned cnt = (unsigned)(b - base < kStage ? b - base : kStage);
+      for (unsigned t = 0; t < cnt; ++t) fold_into<E, OP>(acc, first, e_err, stage[t]);
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    val[r] = acc;
+    if (e_err) raise_error(err, r, e_err);
   }
 }
 
@@ -255,9 +360,11 @@ struct Buf {
     p = nullptr;
     return q;
   }
-  ~Buf() {
+  void reset() {
     if (p) dfree(ctx, p, bytes);
+    p = nullptr;
   }
+  ~Buf() { reset(); }
 };
 
 int combiner_accepts(int combiner, int dtype) {
@@ -396,71 +503,99 @@ struct wg_gathered {
 namespace wg {
 
 template <class E>
-static int run_gather(wg_context* ctx, const Receiving& R0, uint64_t n, int combiner, const E* mval,
+static int run_gather(wg_context* ctx, const Receiving& R, const DevCsr& S, uint64_t n, int combiner, const E* mval,
                       wg_gathered* out) {
   hipStream_t st = ctx->stream;
   const unsigned grid = grid_of(n);
-  Receiving R = R0;
+  const uint64_t m = S.nnz;
   int s;
-  Buf<uint64_t> cnt, flags, rank;
-  Buf<E> val;
-  if ((s = cnt.alloc(ctx, n)) || (s = flags.alloc(ctx, n)) || (s = rank.alloc(ctx, n)) || (s = val.alloc(ctx, n)))
+  // Message counts: per sender entry, prefix sums over the entries, per
+  // receiver, then each receiver's start in the collected order.
+  Buf<uint64_t> ecnt, eprefix, cnt, start, flags, rank;
+  if ((s = ecnt.alloc(ctx, m + 1)) || (s = eprefix.alloc(ctx, m + 1)) || (s = cnt.alloc(ctx, n + 1)) ||
+      (s = start.alloc(ctx, n + 1)) || (s = flags.alloc(ctx, n)) || (s = rank.alloc(ctx, n)))
     return s;
-  if ((s = clear_error(ctx))) return s;
-  switch (combiner) {
-    case C_SUM: fold_receivers<E, PLUS><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    case C_MIN: fold_receivers<E, MIN><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    case C_MAX: fold_receivers<E, MAX><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    case C_BOOL_OR: fold_receivers<E, LOR><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    case C_BOOL_AND: fold_receivers<E, LAND><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    case C_FIRST: fold_receivers<E, FIRST><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-    default: fold_receivers<E, -1><<<grid, kThreads, 0, st>>>(R, n, mval, val.p, cnt.p, ctx->d_err); break;
-  }
+  entry_counts<<<grid_of(m + 1), kThreads, 0, st>>>(S.targets, m, R.moff, ecnt.p);
   WG_HIP(ctx, hipGetLastError());
-  if ((s = collect_error(ctx, "gather"))) return s;
+  if ((s = exclusive_scan_u64(ctx, ecnt.p, eprefix.p, m + 1))) return s;
+  receiver_counts<<<grid_of(n + 1), kThreads, 0, st>>>(R, n, eprefix.p, cnt.p);
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = exclusive_scan_u64(ctx, cnt.p, start.p, n + 1))) return s;
   // Ranks of the receivers that got a message.
   received_flags<<<grid, kThreads, 0, st>>>(cnt.p, n, flags.p);
   WG_HIP(ctx, hipGetLastError());
   if ((s = exclusive_scan_u64(ctx, flags.p, rank.p, n))) return s;
-  uint64_t last_rank = 0, last_flag = 0;
+  uint64_t last_rank = 0, last_flag = 0, total = 0;
   if (n) {
     WG_HIP(ctx, hipMemcpyAsync(&last_rank, rank.p + n - 1, 8, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipMemcpyAsync(&last_flag, flags.p + n - 1, 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipMemcpyAsync(&total, start.p + n, 8, hipMemcpyDeviceToHost, st));
   }
   WG_HIP(ctx, hipStreamSynchronize(st));
   const uint64_t nrecv = last_rank + last_flag;
   out->nrecv = nrecv;
   out->recv_bytes = std::max<uint64_t>(nrecv, 1) * 4;
   if ((s = dalloc(ctx, (void**)&out->recv, out->recv_bytes))) return s;
-  const bool listing = combiner == C_COLLECT || combiner == C_MODE;
-  if (!listing) {
+  if (combiner == C_COUNT) {
     out->nvals = nrecv;
-    out->vals_bytes = std::max<uint64_t>(nrecv, 1) * (combiner == C_COUNT ? 8 : sizeof(E));
+    out->vals_bytes = std::max<uint64_t>(nrecv, 1) * 8;
     if ((s = dalloc(ctx, &out->vals, out->vals_bytes))) return s;
-    compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, val.p, nullptr, out->recv,
-                                                    combiner == C_COUNT ? nullptr : (E*)out->vals,
-                                                    combiner == C_COUNT ? (int64_t*)out->vals : nullptr, nullptr);
+    compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, nullptr, nullptr, out->recv, nullptr,
+                                                    (int64_t*)out->vals, nullptr);
     WG_HIP(ctx, hipGetLastError());
     WG_HIP(ctx, hipStreamSynchronize(st));
     return WG_OK;
   }
-  // Collected lists: each receiver's start, the messages in collect order.
-  Buf<uint64_t> start;
-  if ((s = start.alloc(ctx, n + 1))) return s;
-  if ((s = exclusive_scan_u64(ctx, cnt.p, start.p, n))) return s;
-  uint64_t total = 0, tail = 0;
-  if (n) {
-    WG_HIP(ctx, hipMemcpyAsync(&total, start.p + n - 1, 8, hipMemcpyDeviceToHost, st));
-    WG_HIP(ctx, hipMemcpyAsync(&tail, cnt.p + n - 1, 8, hipMemcpyDeviceToHost, st));
-  }
-  WG_HIP(ctx, hipStreamSynchronize(st));
-  total += tail;
+  // Every receiver's messages in collect order.
   Buf<E> coll;
-  Buf<uint32_t> ord;
-  if ((s = coll.alloc(ctx, total))) return s;
+  Buf<uint32_t> ord, row_of;
+  if ((s = coll.alloc(ctx, total)) || (s = row_of.alloc(ctx, m))) return s;
   if (combiner == C_MODE && (s = ord.alloc(ctx, total))) return s;
-  collect_fill<E><<<grid, kThreads, 0, st>>>(R, n, mval, start.p, rank.p, coll.p,
-                                             combiner == C_MODE ? ord.p : nullptr);
+  if ((s = expand_rows(ctx, S, row_of.p))) return s;
+  uint32_t* ordp = combiner == C_MODE ? ord.p : nullptr;
+  fill_self<E><<<grid, kThreads, 0, st>>>(R, n, mval, start.p, rank.p, coll.p, ordp);
+  fill_entries<E><<<grid_of(m), kThreads, 0, st>>>(R, m, row_of.p, eprefix.p, mval, start.p, rank.p, coll.p, ordp);
+  WG_HIP(ctx, hipGetLastError());
+  row_of.reset();
+  if (combiner != C_COLLECT && combiner != C_MODE) {
+    Buf<E> val;
+    if ((s = val.alloc(ctx, n))) return s;
+    if ((s = clear_error(ctx))) return s;
+    Buf<uint32_t> long_list;
+    Buf<unsigned> nlong;
+    if ((s = long_list.alloc(ctx, n)) || (s = nlong.alloc(ctx, 1))) return s;
+    WG_HIP(ctx, hipMemsetAsync(nlong.p, 0, 4, st));
+    unsigned hlong = 0;
+    auto fold = [&](auto op_tag) -> int {
+      constexpr int OP = decltype(op_tag)::value;
+      fold_segments<E, OP><<<grid, kThreads, 0, st>>>(n, start.p, coll.p, val.p, long_list.p, nlong.p, ctx->d_err);
+      WG_HIP(ctx, hipGetLastError());
+      WG_HIP(ctx, hipMemcpyAsync(&hlong, nlong.p, 4, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      if (hlong)
+        fold_long<E, OP><<<hlong, kLongThreads, 0, st>>>(long_list.p, start.p, coll.p, val.p, ctx->d_err);
+      return WG_OK;
+    };
+    switch (combiner) {
+      case C_SUM: s = fold(std::integral_constant<int, PLUS>()); break;
+      case C_MIN: s = fold(std::integral_constant<int, MIN>()); break;
+      case C_MAX: s = fold(std::integral_constant<int, MAX>()); break;
+      case C_BOOL_OR: s = fold(std::integral_constant<int, LOR>()); break;
+      case C_BOOL_AND: s = fold(std::integral_constant<int, LAND>()); break;
+      default: s = fold(std::integral_constant<int, FIRST>()); break;
+    }
+    if (s) return s;
+    WG_HIP(ctx, hipGetLastError());
+    if ((s = collect_error(ctx, "gather"))) return s;
+    out->nvals = nrecv;
+    out->vals_bytes = std::max<uint64_t>(nrecv, 1) * sizeof(E);
+    if ((s = dalloc(ctx, &out->vals, out->vals_bytes))) return s;
+    compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, val.p, nullptr, out->recv, (E*)out->vals,
+                                                    nullptr, nullptr);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    return WG_OK;
+  }
   out->offs_bytes = (nrecv + 1) * 8;
   if ((s = dalloc(ctx, (void**)&out->offs, out->offs_bytes))) return s;
   compact_receivers<E><<<grid, kThreads, 0, st>>>(cnt.p, rank.p, n, nullptr, start.p, out->recv, nullptr, nullptr,
@@ -578,10 +713,10 @@ extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_op
   res->combiner = o->combiner;
   res->dtype = dtype;
   switch (dtype) {
-    case WG_F32: s = run_gather<float>(ctx, R, n, o->combiner, (const float*)mval.p, res); break;
-    case WG_F64: s = run_gather<double>(ctx, R, n, o->combiner, (const double*)mval.p, res); break;
-    case WG_I64: s = run_gather<int64_t>(ctx, R, n, o->combiner, (const int64_t*)mval.p, res); break;
-    default: s = run_gather<uint8_t>(ctx, R, n, o->combiner, (const uint8_t*)mval.p, res); break;
+    case WG_F32: s = run_gather<float>(ctx, R, *S, n, o->combiner, (const float*)mval.p, res); break;
+    case WG_F64: s = run_gather<double>(ctx, R, *S, n, o->combiner, (const double*)mval.p, res); break;
+    case WG_I64: s = run_gather<int64_t>(ctx, R, *S, n, o->combiner, (const int64_t*)mval.p, res); break;
+    default: s = run_gather<uint8_t>(ctx, R, *S, n, o->combiner, (const uint8_t*)mval.p, res); break;
   }
   if (s) {
     delete res;
