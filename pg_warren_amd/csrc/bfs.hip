@@ -24,6 +24,7 @@
 // no device-scope atomics (on gfx950 they resolve past the per-XCD L2 and
 // serialise on a shared address).
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -34,6 +35,9 @@ namespace wg {
 
 constexpr uint32_t kUnvisited = 0xffffffffu;
 constexpr uint32_t kWordsPerCountBlock = 512;  // bfs_count / bfs_write: 2 words a thread
+// A push level whose frontier rows hold fewer than n / kSmallPushRatio
+// entries sets vias by atomicMin while marking (bfs_mark_min).
+constexpr uint64_t kSmallPushRatio = 4;
 
 struct BfsCounters {
   unsigned long long next_size;    // nodes claimed this level
@@ -87,6 +91,70 @@ __global__ __launch_bounds__(256) void bfs_mark(const uint32_t* __restrict__ que
     for (uint64_t i = a + lane; i < b; i += 64) {
       const uint32_t v = tgt[i];
       if (!bit_of(visited, v) && !(filter && !filter[v])) cand[v] = 1;
+    }
+  }
+}
+
+// Push of a small frontier, whole: each unvisited target takes the smallest
+// frontier node with an edge to it as its via (atomicMin: the first in
+// ascending order, as the reference's push reaches it) and is marked a
+// candidate. A small frontier has few predecessors per target, so the
+// atomics seldom meet; bfs_claim then settles the candidates.
+__global__ __launch_bounds__(256) void bfs_mark_min(const uint32_t* __restrict__ queue, uint64_t qn,
+                                                    const uint64_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ tgt,
+                                                    const uint32_t* __restrict__ visited,
+                                                    const uint8_t* __restrict__ filter,
+                                                    uint8_t* __restrict__ cand, uint32_t* __restrict__ via) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+  for (uint64_t q = wave; q < qn; q += nwaves) {
+    const uint32_t u = queue[q];
+    const uint64_t a = off[u], b = off[u + 1];
+    for (uint64_t i = a + lane; i < b; i += 64) {
+      const uint32_t v = tgt[i];
+      if (!bit_of(visited, v) && !(filter && !filter[v])) {
+        atomicMin(&via[v], u);
+        cand[v] = 1;
+      }
+    }
+  }
+}
+
+// The candidates of a level whose vias are set: each takes depth d1 and its
+// bit in the claim map (a thread per 32-node word, so whole words are
+// stored), and its candidate byte is cleared.
+__global__ __launch_bounds__(256) void bfs_claim(uint64_t n, uint64_t words, uint8_t* __restrict__ cand,
+                                                 uint32_t* __restrict__ depth, uint32_t* __restrict__ claim,
+                                                 uint32_t d1) {
+  for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < words; w += (uint64_t)gridDim.x * 256) {
+    const uint64_t v0 = w * 32;
+    uint32_t bits = 0;
+    if (v0 + 32 <= n) {
+      uint4 c[2];
+      c[0] = ((const uint4*)(cand + v0))[0];
+      c[1] = ((const uint4*)(cand + v0))[1];
+      const uint32_t* cw = (const uint32_t*)c;
+      for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 4; ++j)
+          if ((cw[k] >> (8 * j)) & 0xffu) bits |= 1u << (k * 4 + j);
+      if (bits) {
+        ((uint4*)(cand + v0))[0] = make_uint4(0, 0, 0, 0);
+        ((uint4*)(cand + v0))[1] = make_uint4(0, 0, 0, 0);
+      }
+    } else {
+      for (uint64_t v = v0; v < n && v < v0 + 32; ++v)
+        if (cand[v]) {
+          bits |= 1u << (v - v0);
+          cand[v] = 0;
+        }
+    }
+    claim[w] = bits;
+    while (bits) {
+      const uint32_t k = __ffs(bits) - 1;
+      bits &= bits - 1;
+      depth[v0 + k] = d1;
     }
   }
 }
@@ -476,26 +544,43 @@ struct BfsSearch {
     // unvisited nodes' reverse entries are fewer than alpha * m_f.
     const uint64_t m_u = rev->nnz - visited_rev;
     const bool pull = pulling && frontier_size * 24 > n && m_u < mf;
+    // A push whose frontier rows hold few entries sets vias as it marks;
+    // a larger one marks, then resolves the candidates by the pull kernel
+    // (each scanning its reverse row up to the first frontier node).
+    const bool small_push = !pull && mf * kSmallPushRatio < n;
     if (!pull) {
       if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
-      bfs_mark<<<(unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64), 256, 0, st>>>(
-          w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter, w.cand);
+      const unsigned mgrid = (unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64);
+      if (small_push)
+        bfs_mark_min<<<mgrid, 256, 0, st>>>(w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter,
+                                            w.cand, via);
+      else
+        bfs_mark<<<mgrid, 256, 0, st>>>(w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter,
+                                        w.cand);
     }
     const int cands = pull ? 0 : 1;
-    if (rev->nblocks)
-      bfs_pull_blocks<<<rev->nblocks, kBlockThreads, 0, st>>>(
-          rev->blocks, rev->offsets, rev->targets, w.visited, frontier, filter, w.cand, cands, depth, via,
-          claim, w.piece_hit, w.part_edges, d + 1);
-    if (rev->nlong)
-      bfs_pull_long<<<(rev->nlong + 255) / 256, 256, 0, st>>>(
-          rev->long_rows, rev->long_pbase, rev->nlong, rev->offsets, rev->targets, w.visited, filter,
-          w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev->nblocks, d + 1);
+    if (small_push) {
+      bfs_claim<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 255) / 256, 8192)), 256, 0, st>>>(
+          n, words, w.cand, depth, claim, d + 1);
+    } else {
+      if (rev->nblocks)
+        bfs_pull_blocks<<<rev->nblocks, kBlockThreads, 0, st>>>(
+            rev->blocks, rev->offsets, rev->targets, w.visited, frontier, filter, w.cand, cands, depth, via,
+            claim, w.piece_hit, w.part_edges, d + 1);
+      if (rev->nlong)
+        bfs_pull_long<<<(rev->nlong + 255) / 256, 256, 0, st>>>(
+            rev->long_rows, rev->long_pbase, rev->nlong, rev->offsets, rev->targets, w.visited, filter,
+            w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev->nblocks, d + 1);
+    }
     bfs_count<<<nbc, 256, 0, st>>>(words, claim, w.visited, frontier, adj->offsets, rev->offsets, w.part);
     bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(*rev) : 0, w.cnt);
     WG_HIP(ctx, hipGetLastError());
-    BfsCounters hc{};
-    WG_HIP(ctx, hipMemcpyAsync(&hc, w.cnt, sizeof hc, hipMemcpyDeviceToHost, st));
+    // The counters through pinned memory: a direct copy, no staging.
+    static_assert(sizeof(BfsCounters) <= kHostSmallBytes, "counters fit the pinned room");
+    WG_HIP(ctx, hipMemcpyAsync(ctx->h_small, w.cnt, sizeof(BfsCounters), hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipStreamSynchronize(st));
+    BfsCounters hc;
+    std::memcpy(&hc, ctx->h_small, sizeof hc);
     std::swap(frontier, claim);
     queue_ready = false;
     // A pushed level scans every entry of every frontier row.

@@ -490,7 +490,8 @@ int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
   ctx->limit = memory_limit;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&ctx->d_err, 8) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_small, kHostSmallBytes, hipHostMallocDefault) != hipSuccess) {
     delete ctx;
     return WG_DEVICE_ERROR;
   }
@@ -505,6 +506,7 @@ void wg_context_destroy(wg_context* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
   if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+  if (ctx->h_small) (void)hipHostFree(ctx->h_small);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -30,6 +30,7 @@ struct Block {
 };
 
 constexpr uint32_t kBlockThreads = 256;
+constexpr size_t kHostSmallBytes = 256;  // wg_context::h_small
 constexpr uint32_t kBlockNnz = 2048;   // entries staged in LDS per block
 constexpr uint32_t kBlockRows = 512;   // rows a short block may hold
 
@@ -92,6 +93,7 @@ struct wg_context {
   std::string last_msg;
   unsigned long long* d_err = nullptr;  // (index << 8 | status), ~0 when clear
   unsigned long long* h_err = nullptr;  // pinned mirror
+  void* h_small = nullptr;              // pinned room for a kernel's few counters
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
