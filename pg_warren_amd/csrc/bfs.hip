@@ -9,7 +9,9 @@
 //   push  bfs_mark: a wavefront per frontier node strides its row and marks
 //         each unvisited target a candidate (a plain byte store: every
 //         writer writes 1, so no atomic is needed); the pull kernel then
-//         resolves only the candidates.
+//         resolves only the candidates. Rows longer than a block are pushed
+//         a piece per workgroup (bfs_mark_pieces). A small push sets the
+//         vias as it marks (bfs_mark_min) and bfs_claim settles them.
 //   pull  bfs_pull_blocks over the scanned (reverse) CSR's row blocks, the
 //         SpMV partition: a group of T lanes per row reads T consecutive
 //         entries at a time (coalesced), and the lowest lane whose entry is
@@ -88,9 +90,37 @@ __global__ __launch_bounds__(256) void bfs_mark(const uint32_t* __restrict__ que
   for (uint64_t q = wave; q < qn; q += nwaves) {
     const uint32_t u = queue[q];
     const uint64_t a = off[u], b = off[u + 1];
+    if (b - a > kBlockNnz) continue;  // a split row: bfs_mark_pieces
     for (uint64_t i = a + lane; i < b; i += 64) {
       const uint32_t v = tgt[i];
       if (!bit_of(visited, v) && !(filter && !filter[v])) cand[v] = 1;
+    }
+  }
+}
+
+// Push over the split rows (rows of more than kBlockNnz entries, the CSR's
+// pieces): a workgroup per piece, which returns at once unless its row is in
+// the frontier. Splitting keeps a hub's row (R-MAT scale 22 has one of 161K
+// entries) from serialising a level on one wavefront. kMin: the targets'
+// vias by atomicMin, as bfs_mark_min.
+template <bool kMin>
+__global__ __launch_bounds__(256) void bfs_mark_pieces(const uint2* __restrict__ pieces,
+                                                       const uint64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ tgt,
+                                                       const uint32_t* __restrict__ frontier,
+                                                       const uint32_t* __restrict__ visited,
+                                                       const uint8_t* __restrict__ filter,
+                                                       uint8_t* __restrict__ cand, uint32_t* __restrict__ via) {
+  const uint2 pc = pieces[blockIdx.x];
+  const uint32_t u = pc.x;
+  if (!bit_of(frontier, u)) return;
+  const uint64_t a = off[u] + (uint64_t)pc.y * kBlockNnz;
+  const uint64_t b = min(a + kBlockNnz, off[u + 1]);
+  for (uint64_t i = a + threadIdx.x; i < b; i += 256) {
+    const uint32_t v = tgt[i];
+    if (!bit_of(visited, v) && !(filter && !filter[v])) {
+      if (kMin) atomicMin(&via[v], u);
+      cand[v] = 1;
     }
   }
 }
@@ -112,6 +142,7 @@ __global__ __launch_bounds__(256) void bfs_mark_min(const uint32_t* __restrict__
   for (uint64_t q = wave; q < qn; q += nwaves) {
     const uint32_t u = queue[q];
     const uint64_t a = off[u], b = off[u + 1];
+    if (b - a > kBlockNnz) continue;  // a split row: bfs_mark_pieces
     for (uint64_t i = a + lane; i < b; i += 64) {
       const uint32_t v = tgt[i];
       if (!bit_of(visited, v) && !(filter && !filter[v])) {
@@ -557,6 +588,14 @@ struct BfsSearch {
       else
         bfs_mark<<<mgrid, 256, 0, st>>>(w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter,
                                         w.cand);
+      if (adj->npartials) {
+        if (small_push)
+          bfs_mark_pieces<true><<<adj->npartials, 256, 0, st>>>(adj->pieces, adj->offsets, adj->targets, frontier,
+                                                                w.visited, filter, w.cand, via);
+        else
+          bfs_mark_pieces<false><<<adj->npartials, 256, 0, st>>>(adj->pieces, adj->offsets, adj->targets, frontier,
+                                                                 w.visited, filter, w.cand, via);
+      }
     }
     const int cands = pull ? 0 : 1;
     if (small_push) {

@@ -214,12 +214,20 @@ static int partition_csr(wg_context* ctx, DevCsr& c) {
                                hipMemcpyHostToDevice, ctx->stream));
   // The split rows and their partial slots, in block order.
   std::vector<uint32_t> lrows, lbase;
+  std::vector<uint2> pieces;
   for (const Block& b : blocks)
-    if (b.nrows == 0 && b.piece == 0) {
-      lrows.push_back(b.row0);
-      lbase.push_back(b.pbase);
+    if (b.nrows == 0) {
+      if (b.piece == 0) {
+        lrows.push_back(b.row0);
+        lbase.push_back(b.pbase);
+      }
+      pieces.push_back(make_uint2(b.row0, b.piece));
     }
   lbase.push_back(c.npartials);
+  if ((s = dalloc(ctx, (void**)&c.pieces, std::max<size_t>(pieces.size(), 1) * sizeof(uint2)))) return s;
+  if (!pieces.empty())
+    WG_HIP(ctx, hipMemcpyAsync(c.pieces, pieces.data(), pieces.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                               ctx->stream));
   if ((s = dalloc(ctx, (void**)&c.long_rows, std::max<size_t>(lrows.size(), 1) * 4))) return s;
   if ((s = dalloc(ctx, (void**)&c.long_pbase, lbase.size() * 4))) return s;
   if (!lrows.empty())
@@ -274,6 +282,7 @@ void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.blocks, c.nblocks * sizeof(Block));
   dfree(ctx, c.long_rows, std::max<uint32_t>(c.nlong, 1) * 4);
   dfree(ctx, c.long_pbase, (c.nlong + 1) * 4);
+  dfree(ctx, c.pieces, std::max<uint32_t>(c.npartials, 1) * sizeof(uint2));
   for (int d = 0; d < 5; ++d)
     if (c.wview[d]) dfree(ctx, c.wview[d], c.nnz * dtype_size(d));
   if (c.hot_state == 1) {

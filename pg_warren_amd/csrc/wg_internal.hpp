@@ -49,6 +49,7 @@ struct DevCsr {
   uint32_t npartials = 0;    // pieces of split rows
   uint32_t* long_rows = nullptr;   // nlong: the row of each split row
   uint32_t* long_pbase = nullptr;  // nlong + 1: its partial slots
+  uint2* pieces = nullptr;         // npartials: (row, piece) of each split-row piece, in block order
   uint64_t max_degree = 0;
   void* wview[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int wstatus[5] = {-1, -1, -1, -1, -1};  // -1: not cast yet

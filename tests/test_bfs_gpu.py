@@ -121,3 +121,27 @@ def test_rmat_scale18_depths_and_vias(ctx, oracle):
             preds = rev_tgt[rev_off[node]:rev_off[node + 1]]
             preds = preds[d[preds] == d[node] - 1]
             assert via[node] == preds.min()
+
+
+def test_hub_rows_pushed_in_pieces(ctx, oracle):
+    """Rows longer than a 2048-entry block are pushed a piece per workgroup:
+    from the largest hub, a node beside it and a far node, the visits (vias
+    and order included) equal the oracle's, pushed only or pulled too."""
+    off, tgt = W.rmat(17, 16, seed=9)
+    ref = oracle.Csr(len(off) - 1, off, tgt)
+    g = up(ctx, ref)
+    deg = np.diff(off.astype(np.int64))
+    assert deg.max() > 4 * 2048
+    hub = int(np.argmax(deg))
+    beside = int(tgt[off[hub]])
+    far = int(np.flatnonzero(deg == 1)[0])
+    for s in (hub, beside, far, [hub, far]):
+        print(f"hub rows: starts {s}", flush=True)
+        want, wst = oracle.bfs(ref, np.atleast_1d(s))
+        got, gst = W.bfs(ctx, g, np.atleast_1d(s))
+        assert got == want
+        assert gst["nodes_visited"] == wst["nodes_visited"]
+        d1, v1, _ = W.bfs_levels(ctx, g, np.atleast_1d(s), pulling=False)
+        d1, v1 = d1.numpy(), v1.numpy()
+        d2, v2, _ = W.bfs_levels(ctx, g, np.atleast_1d(s))
+        assert np.array_equal(d1, d2.numpy()) and np.array_equal(v1, v2.numpy())
