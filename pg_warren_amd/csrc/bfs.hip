@@ -6,12 +6,12 @@
 // the frontier: its size, its row entries (m_f) and the reverse-row entries
 // of every visited node (so the unvisited nodes' entries m_u follow by
 // subtraction).
-//   push  bfs_mark: a wavefront per frontier node strides its row and marks
+//   push  bfs_push: a wavefront per frontier node strides its row and marks
 //         each unvisited target a candidate (a plain byte store: every
 //         writer writes 1, so no atomic is needed); the pull kernel then
 //         resolves only the candidates. Rows longer than a block are pushed
-//         a piece per workgroup (bfs_mark_pieces). A small push sets the
-//         vias as it marks (bfs_mark_min) and bfs_claim settles them.
+//         a piece per workgroup. A small push sets the vias as it marks
+//         (atomicMin) and the compaction's first pass claims the candidates.
 //   pull  bfs_pull_blocks over the scanned (reverse) CSR's row blocks, the
 //         SpMV partition: a group of T lanes per row reads T consecutive
 //         entries at a time (coalesced), and the lowest lane whose entry is
@@ -22,9 +22,11 @@
 //         piece that hit. A block's next-frontier bits are gathered in LDS
 //         and stored a word at a time.
 // The next frontier's queue and counters come from the claim bitmap by a
-// count / scan / write compaction with per-block partials. The hot path has
-// no device-scope atomics (on gfx950 they resolve past the per-XCD L2 and
-// serialise on a shared address).
+// count / scan / write compaction with per-block partials; the scan posts
+// the counters to the context's mailbox (coherent host memory), which the
+// host polls, so a level costs no copy or stream synchronisation. The hot
+// path has no device-scope atomics (on gfx950 they resolve past the per-XCD
+// L2 and serialise on a shared address).
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -38,7 +40,7 @@ namespace wg {
 constexpr uint32_t kUnvisited = 0xffffffffu;
 constexpr uint32_t kWordsPerCountBlock = 512;  // bfs_count / bfs_write: 2 words a thread
 // A push level whose frontier rows hold fewer than n / kSmallPushRatio
-// entries sets vias by atomicMin while marking (bfs_mark_min).
+// entries sets vias by atomicMin while marking (bfs_push<true>).
 constexpr uint64_t kSmallPushRatio = 4;
 
 struct BfsCounters {
@@ -77,117 +79,84 @@ __global__ __launch_bounds__(256) void bfs_seed(const uint32_t* __restrict__ sta
   }
 }
 
-// Push, first half: candidates of the next level.
-__global__ __launch_bounds__(256) void bfs_mark(const uint32_t* __restrict__ queue, uint64_t qn,
+// A push's target v of frontier node u: an unvisited target admitted by
+// the filter becomes a candidate (a plain byte store: every writer writes 1).
+// kMin: it also takes the smallest frontier node with an edge to it as its
+// via (atomicMin: the first in ascending order, as the reference's push
+// reaches it).
+template <bool kMin>
+__device__ __forceinline__ void push_target(uint32_t v, uint32_t u, const uint32_t* __restrict__ visited,
+                                            const uint8_t* __restrict__ filter, uint8_t* __restrict__ cand,
+                                            uint32_t* __restrict__ via) {
+  if (!bit_of(visited, v) && !(filter && !filter[v])) {
+    if (kMin) atomicMin(&via[v], u);
+    cand[v] = 1;
+  }
+}
+
+// Push: blocks [0, qblocks) give a wavefront to each queued frontier node
+// in turn, which strides its row; rows of more than kBlockNnz entries are
+// left to the blocks after them, one per split-row piece of the CSR, which
+// return at once unless the piece's row is in the frontier bitmap. Splitting
+// keeps a hub's row (R-MAT scale 22 has one of 161K entries) from
+// serialising a level on one wavefront. kMin (a small frontier, whose
+// targets have few predecessors, so the atomics seldom meet): the vias are
+// set here and bfs_claim_count settles the candidates; otherwise the pull
+// kernel resolves them.
+template <bool kMin>
+__global__ __launch_bounds__(256) void bfs_push(const uint32_t* __restrict__ queue, uint64_t qn, uint32_t qblocks,
+                                                const uint2* __restrict__ pieces,
                                                 const uint64_t* __restrict__ off,
                                                 const uint32_t* __restrict__ tgt,
+                                                const uint32_t* __restrict__ frontier,
                                                 const uint32_t* __restrict__ visited,
                                                 const uint8_t* __restrict__ filter,
-                                                uint8_t* __restrict__ cand) {
+                                                uint8_t* __restrict__ cand, uint32_t* __restrict__ via) {
+  if (blockIdx.x >= qblocks) {
+    const uint2 pc = pieces[blockIdx.x - qblocks];
+    const uint32_t u = pc.x;
+    if (!bit_of(frontier, u)) return;
+    const uint64_t a = off[u] + (uint64_t)pc.y * kBlockNnz;
+    const uint64_t b = min(a + kBlockNnz, off[u + 1]);
+    for (uint64_t i = a + threadIdx.x; i < b; i += 256) push_target<kMin>(tgt[i], u, visited, filter, cand, via);
+    return;
+  }
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+  const uint64_t nwaves = (uint64_t)qblocks * 4;
   for (uint64_t q = wave; q < qn; q += nwaves) {
     const uint32_t u = queue[q];
     const uint64_t a = off[u], b = off[u + 1];
-    if (b - a > kBlockNnz) continue;  // a split row: bfs_mark_pieces
-    for (uint64_t i = a + lane; i < b; i += 64) {
-      const uint32_t v = tgt[i];
-      if (!bit_of(visited, v) && !(filter && !filter[v])) cand[v] = 1;
-    }
+    if (b - a > kBlockNnz) continue;  // a split row: its pieces' blocks
+    for (uint64_t i = a + lane; i < b; i += 64) push_target<kMin>(tgt[i], u, visited, filter, cand, via);
   }
 }
 
-// Push over the split rows (rows of more than kBlockNnz entries, the CSR's
-// pieces): a workgroup per piece, which returns at once unless its row is in
-// the frontier. Splitting keeps a hub's row (R-MAT scale 22 has one of 161K
-// entries) from serialising a level on one wavefront. kMin: the targets'
-// vias by atomicMin, as bfs_mark_min.
-template <bool kMin>
-__global__ __launch_bounds__(256) void bfs_mark_pieces(const uint2* __restrict__ pieces,
-                                                       const uint64_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ tgt,
-                                                       const uint32_t* __restrict__ frontier,
-                                                       const uint32_t* __restrict__ visited,
-                                                       const uint8_t* __restrict__ filter,
-                                                       uint8_t* __restrict__ cand, uint32_t* __restrict__ via) {
-  const uint2 pc = pieces[blockIdx.x];
-  const uint32_t u = pc.x;
-  if (!bit_of(frontier, u)) return;
-  const uint64_t a = off[u] + (uint64_t)pc.y * kBlockNnz;
-  const uint64_t b = min(a + kBlockNnz, off[u + 1]);
-  for (uint64_t i = a + threadIdx.x; i < b; i += 256) {
-    const uint32_t v = tgt[i];
-    if (!bit_of(visited, v) && !(filter && !filter[v])) {
-      if (kMin) atomicMin(&via[v], u);
-      cand[v] = 1;
+// The claim bits of 32-node word w from the candidate bytes, which are
+// cleared.
+__device__ __forceinline__ uint32_t claim_word(uint64_t n, uint64_t w, uint8_t* __restrict__ cand) {
+  const uint64_t v0 = w * 32;
+  uint32_t bits = 0;
+  if (v0 + 32 <= n) {
+    uint4 c[2];
+    c[0] = ((const uint4*)(cand + v0))[0];
+    c[1] = ((const uint4*)(cand + v0))[1];
+    const uint32_t* cw = (const uint32_t*)c;
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 4; ++j)
+        if ((cw[k] >> (8 * j)) & 0xffu) bits |= 1u << (k * 4 + j);
+    if (bits) {
+      ((uint4*)(cand + v0))[0] = make_uint4(0, 0, 0, 0);
+      ((uint4*)(cand + v0))[1] = make_uint4(0, 0, 0, 0);
     }
-  }
-}
-
-// Push of a small frontier, whole: each unvisited target takes the smallest
-// frontier node with an edge to it as its via (atomicMin: the first in
-// ascending order, as the reference's push reaches it) and is marked a
-// candidate. A small frontier has few predecessors per target, so the
-// atomics seldom meet; bfs_claim then settles the candidates.
-__global__ __launch_bounds__(256) void bfs_mark_min(const uint32_t* __restrict__ queue, uint64_t qn,
-                                                    const uint64_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ tgt,
-                                                    const uint32_t* __restrict__ visited,
-                                                    const uint8_t* __restrict__ filter,
-                                                    uint8_t* __restrict__ cand, uint32_t* __restrict__ via) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const uint64_t nwaves = (uint64_t)gridDim.x * 4;
-  for (uint64_t q = wave; q < qn; q += nwaves) {
-    const uint32_t u = queue[q];
-    const uint64_t a = off[u], b = off[u + 1];
-    if (b - a > kBlockNnz) continue;  // a split row: bfs_mark_pieces
-    for (uint64_t i = a + lane; i < b; i += 64) {
-      const uint32_t v = tgt[i];
-      if (!bit_of(visited, v) && !(filter && !filter[v])) {
-        atomicMin(&via[v], u);
-        cand[v] = 1;
+  } else {
+    for (uint64_t v = v0; v < n && v < v0 + 32; ++v)
+      if (cand[v]) {
+        bits |= 1u << (v - v0);
+        cand[v] = 0;
       }
-    }
   }
-}
-
-// The candidates of a level whose vias are set: each takes depth d1 and its
-// bit in the claim map (a thread per 32-node word, so whole words are
-// stored), and its candidate byte is cleared.
-__global__ __launch_bounds__(256) void bfs_claim(uint64_t n, uint64_t words, uint8_t* __restrict__ cand,
-                                                 uint32_t* __restrict__ depth, uint32_t* __restrict__ claim,
-                                                 uint32_t d1) {
-  for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < words; w += (uint64_t)gridDim.x * 256) {
-    const uint64_t v0 = w * 32;
-    uint32_t bits = 0;
-    if (v0 + 32 <= n) {
-      uint4 c[2];
-      c[0] = ((const uint4*)(cand + v0))[0];
-      c[1] = ((const uint4*)(cand + v0))[1];
-      const uint32_t* cw = (const uint32_t*)c;
-      for (int k = 0; k < 8; ++k)
-        for (int j = 0; j < 4; ++j)
-          if ((cw[k] >> (8 * j)) & 0xffu) bits |= 1u << (k * 4 + j);
-      if (bits) {
-        ((uint4*)(cand + v0))[0] = make_uint4(0, 0, 0, 0);
-        ((uint4*)(cand + v0))[1] = make_uint4(0, 0, 0, 0);
-      }
-    } else {
-      for (uint64_t v = v0; v < n && v < v0 + 32; ++v)
-        if (cand[v]) {
-          bits |= 1u << (v - v0);
-          cand[v] = 0;
-        }
-    }
-    claim[w] = bits;
-    while (bits) {
-      const uint32_t k = __ffs(bits) - 1;
-      bits &= bits - 1;
-      depth[v0 + k] = d1;
-    }
-  }
+  return bits;
 }
 
 // Pull over the scanned CSR's row blocks (roff/rtgt). candidates == 0: every
@@ -338,23 +307,35 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
 // Compaction, first pass: per block of words, the claimed nodes and their
 // forward / reverse row entries; also folds the claims into visited and
 // clears the old frontier bitmap (it becomes the next level's claim map).
-__global__ __launch_bounds__(256) void bfs_count(uint64_t words, const uint32_t* __restrict__ claim,
+// kClaim (after a small push, whose vias are set): the claims come from the
+// candidate bytes here, and the claimed nodes take depth d1.
+template <bool kClaim>
+__global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uint32_t* __restrict__ claim,
                                                  uint32_t* __restrict__ visited,
                                                  uint32_t* __restrict__ old_frontier,
                                                  const uint64_t* __restrict__ off,
                                                  const uint64_t* __restrict__ roff,
-                                                 unsigned long long* __restrict__ part) {
+                                                 unsigned long long* __restrict__ part,
+                                                 uint8_t* __restrict__ cand, uint32_t* __restrict__ depth,
+                                                 uint32_t d1) {
   __shared__ unsigned long long red[4];
   unsigned long long cnt = 0, ent = 0, vr = 0;
   const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerCountBlock + threadIdx.x * 2;
   for (uint64_t w = w0; w < w0 + 2 && w < words; ++w) {
-    uint32_t bits = claim[w];
+    uint32_t bits;
+    if (kClaim) {
+      bits = claim_word(n, w, cand);
+      claim[w] = bits;
+    } else {
+      bits = claim[w];
+    }
     visited[w] |= bits;
     old_frontier[w] = 0;
     cnt += __popc(bits);
     while (bits) {
       const uint32_t v = (uint32_t)(w * 32) + (__ffs(bits) - 1);
       bits &= bits - 1;
+      if (kClaim) depth[v] = d1;
       ent += off[v + 1] - off[v];
       vr += roff[v + 1] - roff[v];
     }
@@ -370,11 +351,13 @@ __global__ __launch_bounds__(256) void bfs_count(uint64_t words, const uint32_t*
 }
 
 // Compaction, second pass (one workgroup): the blocks' queue offsets by an
-// exclusive scan of their counts, and the level's totals.
+// exclusive scan of their counts, and the level's totals, posted to the
+// context's mailbox.
 __global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long* __restrict__ part,
                                                  unsigned long long* __restrict__ offsets,
                                                  const unsigned long long* __restrict__ part_edges,
-                                                 uint32_t nedges, BfsCounters* c) {
+                                                 uint32_t nedges, unsigned long long* __restrict__ mail,
+                                                 unsigned long long seq) {
   __shared__ unsigned long long sh[1024];
   __shared__ unsigned long long red[16];
   const uint32_t tid = threadIdx.x;
@@ -403,10 +386,11 @@ __global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long
   vr = block_reduce_sum(vr, red);
   ed = block_reduce_sum(ed, red);
   if (tid == 0) {
-    c->next_size = sh[1023];
-    c->next_entries = ent;
-    c->visited_rev = vr;
-    c->edges = ed;
+    // The level's counters (BfsCounters' order) posted to the host's
+    // mailbox: values first, then the sequence number with release.
+    const unsigned long long v[4] = {sh[1023], ent, vr, ed};
+    for (int i = 0; i < 4; ++i) __hip_atomic_store(&mail[1 + i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&mail[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -447,31 +431,30 @@ struct BfsRun {
 };
 
 // The search's workspace: bitmaps, queues, candidate bytes, split-row piece
-// hits, compaction partials and counters.
+// hits and compaction partials, in kWsSlots buffers.
+constexpr int kWsSlots = 7;
 struct BfsWorkspace {
   uint32_t *visited, *bm_a, *bm_b, *q_a, *q_b, *piece_hit;
   uint8_t* cand;
   unsigned long long *part, *offsets, *part_edges;
-  BfsCounters* cnt;
 };
 
 static uint32_t edge_parts(const DevCsr& c) { return c.nblocks + (c.nlong + 255) / 256; }
 
-// Allocates a workspace's eight buffers into slot[], recording them in
-// `owned` (charged to the context).
-static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, void* slot[8],
+// Allocates a workspace's buffers into slot[], recording them in `owned`
+// (charged to the context).
+static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, void* slot[kWsSlots],
                            std::vector<std::pair<void*, size_t>>& owned) {
   const uint64_t nbc = (words + kWordsPerCountBlock - 1) / kWordsPerCountBlock;
   const uint64_t n = std::max<uint64_t>(g->n, 1);
-  const size_t sizes[8] = {3 * words * 4,
+  const size_t sizes[kWsSlots] = {3 * words * 4,
                            2 * n * 4,
                            std::max<uint32_t>(std::max(g->fwd.npartials, g->rev.npartials), 1) * 4,
                            n,
                            std::max<uint64_t>(nbc, 1) * 3 * 8,
                            std::max<uint64_t>(nbc, 1) * 8,
-                           std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8,
-                           sizeof(BfsCounters)};
-  for (int k = 0; k < 8; ++k) {
+                           std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8};
+  for (int k = 0; k < kWsSlots; ++k) {
     void* p = nullptr;
     int s = dalloc(ctx, &p, sizes[k]);
     if (s) return s;
@@ -483,12 +466,12 @@ static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, v
   return WG_OK;
 }
 
-static void view_workspace(const wg_graph* g, void* const slot[8], uint64_t words, BfsWorkspace& w) {
+static void view_workspace(const wg_graph* g, void* const slot[kWsSlots], uint64_t words, BfsWorkspace& w) {
   uint32_t* bms = (uint32_t*)slot[0];
   uint32_t* qs = (uint32_t*)slot[1];
   w = BfsWorkspace{bms, bms + words, bms + 2 * words, qs, qs + std::max<uint64_t>(g->n, 1),
                    (uint32_t*)slot[2], (uint8_t*)slot[3], (unsigned long long*)slot[4],
-                   (unsigned long long*)slot[5], (unsigned long long*)slot[6], (BfsCounters*)slot[7]};
+                   (unsigned long long*)slot[5], (unsigned long long*)slot[6]};
 }
 
 // The graph's own workspace, allocated on first use and kept with it.
@@ -581,27 +564,20 @@ struct BfsSearch {
     const bool small_push = !pull && mf * kSmallPushRatio < n;
     if (!pull) {
       if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
-      const unsigned mgrid = (unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64);
+      const unsigned qblocks = (unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64);
+      const unsigned grid = qblocks + adj->npartials;
       if (small_push)
-        bfs_mark_min<<<mgrid, 256, 0, st>>>(w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter,
-                                            w.cand, via);
+        bfs_push<true><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, adj->pieces, adj->offsets, adj->targets,
+                                             frontier, w.visited, filter, w.cand, via);
       else
-        bfs_mark<<<mgrid, 256, 0, st>>>(w.q_a, frontier_size, adj->offsets, adj->targets, w.visited, filter,
-                                        w.cand);
-      if (adj->npartials) {
-        if (small_push)
-          bfs_mark_pieces<true><<<adj->npartials, 256, 0, st>>>(adj->pieces, adj->offsets, adj->targets, frontier,
-                                                                w.visited, filter, w.cand, via);
-        else
-          bfs_mark_pieces<false><<<adj->npartials, 256, 0, st>>>(adj->pieces, adj->offsets, adj->targets, frontier,
-                                                                 w.visited, filter, w.cand, via);
-      }
+        bfs_push<false><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, adj->pieces, adj->offsets,
+                                              adj->targets, frontier, w.visited, filter, w.cand, via);
     }
-    const int cands = pull ? 0 : 1;
     if (small_push) {
-      bfs_claim<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 255) / 256, 8192)), 256, 0, st>>>(
-          n, words, w.cand, depth, claim, d + 1);
+      bfs_count<true><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, adj->offsets, rev->offsets, w.part,
+                                           w.cand, depth, d + 1);
     } else {
+      const int cands = pull ? 0 : 1;
       if (rev->nblocks)
         bfs_pull_blocks<<<rev->nblocks, kBlockThreads, 0, st>>>(
             rev->blocks, rev->offsets, rev->targets, w.visited, frontier, filter, w.cand, cands, depth, via,
@@ -610,18 +586,22 @@ struct BfsSearch {
         bfs_pull_long<<<(rev->nlong + 255) / 256, 256, 0, st>>>(
             rev->long_rows, rev->long_pbase, rev->nlong, rev->offsets, rev->targets, w.visited, filter,
             w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev->nblocks, d + 1);
+      bfs_count<false><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, adj->offsets, rev->offsets,
+                                            w.part, nullptr, nullptr, 0);
     }
-    bfs_count<<<nbc, 256, 0, st>>>(words, claim, w.visited, frontier, adj->offsets, rev->offsets, w.part);
-    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(*rev) : 0, w.cnt);
+    const unsigned long long seq = ++ctx->mail_seq;
+    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(*rev) : 0, ctx->d_mail,
+                                 seq);
+    // After a push the next level is most often pushed too: its queue is
+    // listed now, while the host waits for the counters.
+    if (!pull) bfs_write<<<nbc, 256, 0, st>>>(words, claim, w.offsets, w.q_a);
     WG_HIP(ctx, hipGetLastError());
-    // The counters through pinned memory: a direct copy, no staging.
-    static_assert(sizeof(BfsCounters) <= kHostSmallBytes, "counters fit the pinned room");
-    WG_HIP(ctx, hipMemcpyAsync(ctx->h_small, w.cnt, sizeof(BfsCounters), hipMemcpyDeviceToHost, st));
-    WG_HIP(ctx, hipStreamSynchronize(st));
     BfsCounters hc;
-    std::memcpy(&hc, ctx->h_small, sizeof hc);
+    static_assert(sizeof(BfsCounters) == 4 * sizeof(unsigned long long), "four counters");
+    int s = wait_mail(ctx, seq, (unsigned long long*)&hc, 4);
+    if (s) return s;
     std::swap(frontier, claim);
-    queue_ready = false;
+    queue_ready = !pull;
     // A pushed level scans every entry of every frontier row.
     run.edges += pull ? hc.edges : mf;
     frontier_size = hc.next_size;
@@ -720,7 +700,7 @@ struct LevelEmitter {
   wg_context* ctx = nullptr;
   wg_graph* g = nullptr;
   std::vector<std::pair<void*, size_t>> owned;  // workspace and buffers
-  void* slot[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* slot[kWsSlots] = {};
   BfsSearch S;
   std::vector<uint32_t> starts;
   uint32_t *depth = nullptr, *via = nullptr, *keys_in = nullptr, *keys_out = nullptr;

@@ -117,6 +117,21 @@ int collect_error(wg_context* ctx, const char* what) {
   return ok(ctx);
 }
 
+int wait_mail(wg_context* ctx, unsigned long long seq, unsigned long long* out, int k) {
+  unsigned long long* m = ctx->h_mail;
+  for (uint32_t spin = 1; __atomic_load_n(&m[0], __ATOMIC_ACQUIRE) != seq; ++spin) {
+    if (spin % 64) continue;
+    const hipError_t e = hipStreamQuery(ctx->stream);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return hip_fail(ctx, e, "waiting for a kernel's results");
+    // The stream drained: a post it held is visible by now.
+    if (__atomic_load_n(&m[0], __ATOMIC_ACQUIRE) != seq)
+      return fail(ctx, WG_DEVICE_ERROR, "a kernel ended without posting its results");
+  }
+  for (int i = 0; i < k; ++i) out[i] = __atomic_load_n(&m[1 + i], __ATOMIC_RELAXED);
+  return WG_OK;
+}
+
 bool is_cancelled(wg_context* ctx) { return ctx->cancelled.load() != 0; }
 
 template <class E>
@@ -500,7 +515,9 @@ int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&ctx->d_err, 8) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&ctx->h_small, kHostSmallBytes, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->h_mail, kMailWords * 8, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void**)&ctx->d_mail, ctx->h_mail, 0) != hipSuccess) {
     delete ctx;
     return WG_DEVICE_ERROR;
   }
@@ -515,7 +532,7 @@ void wg_context_destroy(wg_context* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
   if (ctx->h_err) (void)hipHostFree(ctx->h_err);
-  if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  if (ctx->h_mail) (void)hipHostFree(ctx->h_mail);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

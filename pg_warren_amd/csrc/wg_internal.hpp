@@ -30,7 +30,7 @@ struct Block {
 };
 
 constexpr uint32_t kBlockThreads = 256;
-constexpr size_t kHostSmallBytes = 256;  // wg_context::h_small
+constexpr size_t kMailWords = 32;  // wg_context mailbox: seq + 31 values
 constexpr uint32_t kBlockNnz = 2048;   // entries staged in LDS per block
 constexpr uint32_t kBlockRows = 512;   // rows a short block may hold
 
@@ -94,7 +94,12 @@ struct wg_context {
   std::string last_msg;
   unsigned long long* d_err = nullptr;  // (index << 8 | status), ~0 when clear
   unsigned long long* h_err = nullptr;  // pinned mirror
-  void* h_small = nullptr;              // pinned room for a kernel's few counters
+  // Coherent pinned words a kernel posts a few results into: mail[0] the
+  // sequence number, then the values; the host polls (wait_mail) instead of
+  // copying and synchronising.
+  unsigned long long* h_mail = nullptr;  // host view
+  unsigned long long* d_mail = nullptr;  // device view
+  unsigned long long mail_seq = 0;
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
@@ -117,7 +122,7 @@ struct wg_graph {
   // Search workspace sized by the graph, allocated on first use and kept
   // (charged to the context) until the graph is destroyed.
   std::vector<std::pair<void*, size_t>> workspace;
-  void* bfs_ws[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* bfs_ws[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // Other kernels' per-graph buffers by slot: (pointer, bytes).
   std::map<int, std::pair<void*, size_t>> cached;
   // Hot-first numbering of message sources (built on first use): nodes with
@@ -197,6 +202,10 @@ size_t dtype_size(int dtype);
 int fail(wg_context* ctx, int status, const std::string& msg);
 int ok(wg_context* ctx);
 int hip_fail(wg_context* ctx, hipError_t e, const char* what);
+// Waits until a kernel on the context's stream posts mailbox sequence
+// number `seq`, then copies its first k values into out. The wait ends
+// with an error if the stream drains or fails without the post.
+int wait_mail(wg_context* ctx, unsigned long long seq, unsigned long long* out, int k);
 // Device memory charged to the context's budget before it is allocated
 // (ComputeMemory::charge, compute/memory.rs:97).
 int dalloc(wg_context* ctx, void** p, size_t bytes);
