@@ -620,7 +620,7 @@ def main():
     for s in sources[:8]:
         st_arr = np.array([s], np.uint64)
         t1 = time.perf_counter()
-        ctx.check(lib.wg_bfs(ctx.h, gf.h, 0, st_arr.ctypes.data, 1, None, -1, -1, C.addressof(out),
+        ctx.check(lib.wg_bfs(ctx.h, gf.h, 0, None, 0, st_arr.ctypes.data, 1, None, -1, -1, C.addressof(out),
                              C.byref(cnt), C.byref(stt)))
         row_times.append(time.perf_counter() - t1)
         rows.append(cnt.value)
@@ -629,7 +629,7 @@ def main():
         st_arr = np.array([s], np.uint64)
         h, more, k = C.c_void_p(), C.c_int(1), C.c_uint64()
         t1 = time.perf_counter()
-        ctx.check(lib.wg_bfs_cursor_create(ctx.h, gf.h, 0, st_arr.ctypes.data, 1, None, -1, -1, 0,
+        ctx.check(lib.wg_bfs_cursor_create(ctx.h, gf.h, 0, None, 0, st_arr.ctypes.data, 1, None, -1, -1, 0,
                                            C.byref(h)))
         while more.value:
             ctx.check(lib.wg_bfs_cursor_next(ctx.h, h, C.addressof(out), C.byref(k), C.byref(more)))

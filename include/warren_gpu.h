@@ -70,7 +70,7 @@ enum wg_binary_op {
 };
 
 /* ---- traversal direction (graph::Direction, traversal/adjacency.rs:20-60) */
-enum wg_direction { WG_OUT = 0, WG_IN = 1, WG_BOTH = 2 /* gather only */ };
+enum wg_direction { WG_OUT = 0, WG_IN = 1, WG_BOTH = 2 /* traversals and gather */ };
 
 typedef struct wg_context wg_context;
 typedef struct wg_graph wg_graph;
@@ -305,41 +305,57 @@ typedef struct {
 
 typedef struct {
   uint64_t nodes_visited;
-  uint64_t edges_traversed; /* row entries scanned; pushed levels scan every
-                               frontier row, pulled levels stop at the first
-                               frontier hit, as the reference counts them */
+  uint64_t edges_traversed; /* row entries scanned, as the reference counts
+                               them: a pushed level scans every frontier row
+                               (up to the refused visit's entry when
+                               max_results cuts it), a pulled level stops at
+                               the first frontier hit */
   uint32_t truncated;
   uint32_t levels_pulled;
 } wg_traversal_stats;
 
+/* Traversals read the CSRs of Adjacency::new (traversal/adjacency.rs:68-152):
+ * relations NULL: every relation, through the graph's own (all-relations)
+ * CSR pair; else the subset relations[0..nrelations) (sorted, repeats
+ * dropped; an index not below the relation count is WG_UNKNOWN_RELATION),
+ * each relation's CSR pair in index order. direction WG_OUT reads the
+ * forward rows, WG_IN the reverse ones, WG_BOTH the forward then the
+ * reverse; a node's row entries are its rows in that order. At most 32 CSRs
+ * (16 relations both ways). */
+
 /* The device-resident search: depth and via (u32 vectors over the graph,
- * 0xffffffff where unreached) of every node reached from starts along
- * direction (WG_OUT: the forward CSR, WG_IN: the reverse), levels pushed or,
- * when pulling != 0 and Beamer's switch (alpha 1, beta 24, traversal/
- * level.rs:19-29,184-211) says so, pulled. via is the smallest frontier
- * predecessor, as the reference reports it in both directions. filter: n host
- * bytes (non-zero admits) or NULL; max_depth < 0: unbounded. Starts must lie
- * in the id space (NodeOutsideIdSpace) and repeats count once. */
-int wg_bfs_levels(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
-                  uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int pulling,
-                  wg_vector* depth, wg_vector* via, wg_traversal_stats* stats);
+ * 0xffffffff where unreached) of every node reached from starts, levels
+ * pushed or, when pulling != 0 and Beamer's switch (alpha 1, beta 24,
+ * traversal/level.rs:19-29,184-211) says so, pulled. via is the one a push
+ * gives (the smallest frontier node with an edge to the node) either way.
+ * filter: n host bytes (non-zero admits) or NULL; max_depth < 0: unbounded.
+ * Starts must lie in the id space (NodeOutsideIdSpace) and repeats count
+ * once. The stats count the device's own work (pulled levels included). */
+int wg_bfs_levels(wg_context* ctx, const wg_graph* g, int direction, const uint32_t* relations,
+                  uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
+                  int64_t max_depth, int pulling, wg_vector* depth, wg_vector* via, wg_traversal_stats* stats);
 
 /* bfs (traversal/bfs.rs:34-45): every node reachable from starts, once, in
- * BFS order -- by depth; the starts in the order given; within a depth by
- * (via, node), the order the reference's push reaches them. filter prunes
+ * BFS order -- by depth; the starts in the order given; within a depth in
+ * the order the reference's push reaches them: by via, then by the first
+ * CSR whose row of the via holds the node, then by node. filter prunes
  * admission (starts are always admitted); max_results < 0: unbounded, else
  * the output stops there and stats->truncated says whether a visit was cut.
- * out must hold n visits. */
-int wg_bfs(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
-           uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
+ * The reference never pulls a bfs level, so the stats are a push's (no
+ * level pulled) although the device pulls wide levels. out must hold n
+ * visits. */
+int wg_bfs(wg_context* ctx, const wg_graph* g, int direction, const uint32_t* relations, uint32_t nrelations,
+           const uint64_t* starts, uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
            wg_visit* out, uint64_t* count, wg_traversal_stats* stats);
 
 /* reachable_set (traversal/reachable.rs:48-90): the direction-optimising
  * search's nodes by depth, pushed levels in the order reached, pulled
- * levels ascending by node. out must hold n ids. */
-int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
-                     uint64_t nstarts, int64_t max_depth, uint64_t* out, uint64_t* count,
-                     wg_traversal_stats* stats);
+ * levels ascending by node (a pulled node's via is the first frontier node
+ * its reversed rows name, in their concatenated order). out must hold n
+ * ids. */
+int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const uint32_t* relations,
+                     uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, int64_t max_depth, uint64_t* out,
+                     uint64_t* count, wg_traversal_stats* stats);
 
 /* BfsCursor (traversal/bfs_cursor.rs:23-135): the search of wg_bfs (or,
  * with reachable, of wg_reachable_set: no filter or result limit, a pulled
@@ -349,12 +365,11 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* g, int direction, const ui
  * *more = 0 once no level follows (the search is exhausted, max_depth
  * levels are done, or max_results visits were handed out and another was
  * found: stats.truncated). The cursor keeps its own workspace, so several
- * may be open on one graph; a relation subset is the graph
- * wg_graph_select returns. Arguments are checked at creation. */
+ * may be open on one graph. Arguments are checked at creation. */
 typedef struct wg_bfs_cursor wg_bfs_cursor;
-int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
-                         uint64_t nstarts, const uint8_t* filter, int64_t max_depth,
-                         int64_t max_results, int reachable, wg_bfs_cursor** out);
+int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint32_t* relations,
+                         uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
+                         int64_t max_depth, int64_t max_results, int reachable, wg_bfs_cursor** out);
 int wg_bfs_cursor_next(wg_context* ctx, wg_bfs_cursor* cursor, wg_visit* out, uint64_t* count,
                        int* more);
 int wg_bfs_cursor_stats(const wg_bfs_cursor* cursor, wg_traversal_stats* stats);

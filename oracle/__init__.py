@@ -86,6 +86,8 @@ def lib():
         L.or_pagerank_backward.argtypes = [P, P, C.c_double, C.c_double, C.c_int, C.c_uint32, P,
                                            P, P, P, P, P, C.c_int]
         L.or_bfs.argtypes = [P, P, P, C.c_uint64, P, C.c_int64, C.c_int64, P, P, P]
+        L.or_bfs_rows.argtypes = [C.c_uint64, P, C.c_uint32, P, C.c_uint32, P, C.c_uint64, P, C.c_int64,
+                                  C.c_int64, P, P, P]
         L.or_bfs_depths.argtypes = [P, P, C.c_uint64, P, C.c_int]
         L.or_apply.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
         L.or_from_weight.argtypes = [C.c_int, C.c_double, P]
@@ -267,18 +269,43 @@ def pagerank_backward(fwd, x, g, *, damping=0.85, tol=1e-6, max_iter=100, seed=N
     return dw[:fwd.nnz], lam, int(it[0])
 
 
-def bfs(adj, starts, *, pull_rev=None, node_filter=None, max_depth=None, max_results=None):
-    """Returns (visits as list of (node, depth, via|None), stats dict)."""
-    n = adj.n
+def adjacency(merged, relation_csrs, direction="out", relations=None):
+    """The CSR lists a traversal reads (Adjacency::new / reversed,
+    traversal/adjacency.rs:68-152, 288-320): (rows, reversed rows). merged:
+    the all-relations (forward, reverse) pair; relation_csrs: each
+    relation's (forward, reverse) pair in relation order. Every relation:
+    the merged pair, Out [fwd], In [rev], Both [fwd, rev]; a subset (sorted,
+    deduplicated): each relation's pair in turn, read the same way. The
+    reversed adjacency reads the opposite direction (Both stays Both)."""
+    def read(pairs, d):
+        out = []
+        for f, r in pairs:
+            out += {"out": [f], "in": [r], "both": [f, r]}[d]
+        return out
+    pairs = [merged] if relations is None else [relation_csrs[i] for i in sorted(set(relations))]
+    back = {"out": "in", "in": "out", "both": "both"}[direction]
+    return read(pairs, direction), read(pairs, back)
+
+
+def bfs(adj, starts, *, pull_rev=None, node_filter=None, max_depth=None, max_results=None, n=None):
+    """Returns (visits as list of (node, depth, via|None), stats dict). adj
+    and pull_rev are a Csr or a list of them (an Adjacency's rows, read in
+    order); n the id space when the lists are empty."""
+    rows = list(adj) if isinstance(adj, (list, tuple)) else [adj]
+    rrows = None if pull_rev is None else (list(pull_rev) if isinstance(pull_rev, (list, tuple)) else [pull_rev])
+    n = rows[0].n if n is None else n
     st = np.ascontiguousarray(starts, np.uint64)
     vis = (_Visit * max(n, 1))()
     cnt = np.zeros(1, np.uint64)
     stats = _Stats()
     flt = None if node_filter is None else np.ascontiguousarray(node_filter, np.uint8)
-    _check(lib().or_bfs(adj.ref(), None if pull_rev is None else pull_rev.ref(), _p(st), len(st),
-                        _p(flt), -1 if max_depth is None else max_depth,
-                        -1 if max_results is None else max_results, C.addressof(vis), _p(cnt),
-                        C.byref(stats)))
+    rp = (C.POINTER(_Csr) * max(len(rows), 1))(*[C.pointer(c._c) for c in rows])
+    rr = None if rrows is None else (C.POINTER(_Csr) * max(len(rrows), 1))(*[C.pointer(c._c) for c in rrows])
+    _check(lib().or_bfs_rows(n, C.cast(rp, C.c_void_p), len(rows), None if rr is None else C.cast(rr, C.c_void_p),
+                             0 if rrows is None else len(rrows), _p(st), len(st),
+                             _p(flt), -1 if max_depth is None else max_depth,
+                             -1 if max_results is None else max_results, C.addressof(vis), _p(cnt),
+                             C.byref(stats)))
     out = []
     for i in range(int(cnt[0])):
         v = vis[i]

@@ -702,12 +702,16 @@ int or_pagerank(const or_csr* fwd, const or_csr* rev, const uint8_t* tomb, doubl
   return OR_OK;
 }
 
-// LevelSearch (level.rs:88-337) with the Results admission of bfs.rs:84-111.
-int or_bfs(const or_csr* adj, const or_csr* rev, const uint64_t* starts, uint64_t nstarts,
-           const uint8_t* filter, int64_t max_depth, int64_t max_results, or_visit* visits,
-           uint64_t* count_out, or_stats* stats) {
+// LevelSearch (level.rs:88-337) with the Results admission of bfs.rs:84-111,
+// over an Adjacency's rows (traversal/adjacency.rs:20-221): `rows` are the
+// CSRs read in order (a node's row entries are its rows concatenated:
+// Adjacency::row_entries), `rrows` those of the reversed adjacency, which
+// a pulled level scans (NULL: every level pushed). Degrees are summed over
+// the CSRs read (row_entry_bound).
+int or_bfs_rows(uint64_t n, const or_csr* const* rows, uint32_t k, const or_csr* const* rrows, uint32_t kr,
+                const uint64_t* starts, uint64_t nstarts, const uint8_t* filter, int64_t max_depth,
+                int64_t max_results, or_visit* visits, uint64_t* count_out, or_stats* stats) {
   const uint64_t ALPHA = 1, BETA = 24;
-  uint64_t n = adj->n;
   or_stats st{0, 0, 0, 0};
   uint64_t count = 0;
   for (uint64_t i = 0; i < nstarts; ++i)
@@ -736,26 +740,28 @@ int or_bfs(const or_csr* adj, const or_csr* rev, const uint64_t* starts, uint64_
   while (!stop && !frontier.empty() && !(max_depth >= 0 && (int64_t)depth >= max_depth)) {
     std::sort(frontier.begin(), frontier.end());
     bool pulls = false;
-    if (rev && frontier.size() * BETA > n) {
+    if (rrows && frontier.size() * BETA > n) {
       uint64_t fe = 0;
-      for (uint64_t f : frontier) fe += adj->offsets[f + 1] - adj->offsets[f];
+      for (uint64_t f : frontier)
+        for (uint32_t j = 0; j < k; ++j) fe += rows[j]->offsets[f + 1] - rows[j]->offsets[f];
       uint64_t limit = fe * ALPHA, ue = 0;
       pulls = true;
       for (uint64_t v = 0; v < n && pulls; ++v)
         if (!visited.get(v)) {
-          ue += rev->offsets[v + 1] - rev->offsets[v];
+          for (uint32_t j = 0; j < kr; ++j) ue += rrows[j]->offsets[v + 1] - rrows[j]->offsets[v];
           if (ue >= limit) pulls = false;
         }
     }
     uint32_t d = depth + 1;
     if (!pulls) {
       for (uint64_t via : frontier) {
-        for (uint64_t p = adj->offsets[via]; p < adj->offsets[via + 1] && !stop; ++p) {
-          uint64_t node = adj->targets[p];
-          st.edges_traversed++;
-          if (visited.get(node) || (filter && !filter[node])) continue;
-          if (!offer(node, d, via)) stop = true;
-        }
+        for (uint32_t j = 0; j < k && !stop; ++j)
+          for (uint64_t p = rows[j]->offsets[via]; p < rows[j]->offsets[via + 1] && !stop; ++p) {
+            uint64_t node = rows[j]->targets[p];
+            st.edges_traversed++;
+            if (visited.get(node) || (filter && !filter[node])) continue;
+            if (!offer(node, d, via)) stop = true;
+          }
         if (stop) break;
       }
     } else {
@@ -766,13 +772,16 @@ int or_bfs(const or_csr* adj, const or_csr* rev, const uint64_t* starts, uint64_
       for (uint64_t node = 0; node < n && !stop; ++node) {
         if (seen.get(node)) continue;
         if (filter && !filter[node]) continue;
-        for (uint64_t p = rev->offsets[node]; p < rev->offsets[node + 1]; ++p) {
-          uint64_t via = rev->targets[p];
-          st.edges_traversed++;
-          if (!fr.get(via)) continue;
-          if (!offer(node, d, via)) stop = true;
-          break;
-        }
+        bool hit = false;
+        for (uint32_t j = 0; j < kr && !hit; ++j)
+          for (uint64_t p = rrows[j]->offsets[node]; p < rrows[j]->offsets[node + 1]; ++p) {
+            uint64_t via = rrows[j]->targets[p];
+            st.edges_traversed++;
+            if (!fr.get(via)) continue;
+            if (!offer(node, d, via)) stop = true;
+            hit = true;
+            break;
+          }
       }
     }
     std::swap(frontier, next);
@@ -782,6 +791,13 @@ int or_bfs(const or_csr* adj, const or_csr* rev, const uint64_t* starts, uint64_
   *count_out = count;
   if (stats) *stats = st;
   return OR_OK;
+}
+
+int or_bfs(const or_csr* adj, const or_csr* rev, const uint64_t* starts, uint64_t nstarts,
+           const uint8_t* filter, int64_t max_depth, int64_t max_results, or_visit* visits,
+           uint64_t* count_out, or_stats* stats) {
+  return or_bfs_rows(adj->n, &adj, 1, rev ? &rev : nullptr, rev ? 1 : 0, starts, nstarts, filter, max_depth,
+                     max_results, visits, count_out, stats);
 }
 
 int or_bfs_depths(const or_csr* adj, const uint64_t* sources, uint64_t ns, uint32_t* depths,

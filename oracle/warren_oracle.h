@@ -139,6 +139,13 @@ int or_bfs(const or_csr* adj, const or_csr* pull_rev, const uint64_t* starts, ui
            const uint8_t* filter, int64_t max_depth, int64_t max_results, or_visit* visits_out,
            uint64_t* count_out, or_stats* stats_out);
 
+/* or_bfs over an Adjacency's CSR lists (traversal/adjacency.rs:20-221):
+ * rows[0..k) read in order for a node's row entries, rrows[0..kr) those of
+ * the reversed adjacency (NULL: no pulled level); id space n. */
+int or_bfs_rows(uint64_t n, const or_csr* const* rows, uint32_t k, const or_csr* const* rrows, uint32_t kr,
+                const uint64_t* starts, uint64_t nstarts, const uint8_t* filter, int64_t max_depth,
+                int64_t max_results, or_visit* visits_out, uint64_t* count_out, or_stats* stats_out);
+
 /* Depth of every node from each source (UINT32_MAX: unreached): the
  * per-source ground truth a multi-source BFS must match
  * (compute/centrality/multi_source.rs:92-193 reports the same distances). */

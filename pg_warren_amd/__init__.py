@@ -459,7 +459,7 @@ def pagerank_batch(ctx, graph, seed_sets, *, seed_weights=None, damping=0.85, to
     return out, its
 
 
-DIRECTIONS = {"out": 0, "in": 1}
+DIRECTIONS = {"out": 0, "in": 1, "both": 2}
 
 
 def _stats(st):
@@ -475,16 +475,29 @@ def _filter(node_filter):
     return None if node_filter is None else np.ascontiguousarray(node_filter, dtype=np.uint8)
 
 
-def bfs(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
+def _relations(relations):
+    """(array or None, count): a traversal's relation subset, None for every
+    relation (Adjacency::new, traversal/adjacency.rs:68-105)."""
+    if relations is None:
+        return None, 0
+    r = np.atleast_1d(np.asarray(relations, dtype=np.int64))
+    buf = np.zeros(max(len(r), 1), np.uint32)  # never a null pointer, even for no relation
+    buf[:len(r)] = r
+    return buf, len(r)
+
+
+def bfs(ctx, graph, starts, *, direction="out", relations=None, node_filter=None, max_depth=None,
         max_results=None):
     """bfs (traversal/bfs.rs:34-45): [(node, depth, via or None)] in BFS order
-    and the traversal stats."""
+    and the traversal stats. direction "out", "in" or "both"; relations a
+    subset of the graph's relations (None: all)."""
     st = _starts(starts)
     flt = _filter(node_filter)
+    rel, nrel = _relations(relations)
     out = (_lib.Visit * max(graph.n, 1))()
     cnt = C.c_uint64()
     stats = _lib.TraversalStats()
-    ctx.check(ctx.lib.wg_bfs(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st), _p(flt),
+    ctx.check(ctx.lib.wg_bfs(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st), len(st), _p(flt),
                              -1 if max_depth is None else max_depth,
                              -1 if max_results is None else max_results, C.addressof(out),
                              C.byref(cnt), C.byref(stats)))
@@ -494,13 +507,14 @@ def bfs(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None
     return visits, _stats(stats)
 
 
-def reachable_set(ctx, graph, starts, *, direction="out", max_depth=None):
+def reachable_set(ctx, graph, starts, *, direction="out", relations=None, max_depth=None):
     """reachable_set (traversal/reachable.rs:48-90): node ids in BFS order."""
     st = _starts(starts)
+    rel, nrel = _relations(relations)
     out = np.zeros(max(graph.n, 1), np.uint64)
     cnt = C.c_uint64()
     stats = _lib.TraversalStats()
-    ctx.check(ctx.lib.wg_reachable_set(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+    ctx.check(ctx.lib.wg_reachable_set(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st), len(st),
                                        -1 if max_depth is None else max_depth, _p(out),
                                        C.byref(cnt), C.byref(stats)))
     return out[:cnt.value].copy(), _stats(stats)
@@ -567,13 +581,15 @@ class BfsCursor:
     call, the starts first. next_level returns (visits, more): visits as
     [(node, depth, via or None)] and more False once no level follows."""
 
-    def __init__(self, ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
-                 max_results=None, reachable=False):
+    def __init__(self, ctx, graph, starts, *, direction="out", relations=None, node_filter=None,
+                 max_depth=None, max_results=None, reachable=False):
         self.ctx, self.graph = ctx, graph
         st = _starts(starts)
         flt = _filter(node_filter)
+        rel, nrel = _relations(relations)
         h = C.c_void_p()
-        ctx.check(ctx.lib.wg_bfs_cursor_create(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+        ctx.check(ctx.lib.wg_bfs_cursor_create(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st),
+                                               len(st),
                                                _p(flt), -1 if max_depth is None else max_depth,
                                                -1 if max_results is None else max_results,
                                                int(reachable), C.byref(h)))
@@ -617,7 +633,7 @@ class BfsCursor:
             pass
 
 
-def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_depth=None,
+def bfs_levels(ctx, graph, starts, *, direction="out", relations=None, node_filter=None, max_depth=None,
                pulling=True, depth=None, via=None):
     """The device-resident search: (depth, via) u32 Vectors (0xffffffff where
     unreached) and the traversal stats."""
@@ -628,7 +644,8 @@ def bfs_levels(ctx, graph, starts, *, direction="out", node_filter=None, max_dep
     if via is None:
         via = Vector(ctx, dtype="u32", id_space=graph.n)
     stats = _lib.TraversalStats()
-    ctx.check(ctx.lib.wg_bfs_levels(ctx.h, graph.h, DIRECTIONS[direction], _p(st), len(st),
+    rel, nrel = _relations(relations)
+    ctx.check(ctx.lib.wg_bfs_levels(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st), len(st),
                                     _p(flt), -1 if max_depth is None else max_depth,
                                     int(pulling), depth.h, via.h, C.byref(stats)))
     return depth, via, _stats(stats)

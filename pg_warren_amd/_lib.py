@@ -101,7 +101,7 @@ SIGNATURES = [
     ("wg_graph_load_segments", I32, [P, U64, P, P, P]),
     ("wg_csr_segment_lengths", I32, [U64, U64, P, P, P]),
     ("wg_graph_encode_segments", I32, [P, P, I32, P, P, P]),
-    ("wg_bfs_cursor_create", I32, [P, P, I32, P, U64, P, C.c_int64, C.c_int64, I32, P]),
+    ("wg_bfs_cursor_create", I32, [P, P, I32, P, U32, P, U64, P, C.c_int64, C.c_int64, I32, P]),
     ("wg_bfs_cursor_next", I32, [P, P, P, P, P]),
     ("wg_bfs_cursor_stats", I32, [P, P]),
     ("wg_bfs_cursor_destroy", None, [P]),
@@ -122,9 +122,9 @@ SIGNATURES = [
     ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
     ("wg_pagerank_backward", I32, [P, P, P, P, P, P, P, P, P, P]),
     ("wg_eigen", I32, [P, P, P, P, P, P, P]),
-    ("wg_bfs_levels", I32, [P, P, I32, P, U64, P, I64, I32, P, P, P]),
-    ("wg_bfs", I32, [P, P, I32, P, U64, P, I64, I64, P, P, P]),
-    ("wg_reachable_set", I32, [P, P, I32, P, U64, I64, P, P, P]),
+    ("wg_bfs_levels", I32, [P, P, I32, P, U32, P, U64, P, I64, I32, P, P, P]),
+    ("wg_bfs", I32, [P, P, I32, P, U32, P, U64, P, I64, I64, P, P, P]),
+    ("wg_reachable_set", I32, [P, P, I32, P, U32, P, U64, I64, P, P, P]),
     ("wg_msbfs", I32, [P, P, I32, P, U32, I64, P, P, P, P]),
     ("wg_device_copy", I32, [P, P, P, U64]),
     ("wg_rmat_csr", I32, [U32, U32, D, D, D, U64, I32, P, P, U64, P, I32]),

@@ -43,6 +43,22 @@ constexpr uint32_t kWordsPerCountBlock = 512;  // bfs_count / bfs_write: 2 words
 // entries sets vias by atomicMin while marking (bfs_push<true>).
 constexpr uint64_t kSmallPushRatio = 4;
 
+// The CSRs a traversal reads, in order (a node's row entries are its rows
+// concatenated, Adjacency::row_entries): passed by value to the kernels that
+// sum a node's degrees or find where a push meets a node.
+constexpr uint32_t kMaxRowSets = 32;
+struct RowSets {
+  const uint64_t* off[kMaxRowSets];
+  const uint32_t* tgt[kMaxRowSets];
+  uint32_t k;
+};
+
+__device__ __forceinline__ uint64_t degree_in(const RowSets& r, uint32_t v) {
+  uint64_t d = 0;
+  for (uint32_t j = 0; j < r.k; ++j) d += r.off[j][v + 1] - r.off[j][v];
+  return d;
+}
+
 struct BfsCounters {
   unsigned long long next_size;    // nodes claimed this level
   unsigned long long next_entries; // their rows' entries (m_f of the next level)
@@ -159,16 +175,48 @@ __device__ __forceinline__ uint32_t claim_word(uint64_t n, uint64_t w, uint8_t* 
   return bits;
 }
 
+// A pull over several CSRs (a traversal reading several row sets) runs one
+// pass per CSR in the reversed adjacency's order. Pass kind of a later pass:
+constexpr int kPassFirst = 1;  // rows claimed by an earlier pass are done:
+                               // the via is the first frontier node in the
+                               // rows' concatenated order (a pulled level)
+constexpr int kPassMin = 2;    // every pass searches every unvisited row and
+                               // the via is the smallest frontier node met
+                               // (the via a push gives)
+
+// Whether row v searches in a pull pass.
+__device__ __forceinline__ bool pull_searches(uint32_t v, int candidates, int pass, const uint8_t* cand,
+                                              const uint32_t* visited, const uint8_t* filter,
+                                              const uint32_t* depth, uint32_t d1) {
+  if (candidates) return cand[v] != 0;
+  if (bit_of(visited, v) || (filter && !filter[v])) return false;
+  return pass != kPassFirst || depth[v] != d1;
+}
+
+// Row v met frontier node u: claimed at depth d1 with via u, or, in a
+// kPassMin pass when an earlier pass claimed it, its via lowered to u.
+__device__ __forceinline__ void pull_claims(uint32_t v, uint32_t u, int pass, uint32_t* depth, uint32_t* via,
+                                            uint32_t d1) {
+  if (pass == kPassMin && depth[v] == d1) {
+    if (u < via[v]) via[v] = u;
+  } else {
+    via[v] = u;
+    depth[v] = d1;
+  }
+}
+
 // Pull over the scanned CSR's row blocks (roff/rtgt). candidates == 0: every
-// unvisited row admitted by the filter searches; otherwise only rows marked
-// in cand (the push's second half), which are cleared. Pieces of split rows
-// record their first hit (offset within the row) or ~0 in piece_hit for
-// bfs_pull_long. A pulled level's scanned entries go to part_edges.
+// unvisited row admitted by the filter searches (as `pass` allows);
+// otherwise only rows marked in cand (the push's second half), which are
+// cleared. Pieces of split rows record their first hit (offset within the
+// row) or ~0 in piece_hit for bfs_pull_long. A pulled level's scanned
+// entries go to part_edges. Claims are ORed into next_bm (zero at the
+// level's start).
 __global__ __launch_bounds__(256) void bfs_pull_blocks(
     const Block* __restrict__ blocks, const uint64_t* __restrict__ roff,
     const uint32_t* __restrict__ rtgt, const uint32_t* __restrict__ visited,
     const uint32_t* __restrict__ frontier, const uint8_t* __restrict__ filter,
-    uint8_t* __restrict__ cand, int candidates, uint32_t* __restrict__ depth,
+    uint8_t* __restrict__ cand, int candidates, int pass, uint32_t* __restrict__ depth,
     uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm, uint32_t* __restrict__ piece_hit,
     unsigned long long* __restrict__ part_edges, uint32_t d1) {
   __shared__ uint32_t lbm[kBlockRows / 32 + 2];
@@ -193,13 +241,7 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
     const unsigned long long gmask = T == 64 ? ~0ull : ((1ull << T) - 1);
     for (uint32_t r = g; r < nr; r += groups) {
       const uint32_t v = b.row0 + r;
-      bool search;
-      if (candidates) {
-        search = cand[v] != 0;
-      } else {
-        search = !bit_of(visited, v) && !(filter && !filter[v]);
-      }
-      if (search) {
+      if (pull_searches(v, candidates, pass, cand, visited, filter, depth, d1)) {
         const uint64_t a = roff[v], e = roff[v + 1];
         bool found = false;
         for (uint64_t base = a; base < e; base += T) {
@@ -214,8 +256,7 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
           if (gm) {
             const uint32_t first = __ffsll((long long)gm) - 1;
             if (l == first) {
-              via[v] = u;
-              depth[v] = d1;
+              pull_claims(v, u, pass, depth, via, d1);
               atomicOr(&lbm[(v >> 5) - w0], 1u << (v & 31));
             }
             if (l == 0) edges += (base - a) + first + 1;
@@ -233,16 +274,16 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
     for (uint32_t k = tid; k < nw; k += kBlockThreads) {
       const uint32_t word = w0 + k, bits = lbm[k];
       const bool whole = word * 32 >= b.row0 && word * 32 + 31 < b.row0 + nr;
+      if (!bits) continue;
       if (whole)
-        next_bm[word] = bits;
-      else if (bits)
+        next_bm[word] |= bits;
+      else
         atomicOr(&next_bm[word], bits);
     }
   } else {
     const uint32_t v = b.row0;
     uint32_t hit_at = ~0u;
-    const bool search = candidates ? cand[v] != 0 : (!bit_of(visited, v) && !(filter && !filter[v]));
-    if (search) {
+    if (pull_searches(v, candidates, pass, cand, visited, filter, depth, d1)) {
       const uint64_t a = roff[v], re = roff[v + 1];
       const uint64_t e0 = a + (uint64_t)b.piece * kBlockNnz;
       const uint64_t e1 = e0 + kBlockNnz < re ? e0 + kBlockNnz : re;
@@ -274,7 +315,7 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
     const uint32_t* __restrict__ long_rows, const uint32_t* __restrict__ long_pbase, uint32_t nlong,
     const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rtgt,
     const uint32_t* __restrict__ visited, const uint8_t* __restrict__ filter, uint8_t* __restrict__ cand,
-    int candidates, const uint32_t* __restrict__ piece_hit, uint32_t* __restrict__ depth,
+    int candidates, int pass, const uint32_t* __restrict__ piece_hit, uint32_t* __restrict__ depth,
     uint32_t* __restrict__ via, uint32_t* __restrict__ next_bm,
     unsigned long long* __restrict__ part_edges, uint32_t d1) {
   __shared__ unsigned long long red[4];
@@ -282,14 +323,12 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
   unsigned long long edges = 0;
   if (i < nlong) {
     const uint32_t v = long_rows[i];
-    const bool search = candidates ? cand[v] != 0 : (!bit_of(visited, v) && !(filter && !filter[v]));
-    if (search) {
+    if (pull_searches(v, candidates, pass, cand, visited, filter, depth, d1)) {
       const uint64_t a = roff[v], e = roff[v + 1];
       uint32_t at = ~0u;
       for (uint32_t p = long_pbase[i]; p < long_pbase[i + 1] && at == ~0u; ++p) at = piece_hit[p];
       if (at != ~0u) {
-        via[v] = rtgt[a + at];
-        depth[v] = d1;
+        pull_claims(v, rtgt[a + at], pass, depth, via, d1);
         atomicOr(&next_bm[v >> 5], 1u << (v & 31));
         edges = (unsigned long long)at + 1;
       } else {
@@ -305,17 +344,16 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
 }
 
 // Compaction, first pass: per block of words, the claimed nodes and their
-// forward / reverse row entries; also folds the claims into visited and
+// row entries in the rows read and in the reversed rows (summed over the
+// CSRs of each); also folds the claims into visited and
 // clears the old frontier bitmap (it becomes the next level's claim map).
 // kClaim (after a small push, whose vias are set): the claims come from the
 // candidate bytes here, and the claimed nodes take depth d1.
 template <bool kClaim>
 __global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uint32_t* __restrict__ claim,
                                                  uint32_t* __restrict__ visited,
-                                                 uint32_t* __restrict__ old_frontier,
-                                                 const uint64_t* __restrict__ off,
-                                                 const uint64_t* __restrict__ roff,
-                                                 unsigned long long* __restrict__ part,
+                                                 uint32_t* __restrict__ old_frontier, const RowSets adj,
+                                                 const RowSets rev, unsigned long long* __restrict__ part,
                                                  uint8_t* __restrict__ cand, uint32_t* __restrict__ depth,
                                                  uint32_t d1) {
   __shared__ unsigned long long red[4];
@@ -336,8 +374,8 @@ __global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uin
       const uint32_t v = (uint32_t)(w * 32) + (__ffs(bits) - 1);
       bits &= bits - 1;
       if (kClaim) depth[v] = d1;
-      ent += off[v + 1] - off[v];
-      vr += roff[v + 1] - roff[v];
+      ent += degree_in(adj, v);
+      vr += degree_in(rev, v);
     }
   }
   cnt = block_reduce_sum(cnt, red);
@@ -427,8 +465,63 @@ struct BfsRun {
   std::vector<uint32_t> level_pulled;  // per expanded level: 1 if pulled
   std::vector<uint64_t> level_size;    // nodes admitted at each depth (0 = starts)
   uint64_t visited = 0, edges = 0;
+  uint64_t push_edges = 0;  // the entries a push of every level would scan
   uint32_t levels = 0;
 };
+
+// The CSRs a traversal reads (Adjacency::new and Adjacency::reversed,
+// traversal/adjacency.rs:68-152, 288-320): with every relation, the graph's
+// own (all-relations) pair; with a subset (sorted, repeats dropped), each
+// relation's pair in index order. Out reads the forward rows, In the
+// reverse ones, Both the forward then the reverse; the reversed adjacency
+// (a pulled level's) reads the opposite direction, Both for Both.
+// first_via: a pulled level's via is the first frontier node in the
+// reversed rows' concatenated order (reachable sets); otherwise the one a
+// push gives, the smallest frontier node with an edge to the node.
+struct Traversal {
+  std::vector<DevCsr*> rows, rrows;
+  bool first_via = false;
+};
+
+static int traversal_of(wg_context* ctx, wg_graph* g, int direction, const uint32_t* rels, uint32_t nrel,
+                        bool first_via, Traversal& t) {
+  if (direction != WG_OUT && direction != WG_IN && direction != WG_BOTH)
+    return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
+  std::vector<wg_graph*> pairs;
+  if (!rels) {
+    pairs.push_back(g);
+  } else {
+    std::vector<uint32_t> sel(rels, rels + nrel);
+    std::sort(sel.begin(), sel.end());
+    sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
+    for (uint32_t r : sel)
+      if (r >= g->relation_count)
+        return fail(ctx, WG_UNKNOWN_RELATION, "relation " + std::to_string(r) + " is not in the graph");
+    for (uint32_t r : sel) pairs.push_back(g->relations.empty() ? g : g->relations[r]);
+  }
+  if (pairs.size() * (direction == WG_BOTH ? 2 : 1) > kMaxRowSets)
+    return fail(ctx, WG_INVALID_ARGUMENT, "a traversal reads at most 32 CSRs");
+  auto read = [&](int d, std::vector<DevCsr*>& out) {
+    for (wg_graph* pr : pairs) {
+      if (d != WG_IN) out.push_back(&pr->fwd);
+      if (d != WG_OUT) out.push_back(&pr->rev);
+    }
+  };
+  read(direction, t.rows);
+  read(direction == WG_OUT ? WG_IN : direction == WG_IN ? WG_OUT : WG_BOTH, t.rrows);
+  t.first_via = first_via;
+  return WG_OK;
+}
+
+static RowSets row_sets(const std::vector<DevCsr*>& rows) {
+  RowSets r{};
+  r.k = (uint32_t)rows.size();
+  for (uint32_t j = 0; j < r.k; ++j) {
+    r.off[j] = rows[j]->offsets;
+    r.tgt[j] = rows[j]->targets;
+  }
+  return r;
+}
 
 // The search's workspace: bitmaps, queues, candidate bytes, split-row piece
 // hits and compaction partials, in kWsSlots buffers.
@@ -441,19 +534,26 @@ struct BfsWorkspace {
 
 static uint32_t edge_parts(const DevCsr& c) { return c.nblocks + (c.nlong + 255) / 256; }
 
-// Allocates a workspace's buffers into slot[], recording them in `owned`
-// (charged to the context).
-static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, void* slot[kWsSlots],
-                           std::vector<std::pair<void*, size_t>>& owned) {
+// The workspace's buffer sizes for a search of g reading t (the pieces of
+// split rows and the blocks of every reversed CSR a pulled level scans).
+static void workspace_sizes(const wg_graph* g, const Traversal& t, size_t sizes[kWsSlots]) {
+  const uint64_t words = (g->n + 63) / 64 * 2;
   const uint64_t nbc = (words + kWordsPerCountBlock - 1) / kWordsPerCountBlock;
   const uint64_t n = std::max<uint64_t>(g->n, 1);
-  const size_t sizes[kWsSlots] = {3 * words * 4,
-                           2 * n * 4,
-                           std::max<uint32_t>(std::max(g->fwd.npartials, g->rev.npartials), 1) * 4,
-                           n,
-                           std::max<uint64_t>(nbc, 1) * 3 * 8,
-                           std::max<uint64_t>(nbc, 1) * 8,
-                           std::max<uint32_t>(std::max(edge_parts(g->fwd), edge_parts(g->rev)), 1) * 8};
+  uint64_t pieces = 1, parts = 0;
+  for (const DevCsr* c : t.rrows) {
+    pieces = std::max<uint64_t>(pieces, c->npartials);
+    parts += edge_parts(*c);
+  }
+  const size_t s[kWsSlots] = {3 * words * 4, 2 * n * 4, pieces * 4, n, std::max<uint64_t>(nbc, 1) * 3 * 8,
+                              std::max<uint64_t>(nbc, 1) * 8, std::max<uint64_t>(parts, 1) * 8};
+  for (int k = 0; k < kWsSlots; ++k) sizes[k] = s[k];
+}
+
+// Allocates a workspace's buffers of the given sizes into slot[], recording
+// them in `owned` (charged to the context).
+static int alloc_workspace(wg_context* ctx, const size_t sizes[kWsSlots], void* slot[kWsSlots],
+                           std::vector<std::pair<void*, size_t>>& owned) {
   for (int k = 0; k < kWsSlots; ++k) {
     void* p = nullptr;
     int s = dalloc(ctx, &p, sizes[k]);
@@ -462,11 +562,12 @@ static int alloc_workspace(wg_context* ctx, const wg_graph* g, uint64_t words, v
     slot[k] = p;
   }
   // Candidate bytes start clear and every resolve clears what it reads.
-  WG_HIP(ctx, hipMemsetAsync(slot[3], 0, n, ctx->stream));
+  WG_HIP(ctx, hipMemsetAsync(slot[3], 0, sizes[3], ctx->stream));
   return WG_OK;
 }
 
-static void view_workspace(const wg_graph* g, void* const slot[kWsSlots], uint64_t words, BfsWorkspace& w) {
+static void view_workspace(const wg_graph* g, void* const slot[kWsSlots], BfsWorkspace& w) {
+  const uint64_t words = (g->n + 63) / 64 * 2;
   uint32_t* bms = (uint32_t*)slot[0];
   uint32_t* qs = (uint32_t*)slot[1];
   w = BfsWorkspace{bms, bms + words, bms + 2 * words, qs, qs + std::max<uint64_t>(g->n, 1),
@@ -474,13 +575,42 @@ static void view_workspace(const wg_graph* g, void* const slot[kWsSlots], uint64
                    (unsigned long long*)slot[5], (unsigned long long*)slot[6]};
 }
 
-// The graph's own workspace, allocated on first use and kept with it.
-static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorkspace& w) {
+// The graph's own workspace, allocated on first use (sized for its own
+// pair read both ways) and kept with it; a search that needs more (several
+// relations' CSRs) gets one of its own in `own`, freed with it.
+struct WorkspaceLease {
+  wg_context* ctx = nullptr;
+  void* slot[kWsSlots] = {};
+  std::vector<std::pair<void*, size_t>> own;
+  ~WorkspaceLease() {
+    if (own.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : own) dfree(ctx, b.first, b.second);
+  }
+};
+
+static int bfs_workspace(wg_context* ctx, wg_graph* g, const Traversal& t, BfsWorkspace& w, WorkspaceLease& lease) {
+  size_t need[kWsSlots];
+  workspace_sizes(g, t, need);
   if (!g->bfs_ws[0]) {
-    int s = alloc_workspace(ctx, g, words, g->bfs_ws, g->workspace);
+    Traversal both;
+    both.rows = both.rrows = {&g->fwd, &g->rev};
+    size_t mine[kWsSlots];
+    workspace_sizes(g, both, mine);
+    for (int k = 0; k < kWsSlots; ++k) g->bfs_ws_bytes[k] = std::max(mine[k], need[k]);
+    int s = alloc_workspace(ctx, g->bfs_ws_bytes, g->bfs_ws, g->workspace);
     if (s) return s;
   }
-  view_workspace(g, g->bfs_ws, words, w);
+  bool fits = true;
+  for (int k = 0; k < kWsSlots; ++k) fits &= need[k] <= g->bfs_ws_bytes[k];
+  if (fits) {
+    view_workspace(g, g->bfs_ws, w);
+    return WG_OK;
+  }
+  lease.ctx = ctx;
+  int s = alloc_workspace(ctx, need, lease.slot, lease.own);
+  if (s) return s;
+  view_workspace(g, lease.slot, w);
   return WG_OK;
 }
 
@@ -488,11 +618,16 @@ static int bfs_workspace(wg_context* ctx, wg_graph* g, uint64_t words, BfsWorksp
 // steps (LevelSearch, traversal/level.rs:88-337). After init the frontier
 // is the starts; each step expands it into the next level, whose nodes are
 // then the frontier bitmap (frontier_size of them) and whose depth is d.
+// Levels are pushed or pulled by Beamer's switch (level.rs:184-211); a push
+// over several CSRs sets vias by atomicMin (the smallest frontier node that
+// reaches a node through any of them, as the reference's push); a pull over
+// several scans one CSR per pass in the reversed adjacency's order.
 struct BfsSearch {
   wg_context* ctx = nullptr;
   wg_graph* g = nullptr;
-  DevCsr* adj = nullptr;
-  DevCsr* rev = nullptr;
+  Traversal t;
+  RowSets ra{}, rr{};
+  uint64_t rr_nnz = 0;
   uint64_t n = 0, words = 0;
   uint32_t nbc = 0;
   BfsWorkspace w{};
@@ -506,14 +641,17 @@ struct BfsSearch {
   bool queue_ready = true;  // q_a holds the frontier as an ascending list
   BfsRun run;
 
-  int init(wg_context* c, wg_graph* gr, int direction, const std::vector<uint32_t>& starts,
+  int init(wg_context* c, wg_graph* gr, const Traversal& tr, const std::vector<uint32_t>& starts,
            const uint8_t* filter_d, int64_t maxd, bool pull_ok, uint32_t* depth_d, uint32_t* via_d,
            const BfsWorkspace& ws) {
     ctx = c;
     g = gr;
     n = g->n;
-    adj = direction == WG_IN ? &g->rev : &g->fwd;
-    rev = direction == WG_IN ? &g->fwd : &g->rev;
+    t = tr;
+    ra = row_sets(t.rows);
+    rr = row_sets(t.rrows);
+    rr_nnz = 0;
+    for (const DevCsr* cs : t.rrows) rr_nnz += cs->nnz;
     words = (n + 63) / 64 * 2;  // 32-bit words, whole 64-node waves
     nbc = (uint32_t)((words + kWordsPerCountBlock - 1) / kWordsPerCountBlock);
     w = ws;
@@ -533,8 +671,8 @@ struct BfsSearch {
           w.q_a, (uint32_t)starts.size(), depth, w.visited, w.bm_a);
       WG_HIP(ctx, hipGetLastError());
       for (uint32_t v : starts) {
-        mf += adj->h_offsets[v + 1] - adj->h_offsets[v];
-        visited_rev += rev->h_offsets[v + 1] - rev->h_offsets[v];
+        for (const DevCsr* cs : t.rows) mf += cs->h_offsets[v + 1] - cs->h_offsets[v];
+        for (const DevCsr* cs : t.rrows) visited_rev += cs->h_offsets[v + 1] - cs->h_offsets[v];
       }
     }
     frontier_size = starts.size();
@@ -555,43 +693,55 @@ struct BfsSearch {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
     hipStream_t st = ctx->stream;
     // Beamer's switch (level.rs:184-211): pull iff |F| * 24 > n and the
-    // unvisited nodes' reverse entries are fewer than alpha * m_f.
-    const uint64_t m_u = rev->nnz - visited_rev;
-    const bool pull = pulling && frontier_size * 24 > n && m_u < mf;
-    // A push whose frontier rows hold few entries sets vias as it marks;
-    // a larger one marks, then resolves the candidates by the pull kernel
-    // (each scanning its reverse row up to the first frontier node).
-    const bool small_push = !pull && mf * kSmallPushRatio < n;
+    // unvisited nodes' reverse entries are fewer than alpha * m_f (or no
+    // node is unvisited).
+    const uint64_t m_u = rr_nnz - visited_rev;
+    const bool pull = pulling && frontier_size * 24 > n && (m_u < mf || run.visited == n);
+    const bool multi = t.rows.size() > 1;
+    // A push whose frontier rows hold few entries, or that reads several
+    // CSRs, sets vias as it marks; a larger one over one CSR marks, then
+    // resolves the candidates by the pull kernel (each scanning its reverse
+    // row up to the first frontier node).
+    const bool small_push = !pull && (multi || mf * kSmallPushRatio < n);
     if (!pull) {
       if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
       const unsigned qblocks = (unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64);
-      const unsigned grid = qblocks + adj->npartials;
-      if (small_push)
-        bfs_push<true><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, adj->pieces, adj->offsets, adj->targets,
-                                             frontier, w.visited, filter, w.cand, via);
-      else
-        bfs_push<false><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, adj->pieces, adj->offsets,
-                                              adj->targets, frontier, w.visited, filter, w.cand, via);
+      for (DevCsr* a : t.rows) {
+        const unsigned grid = qblocks + a->npartials;
+        if (small_push)
+          bfs_push<true><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, a->pieces, a->offsets, a->targets,
+                                               frontier, w.visited, filter, w.cand, via);
+        else
+          bfs_push<false><<<grid, 256, 0, st>>>(w.q_a, frontier_size, qblocks, a->pieces, a->offsets,
+                                                a->targets, frontier, w.visited, filter, w.cand, via);
+      }
     }
+    uint32_t nedges = 0;
     if (small_push) {
-      bfs_count<true><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, adj->offsets, rev->offsets, w.part,
-                                           w.cand, depth, d + 1);
+      bfs_count<true><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, w.cand, depth,
+                                           d + 1);
     } else {
-      const int cands = pull ? 0 : 1;
-      if (rev->nblocks)
-        bfs_pull_blocks<<<rev->nblocks, kBlockThreads, 0, st>>>(
-            rev->blocks, rev->offsets, rev->targets, w.visited, frontier, filter, w.cand, cands, depth, via,
-            claim, w.piece_hit, w.part_edges, d + 1);
-      if (rev->nlong)
-        bfs_pull_long<<<(rev->nlong + 255) / 256, 256, 0, st>>>(
-            rev->long_rows, rev->long_pbase, rev->nlong, rev->offsets, rev->targets, w.visited, filter,
-            w.cand, cands, w.piece_hit, depth, via, claim, w.part_edges + rev->nblocks, d + 1);
-      bfs_count<false><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, adj->offsets, rev->offsets,
-                                            w.part, nullptr, nullptr, 0);
+      // One pass per reversed CSR (one after a push: its candidates).
+      for (size_t j = 0; j < t.rrows.size(); ++j) {
+        const DevCsr* r = t.rrows[j];
+        const int cands = pull ? 0 : 1;
+        const int pass = j == 0 ? 0 : t.first_via ? kPassFirst : kPassMin;
+        if (r->nblocks)
+          bfs_pull_blocks<<<r->nblocks, kBlockThreads, 0, st>>>(
+              r->blocks, r->offsets, r->targets, w.visited, frontier, filter, w.cand, cands, pass, depth, via,
+              claim, w.piece_hit, w.part_edges + nedges, d + 1);
+        if (r->nlong)
+          bfs_pull_long<<<(r->nlong + 255) / 256, 256, 0, st>>>(
+              r->long_rows, r->long_pbase, r->nlong, r->offsets, r->targets, w.visited, filter, w.cand, cands,
+              pass, w.piece_hit, depth, via, claim, w.part_edges + nedges + r->nblocks, d + 1);
+        nedges += edge_parts(*r);
+        if (!pull) break;
+      }
+      bfs_count<false><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, nullptr,
+                                            nullptr, 0);
     }
     const unsigned long long seq = ++ctx->mail_seq;
-    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? edge_parts(*rev) : 0, ctx->d_mail,
-                                 seq);
+    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? nedges : 0, ctx->d_mail, seq);
     // After a push the next level is most often pushed too: its queue is
     // listed now, while the host waits for the counters.
     if (!pull) bfs_write<<<nbc, 256, 0, st>>>(words, claim, w.offsets, w.q_a);
@@ -604,6 +754,7 @@ struct BfsSearch {
     queue_ready = !pull;
     // A pushed level scans every entry of every frontier row.
     run.edges += pull ? hc.edges : mf;
+    run.push_edges += mf;
     frontier_size = hc.next_size;
     mf = hc.next_entries;
     visited_rev += hc.visited_rev;
@@ -627,15 +778,15 @@ struct BfsSearch {
 
 // Runs the search, leaving depth and via (kUnvisited where unreached) in
 // device buffers. starts must be in range and deduplicated, in order.
-int bfs_run(wg_context* ctx, wg_graph* g, int direction, const std::vector<uint32_t>& starts,
+int bfs_run(wg_context* ctx, wg_graph* g, const Traversal& t, const std::vector<uint32_t>& starts,
             const uint8_t* filter_d, int64_t max_depth, bool pulling, uint32_t* depth, uint32_t* via,
             BfsRun& run) {
   BfsWorkspace w;
-  const uint64_t words = (g->n + 63) / 64 * 2;
-  int s = bfs_workspace(ctx, g, words, w);
+  WorkspaceLease lease;
+  int s = bfs_workspace(ctx, g, t, w, lease);
   if (s) return s;
   BfsSearch S;
-  if ((s = S.init(ctx, g, direction, starts, filter_d, max_depth, pulling, depth, via, w))) return s;
+  if ((s = S.init(ctx, g, t, starts, filter_d, max_depth, pulling, depth, via, w))) return s;
   while (!S.exhausted())
     if ((s = S.step())) return s;
   run = S.run;
@@ -678,6 +829,61 @@ __global__ __launch_bounds__(256) void via_keys(const uint32_t* __restrict__ nod
     keys[i] = via[nodes[i]];
 }
 
+// keys[i]: the first of the CSRs read whose row of via[v] holds v =
+// nodes[i] (binary search: rows ascend), so a pushed level over several
+// CSRs lists each via's nodes in the order its concatenated rows reach them.
+__global__ __launch_bounds__(256) void row_set_keys(const uint32_t* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ via, uint64_t count,
+                                                    const RowSets ra, uint32_t* __restrict__ keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = nodes[i], u = via[v];
+    uint32_t key = ra.k;
+    for (uint32_t j = 0; j < ra.k && key == ra.k; ++j) {
+      uint64_t lo = ra.off[j][u], hi = ra.off[j][u + 1];
+      while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ra.tgt[j][mid] < v) lo = mid + 1; else hi = mid;
+      }
+      if (lo < ra.off[j][u + 1] && ra.tgt[j][lo] == v) key = j;
+    }
+    keys[i] = key;
+  }
+}
+
+// The row entries a push scans in a level up to and including the one that
+// reaches *refused (the visit max_results turned away), Progress's count
+// (level.rs:213-236): every row entry of the frontier nodes (depth dprev)
+// below its via, then the via's rows up to the entry. Blocks add their sums
+// to *out, which starts at 0.
+__global__ __launch_bounds__(256) void push_scan_edges(const uint32_t* __restrict__ refused,
+                                                       const uint32_t* __restrict__ via,
+                                                       const uint32_t* __restrict__ depth, uint32_t dprev,
+                                                       const RowSets ra, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[4];
+  const uint32_t v = *refused, u = via[v];
+  unsigned long long sum = 0;
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < u; x += (uint64_t)gridDim.x * 256)
+    if (depth[x] == dprev) sum += degree_in(ra, (uint32_t)x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (uint32_t j = 0; j < ra.k; ++j) {
+      const uint64_t a = ra.off[j][u], e = ra.off[j][u + 1];
+      uint64_t lo = a, hi = e;
+      while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ra.tgt[j][mid] < v) lo = mid + 1; else hi = mid;
+      }
+      if (lo < e && ra.tgt[j][lo] == v) {
+        sum += lo - a + 1;
+        break;
+      }
+      sum += e - a;
+    }
+  }
+  sum = block_reduce_sum(sum, red);
+  if (threadIdx.x == 0 && sum) atomicAdd(out, sum);
+}
+
 // The level's visits in emission order.
 __global__ __launch_bounds__(256) void write_visits(const uint32_t* __restrict__ nodes,
                                                     const uint32_t* __restrict__ via, uint64_t count,
@@ -690,12 +896,15 @@ __global__ __launch_bounds__(256) void write_visits(const uint32_t* __restrict__
 }
 
 // A search that hands out its visits level by level, in the reference's
-// order: the starts in the order given; a pushed level by (via, node), the
-// order reached when frontier nodes expand in ascending order; a pulled
-// level by node when `ascending_pulls` (a reachable set), else also by
-// (via, node). Visits are sorted and written on the device; a level's rows
-// reach the host in one copy. At most max_results visits (< 0: no limit);
-// a visit beyond them sets truncated.
+// order: the starts in the order given; a pushed level by (via, first CSR
+// whose row of the via holds the node, node), the order reached when
+// frontier nodes expand in ascending order; a pulled level by node when
+// `reachable` (a reachable set), else as if pushed. Visits are sorted and
+// written on the device; a level's rows reach the host in one copy. At
+// most max_results visits (< 0: no limit); a visit beyond them sets
+// truncated. Outside a reachable set (whose levels the reference may pull)
+// the stats are a push's: every level's frontier row entries, up to the
+// refused visit's entry in a truncated level, and no level pulled.
 struct LevelEmitter {
   wg_context* ctx = nullptr;
   wg_graph* g = nullptr;
@@ -703,12 +912,14 @@ struct LevelEmitter {
   void* slot[kWsSlots] = {};
   BfsSearch S;
   std::vector<uint32_t> starts;
-  uint32_t *depth = nullptr, *via = nullptr, *keys_in = nullptr, *keys_out = nullptr;
+  uint32_t *depth = nullptr, *via = nullptr, *keys_in = nullptr, *keys_out = nullptr, *nodes_tmp = nullptr;
+  unsigned long long* scan_count = nullptr;
   uint8_t* filter = nullptr;
   wg_visit* visits = nullptr;
   int64_t max_results = -1;
-  bool ascending_pulls = false, seeded = false, finished = false, truncated = false;
+  bool reachable = false, seeded = false, finished = false, truncated = false;
   uint64_t emitted = 0;
+  uint64_t push_edges = 0;  // stats of a push, outside a reachable set
 
   ~LevelEmitter() {
     if (ctx) {
@@ -723,33 +934,63 @@ struct LevelEmitter {
     return s;
   }
 
-  int init(wg_context* c, wg_graph* gr, int direction, const uint64_t* st, uint64_t ns,
-           const uint8_t* host_filter, int64_t max_depth, int64_t maxr, bool pulling, bool asc) {
+  int init(wg_context* c, wg_graph* gr, int direction, const uint32_t* rels, uint32_t nrel, const uint64_t* st,
+           uint64_t ns, const uint8_t* host_filter, int64_t max_depth, int64_t maxr, bool pulling, bool reach) {
     ctx = c;
     g = gr;
     max_results = maxr;
-    ascending_pulls = asc;
-    int s = prepare_starts(ctx, g->n, st, ns, starts);
+    reachable = reach;
+    Traversal t;
+    int s = traversal_of(ctx, g, direction, rels, nrel, reachable, t);
     if (s) return s;
+    if ((s = prepare_starts(ctx, g->n, st, ns, starts))) return s;
     const uint64_t n = std::max<uint64_t>(g->n, 1);
-    const uint64_t words = (g->n + 63) / 64 * 2;
-    if ((s = alloc_workspace(ctx, g, words, slot, owned))) return s;
+    size_t sizes[kWsSlots];
+    workspace_sizes(g, t, sizes);
+    if ((s = alloc_workspace(ctx, sizes, slot, owned))) return s;
     if ((s = buffer(n * 4, (void**)&depth)) || (s = buffer(n * 4, (void**)&via)) ||
         (s = buffer(n * 4, (void**)&keys_in)) || (s = buffer(n * 4, (void**)&keys_out)) ||
-        (s = buffer(n * sizeof(wg_visit), (void**)&visits)))
+        (s = buffer(n * sizeof(wg_visit), (void**)&visits)) || (s = buffer(8, (void**)&scan_count)))
       return s;
+    if (t.rows.size() > 1 && (s = buffer(n * 4, (void**)&nodes_tmp))) return s;
     if (host_filter) {
       if ((s = buffer(n, (void**)&filter))) return s;
       WG_HIP(ctx, hipMemcpy(filter, host_filter, g->n, hipMemcpyHostToDevice));
     }
     BfsWorkspace w;
-    view_workspace(g, slot, words, w);
-    return S.init(ctx, g, direction, starts, filter, max_depth, pulling, depth, via, w);
+    view_workspace(g, slot, w);
+    return S.init(ctx, g, t, starts, filter, max_depth, pulling, depth, via, w);
   }
 
   // The room left under max_results.
   uint64_t room() const {
     return max_results < 0 ? UINT64_MAX : (uint64_t)max_results - std::min<uint64_t>(emitted, max_results);
+  }
+
+  // The level's nodes in emission order (device; size of them).
+  int order_level(uint64_t size, const uint32_t** nodes) {
+    int s;
+    *nodes = S.w.q_a;
+    if (reachable && S.run.level_pulled.back()) return WG_OK;  // by node
+    const unsigned grid = (unsigned)std::min<uint64_t>((size + 255) / 256, 4096);
+    unsigned bits = 1;
+    while (bits < 32 && (1ull << bits) < g->n) ++bits;
+    const uint32_t* by_node = S.w.q_a;
+    if (S.t.rows.size() > 1) {
+      // Stable sorts, least significant key first: by CSR, then by via.
+      row_set_keys<<<grid, 256, 0, ctx->stream>>>(S.w.q_a, via, size, S.ra, keys_in);
+      WG_HIP(ctx, hipGetLastError());
+      unsigned kbits = 1;
+      while ((1u << kbits) <= S.ra.k) ++kbits;
+      if ((s = sort_u32_pairs(ctx, keys_in, keys_out, S.w.q_a, nodes_tmp, size, kbits))) return s;
+      by_node = nodes_tmp;
+    }
+    via_keys<<<grid, 256, 0, ctx->stream>>>(by_node, via, size, keys_in);
+    WG_HIP(ctx, hipGetLastError());
+    // Stable: equal vias keep the order above.
+    if ((s = sort_u32_pairs(ctx, keys_in, keys_out, by_node, S.w.q_b, size, bits))) return s;
+    *nodes = S.w.q_b;
+    return WG_OK;
   }
 
   // The next level's visits into out (host, room for the graph's n);
@@ -769,26 +1010,34 @@ struct LevelEmitter {
       for (uint64_t i = 0; i < k; ++i) out[i] = wg_visit{starts[i], UINT64_MAX, 0, 0};
       *count = k;
     } else {
+      const uint64_t level_entries = S.mf;
       if ((s = S.step())) return s;
       size = S.frontier_size;
       const uint64_t k = std::min<uint64_t>(size, room());
-      if (k) {
+      const bool cut = k < size;
+      push_edges += level_entries;
+      if (k || (cut && !reachable)) {
         if ((s = S.list_frontier())) return s;
-        const uint32_t* nodes = S.w.q_a;
-        const bool by_node = ascending_pulls && S.run.level_pulled.back();
-        const unsigned grid = (unsigned)std::min<uint64_t>((size + 255) / 256, 4096);
-        if (!by_node) {
-          via_keys<<<grid, 256, 0, ctx->stream>>>(S.w.q_a, via, size, keys_in);
+        const uint32_t* nodes;
+        if ((s = order_level(size, &nodes))) return s;
+        if (k) {
+          const unsigned grid = (unsigned)std::min<uint64_t>((k + 255) / 256, 4096);
+          write_visits<<<grid, 256, 0, ctx->stream>>>(nodes, via, k, S.d, visits);
           WG_HIP(ctx, hipGetLastError());
-          // Stable: equal vias keep ascending node order.
-          unsigned bits = 1;
-          while (bits < 32 && (1ull << bits) < g->n) ++bits;
-          if ((s = sort_u32_pairs(ctx, keys_in, keys_out, S.w.q_a, S.w.q_b, size, bits))) return s;
-          nodes = S.w.q_b;
+          WG_HIP(ctx, hipMemcpyAsync(out, visits, k * sizeof(wg_visit), hipMemcpyDeviceToHost, ctx->stream));
         }
-        write_visits<<<grid, 256, 0, ctx->stream>>>(nodes, via, k, S.d, visits);
-        WG_HIP(ctx, hipGetLastError());
-        WG_HIP(ctx, hipMemcpyAsync(out, visits, k * sizeof(wg_visit), hipMemcpyDeviceToHost, ctx->stream));
+        if (cut && !reachable) {
+          // The push stops at the refused visit: its level counts the
+          // entries scanned up to it.
+          unsigned long long scanned = 0;
+          WG_HIP(ctx, hipMemsetAsync(scan_count, 0, 8, ctx->stream));
+          push_scan_edges<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((g->n + 255) / 256, 2048)), 256, 0,
+                            ctx->stream>>>(nodes + k, via, depth, S.d - 1, S.ra, scan_count);
+          WG_HIP(ctx, hipGetLastError());
+          WG_HIP(ctx, hipMemcpyAsync(&scanned, scan_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+          WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+          push_edges += scanned - level_entries;
+        }
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
       }
       *count = k;
@@ -803,6 +1052,7 @@ struct LevelEmitter {
   }
 
   wg_traversal_stats stats() const {
+    if (!reachable) return wg_traversal_stats{emitted, push_edges, truncated ? 1u : 0u, 0};
     uint32_t pulled = 0;
     for (uint32_t p : S.run.level_pulled) pulled += p;
     return wg_traversal_stats{emitted, S.run.edges, truncated ? 1u : 0u, pulled};
@@ -819,26 +1069,27 @@ struct wg_bfs_cursor {
 
 extern "C" {
 
-int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint64_t* starts,
-                  uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int pulling,
-                  wg_vector* depth, wg_vector* via, wg_traversal_stats* stats) {
+int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint32_t* relations,
+                  uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
+                  int64_t max_depth, int pulling, wg_vector* depth, wg_vector* via, wg_traversal_stats* stats) {
   if (!ctx || !gc || !depth || !via) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   wg_graph* g = const_cast<wg_graph*>(gc);
-  if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
+  Traversal t;
+  int s = traversal_of(ctx, g, direction, relations, nrelations, false, t);
+  if (s) return s;
   for (wg_vector* v : {depth, via})
     if (v->dtype != WG_U32 || v->width != 1 || v->id_space != g->n)
       return fail(ctx, WG_INVALID_ARGUMENT, "depth and via must be u32 vectors over the graph");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   std::vector<uint32_t> st;
-  int s = prepare_starts(ctx, g->n, starts, nstarts, st);
-  if (s) return s;
+  if ((s = prepare_starts(ctx, g->n, starts, nstarts, st))) return s;
   DevBytes fb{ctx};
   if (filter) {
     if ((s = fb.alloc(g->n))) return s;
     WG_HIP(ctx, hipMemcpy(fb.p, filter, g->n, hipMemcpyHostToDevice));
   }
   BfsRun run;
-  s = bfs_run(ctx, g, direction, st, (const uint8_t*)fb.p, max_depth, pulling != 0,
+  s = bfs_run(ctx, g, t, st, (const uint8_t*)fb.p, max_depth, pulling != 0,
               (uint32_t*)depth->data, (uint32_t*)via->data, run);
   if (s) return s;
   if (stats) {
@@ -863,31 +1114,29 @@ static int emit_all(wg_context* ctx, LevelEmitter& e, wg_visit* out, uint64_t* c
   return WG_OK;
 }
 
-int wg_bfs(wg_context* ctx, const wg_graph* gc, int direction, const uint64_t* starts,
-           uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
+int wg_bfs(wg_context* ctx, const wg_graph* gc, int direction, const uint32_t* relations, uint32_t nrelations,
+           const uint64_t* starts, uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
            wg_visit* out, uint64_t* count, wg_traversal_stats* stats) {
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
-  if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   LevelEmitter e;
-  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, filter, max_depth,
-                 max_results, true, false);
+  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, relations, nrelations, starts, nstarts, filter,
+                 max_depth, max_results, true, false);
   if (!s) s = emit_all(ctx, e, out, count);
   if (s) return s;
   if (stats) *stats = e.stats();
   return ok(ctx);
 }
 
-int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const uint64_t* starts,
-                     uint64_t nstarts, int64_t max_depth, uint64_t* out, uint64_t* count,
-                     wg_traversal_stats* stats) {
+int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const uint32_t* relations,
+                     uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, int64_t max_depth, uint64_t* out,
+                     uint64_t* count, wg_traversal_stats* stats) {
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
-  if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   LevelEmitter e;
   std::vector<wg_visit> v(std::max<uint64_t>(gc->n, 1));
-  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, starts, nstarts, nullptr, max_depth, -1,
-                 true, true);
+  int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, relations, nrelations, starts, nstarts, nullptr,
+                 max_depth, -1, true, true);
   if (!s) s = emit_all(ctx, e, v.data(), count);
   if (s) return s;
   for (uint64_t i = 0; i < *count; ++i) out[i] = v[i].node;
@@ -895,12 +1144,11 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const u
   return ok(ctx);
 }
 
-int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint64_t* starts,
-                         uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
-                         int reachable, wg_bfs_cursor** out) {
+int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, const uint32_t* relations,
+                         uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
+                         int64_t max_depth, int64_t max_results, int reachable, wg_bfs_cursor** out) {
   if (!ctx || !g || !out) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
-  if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
   if (reachable && (filter || max_results >= 0))
     return fail(ctx, WG_INVALID_ARGUMENT, "a reachable-set cursor takes no filter or result limit");
   WG_HIP(ctx, hipSetDevice(ctx->device));
@@ -908,8 +1156,8 @@ int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, cons
   if (!c) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
   // Levels are pushed or pulled as wg_bfs does (same visits and stats); a
   // reachable set lists a pulled level by node (bfs_cursor.rs:74-87).
-  int s = c->e.init(ctx, const_cast<wg_graph*>(g), direction, starts, nstarts, filter, max_depth,
-                    max_results, true, reachable != 0);
+  int s = c->e.init(ctx, const_cast<wg_graph*>(g), direction, relations, nrelations, starts, nstarts, filter,
+                    max_depth, max_results, true, reachable != 0);
   if (s) {
     delete c;
     return s;

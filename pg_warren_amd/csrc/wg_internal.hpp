@@ -123,6 +123,7 @@ struct wg_graph {
   // (charged to the context) until the graph is destroyed.
   std::vector<std::pair<void*, size_t>> workspace;
   void* bfs_ws[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t bfs_ws_bytes[7] = {};
   // Other kernels' per-graph buffers by slot: (pointer, bytes).
   std::map<int, std::pair<void*, size_t>> cached;
   // Hot-first numbering of message sources (built on first use): nodes with
