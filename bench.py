@@ -591,7 +591,7 @@ def main():
     depth = W.Vector(ctx, dtype="u32", id_space=n)
     via = W.Vector(ctx, dtype="u32", id_space=n)
     W.bfs_levels(ctx, gf, [int(sources[0])], depth=depth, via=via)  # warm
-    teps, times, levels, pulled = [], [], [], []
+    teps, times, levels, pulled, gbs = [], [], [], [], []
     bfs_depths = []
     for s in sources:
         t1 = time.perf_counter()
@@ -603,14 +603,22 @@ def main():
         times.append(el)
         pulled.append(st["levels_pulled"])
         levels.append(int(d[reached].max()))
+        # Algorithmic bytes of a search: the offsets and the three bitmaps
+        # once per expanded level, 4 bytes per row entry scanned, depth and
+        # via written once.
+        expanded = levels[-1] + 1
+        byt = (n + 1) * 8 * expanded + st["edges_traversed"] * 4 + (n // 8) * 3 * expanded + 8 * n
+        gbs.append(byt / el / 1e9)
         if len(bfs_depths) < 2:
             bfs_depths.append((int(s), d))
     hm = len(teps) / sum(1.0 / t for t in teps)
     legs["bfs_64_sources"] = {"gteps_harmonic_mean": round(hm / 1e9, 3),
                               "ms_mean": round(1e3 * float(np.mean(times)), 3),
                               "levels_mean": float(np.mean(levels)),
-                              "levels_pulled_mean": float(np.mean(pulled))}
-    log(f"bfs: {hm / 1e9:.2f} GTEPS (harmonic mean), {1e3 * np.mean(times):.2f} ms/source")
+                              "levels_pulled_mean": float(np.mean(pulled)),
+                              "gbs_algorithmic_mean": round(float(np.mean(gbs)), 1)}
+    log(f"bfs: {hm / 1e9:.2f} GTEPS (harmonic mean), {1e3 * np.mean(times):.2f} ms/source, "
+        f"{np.mean(gbs):.0f} GB/s algorithmic")
     # The visit rows too (wg_bfs: every level sorted by (via, node) and
     # written on the device, copied to host memory level by level), from 8
     # of the sources, and the same through the level cursor.

@@ -406,7 +406,15 @@ __global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long
     ent += part[3 * b + 1];
     vr += part[3 * b + 2];
   }
-  for (uint32_t k = tid; k < nedges; k += 1024) ed += part_edges[k];
+  {
+    // Eight loads in flight a thread (the partials are many after a pull).
+    unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t k = tid;
+    for (; k + 7 * 1024 < nedges; k += 8 * 1024)
+      for (int j = 0; j < 8; ++j) acc[j] += part_edges[k + j * 1024];
+    for (; k < nedges; k += 1024) acc[0] += part_edges[k];
+    for (int j = 0; j < 8; ++j) ed += acc[j];
+  }
   sh[tid] = local;
   __syncthreads();
   for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
