@@ -503,6 +503,8 @@ def main():
                     help="leave out the Lanczos leg (symmetrised R-MAT one scale below)")
     ap.add_argument("--only-spmv", action="store_true",
                     help="profiling runs: the three SpMV legs and the copy kernel only")
+    ap.add_argument("--only-lanczos", action="store_true",
+                    help="profiling runs: the Lanczos leg only (symmetrised R-MAT one scale below)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -511,6 +513,12 @@ def main():
         sys.exit(2)
 
     import pg_warren_amd as W
+
+    if args.only_lanczos:
+        ctx = W.Context(0)
+        leg, _ = bench_lanczos(W, ctx, scale=max(args.scale - 1, 8))
+        print(json.dumps({"legs": {"lanczos": leg}}), flush=True)
+        return
 
     t0 = time.time()
     off, tgt = W.rmat(args.scale, 16, seed=1)
