@@ -233,8 +233,11 @@ int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_ve
  * fixed at 4096-row tasks, compute/product/kernel.rs:15). By default (-1) the
  * first product over a graph orientation of at least 2^22 entries, for each
  * dtype and semiring, times the launch variants and keeps the fastest:
- * bit 0 = the LDS-free direct kernel (only where the monoid's result does not
- * depend on grouping), bit 1 = non-temporal row streams, bit 2 = XCD-contiguous
+ * bit 0 = the LDS-free direct kernel (where the monoid's result does not
+ * depend on grouping, and for float plus, where its grouping gives other bits
+ * within the rounding bound: the tuner leaves it out there, so a tuned
+ * result has the default launch's bits), bit 1 = non-temporal row streams,
+ * bit 2 = XCD-contiguous
  * block order, bit 3 = gathers from x permuted into the hot-first column order
  * (columns ranked by how many entries read them; same values, same fold
  * order, same result bits), bit 4 = a persistent kernel that keeps the

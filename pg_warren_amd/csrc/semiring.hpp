@@ -189,6 +189,17 @@ struct Fold<int64_t, PLUS> {
   }
 };
 
+// u32 min: the u32 image keeps the i64 order (u32::MAX, standing for
+// i64::MAX, is the largest), so the smaller u32 is the i64 min's image and
+// no range check can fail.
+template <>
+struct Fold<uint32_t, MIN> {
+  using A = uint32_t;
+  __device__ static A lift(uint32_t p) { return p; }
+  __device__ static A combine(A a, A b, uint32_t&) { return b < a ? b : a; }
+  __device__ static uint32_t finish(A a, uint32_t&) { return a; }
+};
+
 // u32 plus: terms are non-negative, so prefix sums only grow: the run keeps
 // its finite sum and how many infinities it holds.
 template <>

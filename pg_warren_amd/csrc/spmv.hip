@@ -509,6 +509,11 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       if (csr.nblocks == 0) return WG_OK;
       A* part = (A*)ctx->scratch;
       constexpr bool direct_ok = grouping_free<E, ADD>();
+      // The direct kernel groups a row's float sum by lane stride, the
+      // blocked one by contiguous run: both within the rounding bound, but
+      // not the same bits, so the tuner (whose choice follows timing) only
+      // takes it where the bits are the same.
+      constexpr bool direct_tunable = direct_ok && !(is_float_v<E> && ADD == PLUS);
       // The hot-first gathers change only where x is read from: the same
       // values, folded in the same order, so the result is the same bits.
       auto hot_ok = [&]() {
@@ -600,7 +605,7 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
         const bool hot = hot_ok();
         const bool lds = hot && lds_ok();
         for (int v = 0; v < 32; ++v) {
-          if ((v & kVarDirect) && !direct_ok) continue;
+          if ((v & kVarDirect) && !direct_tunable) continue;
           if ((v & kVarHot) && !hot) continue;
           // The LDS kernel is persistent, LDS-staged and reads x as given.
           if ((v & kVarLds) && (!lds || (v & (kVarDirect | kVarXcd | kVarHot)))) continue;

@@ -312,6 +312,30 @@ def test_autotuner_choice_is_a_valid_variant(ctx, oracle):
     assert tuned.tobytes() == plain.tobytes()
 
 
+def test_tuned_float_plus_has_the_default_bits(ctx, oracle):
+    """Random (not integer-valued) f32 and f64 input: whatever variant the
+    tuner keeps, the result is the default launch's bits (the direct
+    kernel, whose float-plus grouping differs, is not a candidate)."""
+    off, tgt = W.rmat(19, 16, seed=4)
+    n = len(off) - 1
+    w = W.edge_weights(off, tgt, "uniform01", seed=6)
+    rng = np.random.default_rng(8)
+    for dtype in ("f32", "f64"):
+        g = W.Graph(ctx, n, off, tgt, w)
+        x = (rng.random(n) * 2 - 1).astype(W.NP_DTYPES[dtype])
+        xv = W.Vector(ctx, x)
+        for transpose in (False, True):
+            prod = W.Product("plus_times", transpose=transpose)
+            tuned = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+            choice = W.spmv_variant(g, "plus_times", dtype, transpose=transpose)
+            assert 0 <= choice < 32 and not choice & 1
+            W.set_spmv_variant(ctx, 0)
+            plain = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+            W.set_spmv_variant(ctx, -1)
+            assert tuned.tobytes() == plain.tobytes(), (dtype, transpose, choice)
+        g.close()
+
+
 def test_repeated_launches_are_bit_identical(ctx, oracle):
     g = hub_graph(oracle, 9)
     x = np.random.default_rng(1).random(g.n).astype(np.float32)
