@@ -630,9 +630,12 @@ def main():
     # The visit rows too (wg_bfs: every level sorted by (via, node) and
     # written on the device, copied to host memory level by level), from 8
     # of the sources, and the same through the level cursor.
-    out = (W._lib.Visit * n)()
+    # Rows land in page-locked memory (wg_host_alloc), one DMA per level;
+    # a pageable destination is timed too.
+    out = ctx.host_array(W._lib.Visit, n)
+    pageable = (W._lib.Visit * n)()
     cnt, stt = C.c_uint64(), W._lib.TraversalStats()
-    row_times, rows = [], []
+    row_times, rows, pageable_times = [], [], []
     for s in sources[:8]:
         st_arr = np.array([s], np.uint64)
         t1 = time.perf_counter()
@@ -640,6 +643,11 @@ def main():
                              C.byref(cnt), C.byref(stt)))
         row_times.append(time.perf_counter() - t1)
         rows.append(cnt.value)
+        t1 = time.perf_counter()
+        ctx.check(lib.wg_bfs(ctx.h, gf.h, 0, None, 0, st_arr.ctypes.data, 1, None, -1, -1,
+                             C.addressof(pageable), C.byref(cnt), C.byref(stt)))
+        pageable_times.append(time.perf_counter() - t1)
+    del pageable
     cur_times = []
     for s in sources[:8]:
         st_arr = np.array([s], np.uint64)
@@ -654,10 +662,11 @@ def main():
     legs["bfs_64_sources"]["visit_rows"] = {
         "sources": 8, "rows_mean": int(np.mean(rows)),
         "ms_mean_wg_bfs": round(1e3 * float(np.mean(row_times)), 2),
+        "ms_mean_wg_bfs_pageable_destination": round(1e3 * float(np.mean(pageable_times)), 2),
         "ms_mean_cursor_all_levels": round(1e3 * float(np.mean(cur_times)), 2),
         "mrows_per_s": round(float(np.mean(rows)) / float(np.mean(row_times)) / 1e6, 1)}
-    log(f"bfs rows: {np.mean(rows):.0f} visits in {1e3 * np.mean(row_times):.1f} ms (wg_bfs), "
-        f"{1e3 * np.mean(cur_times):.1f} ms (cursor)")
+    log(f"bfs rows: {np.mean(rows):.0f} visits in {1e3 * np.mean(row_times):.1f} ms (wg_bfs; "
+        f"{1e3 * np.mean(pageable_times):.1f} ms into pageable memory), {1e3 * np.mean(cur_times):.1f} ms (cursor)")
 
     # PageRank: global, damping 0.85, tol 1e-10 (stop at L1 change < n tol),
     # at most 100 iterations, f64, over the unweighted structure.

@@ -97,6 +97,14 @@ int wg_synchronize(wg_context* ctx);
  * element type cannot fail (float products) return without waiting, for
  * back-to-back launches; errors surface at wg_synchronize. */
 void wg_set_blocking(wg_context* ctx, int blocking);
+/* Page-locked host memory for the buffers results are copied into (visit
+ * rows, gathered values, vectors read back): a copy into it is one DMA, where
+ * a copy into ordinary (pageable) memory goes through a staging buffer at a
+ * fraction of the link's rate. No counterpart in the reference, whose
+ * results stay in host memory; the adapter's row buffers are its callers.
+ * Host memory, not charged to the context's budget. */
+int wg_host_alloc(wg_context* ctx, uint64_t bytes, void** out);
+void wg_host_free(void* p);
 /* The context's HIP stream (a hipStream_t), for timing with HIP events. */
 void* wg_stream(wg_context* ctx);
 

@@ -84,6 +84,7 @@ class Context:
         # Vectors and graphs hold device memory of this context: they are
         # released before it is (the C ABI requires children to go first).
         self._children = weakref.WeakSet()
+        self._pinned = None  # (pointer, bytes): page-locked room for results
 
     def check(self, code):
         if code:
@@ -110,10 +111,28 @@ class Context:
     def stream(self):
         return self.lib.wg_stream(self.h)
 
+    def host_array(self, ctype, count):
+        """A ctypes array of count ctype over the context's page-locked
+        buffer (wg_host_alloc, grown as needed and reused: its contents last
+        until the next call), which results are copied into by DMA."""
+        need = max(C.sizeof(ctype) * max(int(count), 1), 1)
+        if self._pinned is None or self._pinned[1] < need:
+            self._free_pinned()
+            ptr = C.c_void_p()
+            self.check(self.lib.wg_host_alloc(self.h, need, C.byref(ptr)))
+            self._pinned = (ptr.value, need)
+        return (ctype * max(int(count), 1)).from_address(self._pinned[0])
+
+    def _free_pinned(self):
+        if self._pinned is not None:
+            self.lib.wg_host_free(self._pinned[0])
+            self._pinned = None
+
     def close(self):
         if getattr(self, "h", None):
             for child in list(self._children):
                 child.close()
+            self._free_pinned()
             self.lib.wg_context_destroy(self.h)
             self.h = None
 
@@ -494,7 +513,7 @@ def bfs(ctx, graph, starts, *, direction="out", relations=None, node_filter=None
     st = _starts(starts)
     flt = _filter(node_filter)
     rel, nrel = _relations(relations)
-    out = (_lib.Visit * max(graph.n, 1))()
+    out = ctx.host_array(_lib.Visit, graph.n)
     cnt = C.c_uint64()
     stats = _lib.TraversalStats()
     ctx.check(ctx.lib.wg_bfs(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st), len(st), _p(flt),
@@ -594,15 +613,15 @@ class BfsCursor:
                                                -1 if max_results is None else max_results,
                                                int(reachable), C.byref(h)))
         self.h = h
-        self._out = (_lib.Visit * max(graph.n, 1))()
         ctx._children.add(self)
 
     def next_level(self):
         cnt = C.c_uint64()
         more = C.c_int()
-        self.ctx.check(self.ctx.lib.wg_bfs_cursor_next(self.ctx.h, self.h, C.addressof(self._out),
+        out = self.ctx.host_array(_lib.Visit, self.graph.n)
+        self.ctx.check(self.ctx.lib.wg_bfs_cursor_next(self.ctx.h, self.h, C.addressof(out),
                                                        C.byref(cnt), C.byref(more)))
-        arr = np.ctypeslib.as_array(self._out)[:cnt.value]
+        arr = np.ctypeslib.as_array(out)[:cnt.value]
         visits = [(int(v["node"]), int(v["depth"]), None if v["via"] == 2**64 - 1 else int(v["via"]))
                   for v in arr]
         return visits, bool(more.value)

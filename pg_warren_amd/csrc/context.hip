@@ -547,6 +547,22 @@ int wg_last_error(const wg_context* ctx, char* buf, size_t len) {
   return ctx->last_status;
 }
 
+int wg_host_alloc(wg_context* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return WG_INVALID_ARGUMENT;
+  *out = nullptr;
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+    *out = nullptr;
+    (void)hipGetLastError();
+    return fail(ctx, WG_OUT_OF_MEMORY, "page-locked host allocation of " + std::to_string(bytes) + " bytes");
+  }
+  return ok(ctx);
+}
+
+void wg_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 void wg_cancel(wg_context* ctx) {
   if (ctx) ctx->cancelled = 1;
 }
