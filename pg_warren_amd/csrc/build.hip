@@ -262,7 +262,15 @@ __global__ __launch_bounds__(kThreads) void differs(const T* __restrict__ a, con
   if (__syncthreads_or(d) && threadIdx.x == 0) atomicOr(flag, 1u);
 }
 
-// A device buffer charged to the context, freed on scope exit.
+// Library temporary storage rounded up to a power of two (at least 64 KB),
+// so that released blocks are reused across calls of varying size.
+inline size_t reusable_size(size_t b) {
+  size_t r = size_t(64) << 10;
+  while (r < b) r <<= 1;
+  return r;
+}
+
+// A device buffer charged to the context, released on scope exit.
 template <class T>
 struct Scratch {
   wg_context* ctx = nullptr;
@@ -286,10 +294,10 @@ int sort_pairs(wg_context* ctx, const K* kin, K* kout, const V* vin, V* vout, ui
   size_t tb = 0;
   WG_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, (size_t)m, 0u, bits, ctx->stream));
   Scratch<char> tmp;
-  if (int s = tmp.alloc(ctx, tb)) return s;
+  if (int s = tmp.alloc(ctx, reusable_size(tb))) return s;
   WG_HIP(ctx, rocprim::radix_sort_pairs((void*)tmp.p, tb, kin, kout, vin, vout, (size_t)m, 0u, bits,
                                         ctx->stream));
-  // The scratch is freed on return, after the sort (hipFree waits for it).
+  // The scratch is released on return, after the sort (dfree waits for it).
   return WG_OK;
 }
 
@@ -298,7 +306,7 @@ int inclusive_count(wg_context* ctx, const V* in, V* out, uint64_t m) {
   size_t tb = 0;
   WG_HIP(ctx, rocprim::inclusive_scan(nullptr, tb, in, out, (size_t)m, rocprim::plus<V>(), ctx->stream));
   Scratch<char> tmp;
-  if (int s = tmp.alloc(ctx, tb)) return s;
+  if (int s = tmp.alloc(ctx, reusable_size(tb))) return s;
   WG_HIP(ctx, rocprim::inclusive_scan((void*)tmp.p, tb, in, out, (size_t)m, rocprim::plus<V>(), ctx->stream));
   return WG_OK;
 }
@@ -554,7 +562,7 @@ int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint6
   WG_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, in, out, (uint64_t)0, (size_t)m, rocprim::plus<uint64_t>(),
                                       ctx->stream));
   Scratch<char> tmp;
-  if (int s = tmp.alloc(ctx, tb)) return s;
+  if (int s = tmp.alloc(ctx, reusable_size(tb))) return s;
   WG_HIP(ctx, rocprim::exclusive_scan((void*)tmp.p, tb, in, out, (uint64_t)0, (size_t)m,
                                       rocprim::plus<uint64_t>(), ctx->stream));
   return WG_OK;

@@ -42,6 +42,16 @@ int hip_fail(wg_context* ctx, hipError_t e, const char* what) {
   return fail(ctx, e == hipErrorOutOfMemory ? WG_OUT_OF_MEMORY : WG_DEVICE_ERROR, buf);
 }
 
+// The block cache: blocks up to kCacheBlock bytes, kCacheTotal in all.
+constexpr size_t kCacheBlock = size_t(512) << 20;
+constexpr size_t kCacheTotal = size_t(4) << 30;
+
+static void flush_cache(wg_context* ctx) {
+  for (auto& b : ctx->cache) (void)hipFree(b.second);
+  ctx->cache.clear();
+  ctx->cache_bytes = 0;
+}
+
 int dalloc(wg_context* ctx, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 8;
@@ -52,7 +62,21 @@ int dalloc(wg_context* ctx, void** p, size_t bytes) {
              (unsigned long long)used, bytes, (unsigned long long)ctx->limit);
     return fail(ctx, WG_MEMORY_LIMIT, buf);
   }
+  auto hit = ctx->cache.find(bytes);
+  if (hit != ctx->cache.end()) {
+    *p = hit->second;
+    ctx->cache_bytes -= bytes;
+    ctx->cache.erase(hit);
+    ctx->in_use += bytes;
+    return WG_OK;
+  }
   hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !ctx->cache.empty()) {
+    // Give the cached blocks back and try once more.
+    (void)hipGetLastError();
+    flush_cache(ctx);
+    e = hipMalloc(p, bytes);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
@@ -64,11 +88,25 @@ int dalloc(wg_context* ctx, void** p, size_t bytes) {
   return WG_OK;
 }
 
+// A freed block is cached once the context's stream has drained (no kernel
+// or copy of it still uses the block; the next owner may write it from the
+// host), else, or when too large, given back.
 void dfree(wg_context* ctx, void* p, size_t bytes) {
   if (!p) return;
   if (bytes == 0) bytes = 8;
-  (void)hipFree(p);
   ctx->in_use -= bytes;
+  if (bytes <= kCacheBlock && hipStreamSynchronize(ctx->stream) == hipSuccess) {
+    while (ctx->cache_bytes + bytes > kCacheTotal && !ctx->cache.empty()) {
+      auto largest = std::prev(ctx->cache.end());
+      (void)hipFree(largest->second);
+      ctx->cache_bytes -= largest->first;
+      ctx->cache.erase(largest);
+    }
+    ctx->cache.emplace(bytes, p);
+    ctx->cache_bytes += bytes;
+    return;
+  }
+  (void)hipFree(p);
 }
 
 int ensure_scratch(wg_context* ctx, size_t bytes) {
@@ -529,6 +567,7 @@ void wg_context_destroy(wg_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  flush_cache(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
   if (ctx->h_err) (void)hipHostFree(ctx->h_err);

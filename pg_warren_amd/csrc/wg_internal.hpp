@@ -100,6 +100,10 @@ struct wg_context {
   unsigned long long* h_mail = nullptr;  // host view
   unsigned long long* d_mail = nullptr;  // device view
   unsigned long long mail_seq = 0;
+  // Freed device blocks kept for reuse, by exact size (hipFree waits for
+  // the whole device; a search frees some twenty buffers).
+  std::multimap<size_t, void*> cache;
+  size_t cache_bytes = 0;
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
