@@ -530,13 +530,13 @@ def reachable_set(ctx, graph, starts, *, direction="out", relations=None, max_de
     """reachable_set (traversal/reachable.rs:48-90): node ids in BFS order."""
     st = _starts(starts)
     rel, nrel = _relations(relations)
-    out = np.zeros(max(graph.n, 1), np.uint64)
+    out = ctx.host_array(C.c_uint64, graph.n)
     cnt = C.c_uint64()
     stats = _lib.TraversalStats()
     ctx.check(ctx.lib.wg_reachable_set(ctx.h, graph.h, DIRECTIONS[direction], _p(rel), nrel, _p(st), len(st),
-                                       -1 if max_depth is None else max_depth, _p(out),
+                                       -1 if max_depth is None else max_depth, C.addressof(out),
                                        C.byref(cnt), C.byref(stats)))
-    return out[:cnt.value].copy(), _stats(stats)
+    return np.ctypeslib.as_array(out)[:cnt.value].astype(np.uint64), _stats(stats)
 
 
 COMBINERS = ["collect", "count", "sum", "min", "max", "first", "concat", "bool_or", "bool_and", "mode"]

@@ -892,6 +892,14 @@ __global__ __launch_bounds__(256) void push_scan_edges(const uint32_t* __restric
   if (threadIdx.x == 0 && sum) atomicAdd(out, sum);
 }
 
+// The level's node ids in emission order (a reachable set's rows).
+__global__ __launch_bounds__(256) void write_ids(const uint32_t* __restrict__ nodes, uint64_t count,
+                                                 uint64_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = nodes[i];
+}
+
 // The level's visits in emission order.
 __global__ __launch_bounds__(256) void write_visits(const uint32_t* __restrict__ nodes,
                                                     const uint32_t* __restrict__ via, uint64_t count,
@@ -926,6 +934,7 @@ struct LevelEmitter {
   wg_visit* visits = nullptr;
   int64_t max_results = -1;
   bool reachable = false, seeded = false, finished = false, truncated = false;
+  bool ids_only = false;  // rows are node ids (u64), not wg_visit
   uint64_t emitted = 0;
   uint64_t push_edges = 0;  // stats of a push, outside a reachable set
 
@@ -1001,9 +1010,10 @@ struct LevelEmitter {
     return WG_OK;
   }
 
-  // The next level's visits into out (host, room for the graph's n);
-  // *count of them. Returns with *more = 0 once nothing follows.
-  int next(wg_visit* out, uint64_t* count, int* more) {
+  // The next level's visits into out (host, room for the graph's n: visits,
+  // or node ids when ids_only); *count of them. Returns with *more = 0 once
+  // nothing follows.
+  int next(void* out, uint64_t* count, int* more) {
     *count = 0;
     if (finished) {
       *more = 0;
@@ -1015,7 +1025,11 @@ struct LevelEmitter {
       seeded = true;
       size = starts.size();
       const uint64_t k = std::min<uint64_t>(size, room());
-      for (uint64_t i = 0; i < k; ++i) out[i] = wg_visit{starts[i], UINT64_MAX, 0, 0};
+      for (uint64_t i = 0; i < k; ++i)
+        if (ids_only)
+          ((uint64_t*)out)[i] = starts[i];
+        else
+          ((wg_visit*)out)[i] = wg_visit{starts[i], UINT64_MAX, 0, 0};
       *count = k;
     } else {
       const uint64_t level_entries = S.mf;
@@ -1030,9 +1044,13 @@ struct LevelEmitter {
         if ((s = order_level(size, &nodes))) return s;
         if (k) {
           const unsigned grid = (unsigned)std::min<uint64_t>((k + 255) / 256, 4096);
-          write_visits<<<grid, 256, 0, ctx->stream>>>(nodes, via, k, S.d, visits);
+          const size_t row = ids_only ? sizeof(uint64_t) : sizeof(wg_visit);
+          if (ids_only)
+            write_ids<<<grid, 256, 0, ctx->stream>>>(nodes, k, (uint64_t*)visits);
+          else
+            write_visits<<<grid, 256, 0, ctx->stream>>>(nodes, via, k, S.d, visits);
           WG_HIP(ctx, hipGetLastError());
-          WG_HIP(ctx, hipMemcpyAsync(out, visits, k * sizeof(wg_visit), hipMemcpyDeviceToHost, ctx->stream));
+          WG_HIP(ctx, hipMemcpyAsync(out, visits, k * row, hipMemcpyDeviceToHost, ctx->stream));
         }
         if (cut && !reachable) {
           // The push stops at the refused visit: its level counts the
@@ -1109,12 +1127,13 @@ int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint
 }
 
 // Every level of an emitter into out.
-static int emit_all(wg_context* ctx, LevelEmitter& e, wg_visit* out, uint64_t* count) {
+static int emit_all(wg_context* ctx, LevelEmitter& e, void* out, uint64_t* count) {
   uint64_t total = 0;
   int more = 1;
   while (more) {
     uint64_t k = 0;
-    int s = e.next(out + total, &k, &more);
+    const size_t row = e.ids_only ? sizeof(uint64_t) : sizeof(wg_visit);
+    int s = e.next((char*)out + total * row, &k, &more);
     if (s) return s;
     total += k;
   }
@@ -1142,12 +1161,11 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const u
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   LevelEmitter e;
-  std::vector<wg_visit> v(std::max<uint64_t>(gc->n, 1));
   int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, relations, nrelations, starts, nstarts, nullptr,
                  max_depth, -1, true, true);
-  if (!s) s = emit_all(ctx, e, v.data(), count);
+  e.ids_only = true;
+  if (!s) s = emit_all(ctx, e, out, count);
   if (s) return s;
-  for (uint64_t i = 0; i < *count; ++i) out[i] = v[i].node;
   if (stats) *stats = e.stats();
   return ok(ctx);
 }
