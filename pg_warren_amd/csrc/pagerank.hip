@@ -154,20 +154,28 @@ __global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const doub
   }
   const uint32_t cnt = (uint32_t)(e1 - e0);
   {
-    uint32_t col[kBlockNnz / kBlockThreads];
-    double wv[kBlockNnz / kBlockThreads];
+    // All the block's message gathers issued before the first use: a lane
+    // past the entries reads message 0 (which exists) and drops it.
+    constexpr int kPer = (int)(kBlockNnz / kBlockThreads);
+    uint32_t col[kPer];
+    double wv[kPer];
 #pragma unroll
-    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+    for (int k = 0; k < kPer; ++k) {
       const uint32_t i = tid + k * kBlockThreads;
+      col[k] = 0;
+      wv[k] = 1.0;
       if (i < cnt) {
         col[k] = a.src[e0 + i];
         if (HW) wv[k] = a.w[e0 + i];
       }
     }
+    double mv[kPer];
 #pragma unroll
-    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+    for (int k = 0; k < kPer; ++k) mv[k] = msg_in[col[k]];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
       const uint32_t i = tid + k * kBlockThreads;
-      if (i < cnt) prod[i] = HW ? wv[k] * msg_in[col[k]] : msg_in[col[k]];
+      if (i < cnt) prod[i] = HW ? wv[k] * mv[k] : mv[k];
     }
   }
   if (b.nrows)

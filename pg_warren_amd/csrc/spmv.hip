@@ -131,28 +131,34 @@ __device__ __forceinline__ void block_body(const Block& b, const uint64_t* __res
   }
   const uint32_t cnt = (uint32_t)(e1 - e0);
 
-  // Stage the products: every load of the block issued before any use.
+  // Stage the products: every load of the block issued before any use. The
+  // gathers are unconditional (a lane past the block's entries reads column
+  // 0, which exists, and drops it), so the compiler issues all of them
+  // before the first wait instead of one per guarded branch.
   {
-    uint32_t col[kBlockNnz / kBlockThreads];
-    E wv[kBlockNnz / kBlockThreads];
+    constexpr int kPer = (int)(kBlockNnz / kBlockThreads);
+    uint32_t col[kPer];
+    E wv[kPer];
 #pragma unroll
-    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+    for (int k = 0; k < kPer; ++k) {
       const uint32_t i = tid + k * kBlockThreads;
+      col[k] = 0;
+      wv[k] = E(1);
       if (i < cnt) {
         if constexpr (reads_x<MUL>()) col[k] = ld_stream(tgt + e0 + i, nt);
         if constexpr (HW) wv[k] = ld_stream(w + e0 + i, nt);
       }
     }
+    E xv[kPer];
 #pragma unroll
-    for (int k = 0; k < (int)(kBlockNnz / kBlockThreads); ++k) {
+    for (int k = 0; k < kPer; ++k) {
+      xv[k] = E();
+      if constexpr (reads_x<MUL>()) xv[k] = gather(col[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
       const uint32_t i = tid + k * kBlockThreads;
-      if (i < cnt) {
-        E wk = E(1);
-        if constexpr (HW) wk = wv[k];
-        E xv = E();
-        if constexpr (reads_x<MUL>()) xv = gather(col[k]);
-        prod[i] = multiply<E, ADD, MUL>(wk, xv, e_err);
-      }
+      if (i < cnt) prod[i] = multiply<E, ADD, MUL>(wv[k], xv[k], e_err);
     }
   }
   if (b.nrows)
@@ -328,23 +334,29 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_direct(
   auto fold_strided = [&](uint64_t i, uint64_t end, uint32_t step, A& acc, int& pres) {
     for (; i < end; i += 4 * (uint64_t)step) {
       uint32_t c[4];
-      E wk[4];
+      E wk[4], xk[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint64_t j = i + (uint64_t)k * step;
+        c[k] = 0;
+        wk[k] = E(1);
         if (j < end) {
           if constexpr (reads_x<MUL>()) c[k] = ld_stream(tgt + j, nt);
           if constexpr (HW) wk[k] = ld_stream(w + j, nt);
         }
       }
+      // The four gathers unconditional (column 0 past the end), so they
+      // are in flight together.
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        xk[k] = E();
+        if constexpr (reads_x<MUL>()) xk[k] = x[c[k]];
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint64_t j = i + (uint64_t)k * step;
         if (j < end) {
-          E wv = E(1);
-          if constexpr (HW) wv = wk[k];
-          E xv = E();
-          if constexpr (reads_x<MUL>()) xv = x[c[k]];
+          const E wv = wk[k], xv = xk[k];
           A v = F::lift(multiply<E, ADD, MUL>(wv, xv, e_err));
           acc = pres ? F::combine(acc, v, e_err) : v;
           pres = 1;
