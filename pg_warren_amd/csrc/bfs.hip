@@ -95,18 +95,30 @@ __global__ __launch_bounds__(256) void bfs_seed(const uint32_t* __restrict__ sta
   }
 }
 
-// A push's target v of frontier node u: an unvisited target admitted by
-// the filter becomes a candidate (a plain byte store: every writer writes 1).
-// kMin: it also takes the smallest frontier node with an edge to it as its
-// via (atomicMin: the first in ascending order, as the reference's push
-// reaches it).
+// Entries i, i + step, ... below b of frontier node u's row, pushed four
+// at a time: the targets' loads, then their visited words' loads, go out
+// together (a lane past the end reads node 0's word and drops it). An
+// unvisited target admitted by the filter becomes a candidate (a plain
+// byte store: every writer writes 1); kMin: it also takes the smallest
+// frontier node with an edge to it as its via (atomicMin: the first in
+// ascending order, as the reference's push reaches it).
 template <bool kMin>
-__device__ __forceinline__ void push_target(uint32_t v, uint32_t u, const uint32_t* __restrict__ visited,
-                                            const uint8_t* __restrict__ filter, uint8_t* __restrict__ cand,
-                                            uint32_t* __restrict__ via) {
-  if (!bit_of(visited, v) && !(filter && !filter[v])) {
-    if (kMin) atomicMin(&via[v], u);
-    cand[v] = 1;
+__device__ __forceinline__ void push_run(uint64_t i, uint64_t b, uint32_t step, uint32_t u,
+                                         const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ visited,
+                                         const uint8_t* __restrict__ filter, uint8_t* __restrict__ cand,
+                                         uint32_t* __restrict__ via) {
+  for (; i < b; i += 4 * (uint64_t)step) {
+    uint32_t v[4], word[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i + k * step < b ? tgt[i + k * step] : 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) word[k] = visited[v[k] >> 5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k * step < b && !((word[k] >> (v[k] & 31)) & 1u) && !(filter && !filter[v[k]])) {
+        if (kMin) atomicMin(&via[v[k]], u);
+        cand[v[k]] = 1;
+      }
   }
 }
 
@@ -134,7 +146,7 @@ __global__ __launch_bounds__(256) void bfs_push(const uint32_t* __restrict__ que
     if (!bit_of(frontier, u)) return;
     const uint64_t a = off[u] + (uint64_t)pc.y * kBlockNnz;
     const uint64_t b = min(a + kBlockNnz, off[u + 1]);
-    for (uint64_t i = a + threadIdx.x; i < b; i += 256) push_target<kMin>(tgt[i], u, visited, filter, cand, via);
+    push_run<kMin>(a + threadIdx.x, b, 256, u, tgt, visited, filter, cand, via);
     return;
   }
   const uint32_t lane = threadIdx.x & 63;
@@ -144,7 +156,7 @@ __global__ __launch_bounds__(256) void bfs_push(const uint32_t* __restrict__ que
     const uint32_t u = queue[q];
     const uint64_t a = off[u], b = off[u + 1];
     if (b - a > kBlockNnz) continue;  // a split row: its pieces' blocks
-    for (uint64_t i = a + lane; i < b; i += 64) push_target<kMin>(tgt[i], u, visited, filter, cand, via);
+    push_run<kMin>(a + lane, b, 64, u, tgt, visited, filter, cand, via);
   }
 }
 
