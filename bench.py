@@ -569,6 +569,24 @@ def main():
                       "gbs": round(byt / ms / 1e6, 1), "bytes": byt}
         log(f"{name}: {ms * 1e3:.1f} us  {m / ms / 1e6:.1f} GTEPS  {byt / ms / 1e6:.0f} GB/s")
 
+    # SpMM: plus_times over 16 f32 columns (x and y n x 16, row-major).
+    width = 16
+    xm = W.Vector(ctx, np.random.default_rng(7).random((n, width)).astype(np.float32))
+    ym = W.Vector(ctx, dtype="f32", id_space=n, width=width)
+    pm = W.Product("plus_times").c()
+    run_mm = lambda: ctx.check(lib.wg_spmm(ctx.h, gf.h, C.byref(pm), xm.h, ym.h))
+    run_mm()
+    ctx.synchronize()
+    mm_ms = timed(timer, run_mm, max(3, args.steps // 10))
+    ctx.synchronize()
+    mm_bytes = (n + 1) * 8 + m * 8 + n * width * 4 * 2
+    legs["spmm_plus_times_f32_w16"] = {"ms": round(mm_ms, 3), "gteps": round(m * width / mm_ms / 1e6, 1),
+                                       "gbs": round(mm_bytes / mm_ms / 1e6, 1), "bytes": mm_bytes}
+    log(f"spmm w16: {mm_ms:.2f} ms  {m * width / mm_ms / 1e6:.0f} G entry-columns/s  "
+        f"{mm_bytes / mm_ms / 1e6:.0f} GB/s")
+    xm.close()
+    ym.close()
+
     # Achievable stream bandwidth (a plain copy kernel), for context.
     nbytes = 1 << 30
     a = W.Vector(ctx, dtype="u32", id_space=nbytes // 4)

@@ -460,14 +460,32 @@ __global__ __launch_bounds__(256) void spmm_rows(
   for (uint64_t c = lane; c < width; c += lpr) {
     A acc{};
     int pres = 0;
-    for (uint64_t e = e0; e < e1; ++e) {
-      E wk = E(1);
-      if constexpr (HW) wk = w[e];
-      E xv = E();
-      if constexpr (reads_x<MUL>()) xv = x[(uint64_t)tgt[e] * width + c];
-      A v = F::lift(multiply<E, ADD, MUL>(wk, xv, e_err));
-      acc = pres ? F::combine(acc, v, e_err) : v;
-      pres = 1;
+    // Four entries at a time, their loads in flight together (a slot past
+    // the end reads row 0's column c and drops it); folded in order.
+    for (uint64_t e = e0; e < e1; e += 4) {
+      uint32_t t4[4];
+      E wk[4], xv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        t4[k] = 0;
+        wk[k] = E(1);
+        if (e + k < e1) {
+          if constexpr (reads_x<MUL>()) t4[k] = tgt[e + k];
+          if constexpr (HW) wk[k] = w[e + k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        xv[k] = E();
+        if constexpr (reads_x<MUL>()) xv[k] = x[(uint64_t)t4[k] * width + c];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (e + k < e1) {
+          A v = F::lift(multiply<E, ADD, MUL>(wk[k], xv[k], e_err));
+          acc = pres ? F::combine(acc, v, e_err) : v;
+          pres = 1;
+        }
     }
     if (np == 1) {
       E res = pres ? F::finish(acc, e_err) : E();
