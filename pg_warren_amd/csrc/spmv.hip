@@ -460,13 +460,14 @@ __global__ __launch_bounds__(256) void spmm_rows(
   for (uint64_t c = lane; c < width; c += lpr) {
     A acc{};
     int pres = 0;
-    // Four entries at a time, their loads in flight together (a slot past
+    // Eight entries at a time, their loads in flight together (a slot past
     // the end reads row 0's column c and drops it); folded in order.
-    for (uint64_t e = e0; e < e1; e += 4) {
-      uint32_t t4[4];
-      E wk[4], xv[4];
+    constexpr int kU = 8;
+    for (uint64_t e = e0; e < e1; e += kU) {
+      uint32_t t4[kU];
+      E wk[kU], xv[kU];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kU; ++k) {
         t4[k] = 0;
         wk[k] = E(1);
         if (e + k < e1) {
@@ -475,12 +476,12 @@ __global__ __launch_bounds__(256) void spmm_rows(
         }
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kU; ++k) {
         xv[k] = E();
         if constexpr (reads_x<MUL>()) xv[k] = x[(uint64_t)t4[k] * width + c];
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < kU; ++k)
         if (e + k < e1) {
           A v = F::lift(multiply<E, ADD, MUL>(wk[k], xv[k], e_err));
           acc = pres ? F::combine(acc, v, e_err) : v;

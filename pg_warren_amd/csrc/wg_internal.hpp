@@ -79,7 +79,7 @@ struct DevCsr {
   std::map<int, std::pair<uint32_t*, uint32_t>> lds_tgt;
 };
 
-constexpr uint32_t kSpmmPiece = 2048;
+constexpr uint32_t kSpmmPiece = 256;
 
 }  // namespace wg
 
