@@ -107,12 +107,28 @@ __device__ __forceinline__ bool selected(const Receiving& R, uint64_t r) {
 __global__ __launch_bounds__(kThreads) void entry_counts(const uint32_t* __restrict__ stgt, uint64_t m,
                                                          const uint64_t* __restrict__ moff,
                                                          uint64_t* __restrict__ ecnt) {
-  for (uint64_t p = first_index(); p <= m; p += stride()) {
-    if (p == m) {
-      ecnt[p] = 0;
-    } else {
-      const uint32_t s = stgt[p];
-      ecnt[p] = moff[s + 1] - moff[s];
+  if (m == 0) {  // no entry: only the scan's closing zero (moff may hold one offset)
+    if (first_index() == 0) ecnt[0] = 0;
+    return;
+  }
+  // Four entries a step (p, p + stride, ...), their loads in flight together
+  // (a slot past the end reads sender 0's offsets and drops them).
+  const uint64_t st = stride();
+  for (uint64_t p = first_index(); p <= m; p += 4 * st) {
+    uint32_t s4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s4[k] = p + k * st < m ? stgt[p + k * st] : 0;
+    uint64_t a4[4], b4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a4[k] = moff[s4[k]];
+      b4[k] = moff[s4[k] + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t q = p + k * st;
+      if (q < m) ecnt[q] = b4[k] - a4[k];
+      else if (q == m) ecnt[q] = 0;
     }
   }
 }
@@ -160,14 +176,35 @@ __global__ __launch_bounds__(kThreads) void fill_entries(Receiving R, uint64_t m
                                                          const uint64_t* __restrict__ start,
                                                          const uint64_t* __restrict__ rank, E* __restrict__ out,
                                                          uint32_t* __restrict__ ord) {
-  for (uint64_t p = first_index(); p < m; p += stride()) {
-    const uint32_t r = row_of[p];
-    if (!selected(R, r)) continue;
-    const uint32_t s = R.stgt[p];
-    uint64_t at = start[r] + (R.self ? R.moff[r + 1] - R.moff[r] : 0) + eprefix[p] - eprefix[R.soff[r]];
-    for (uint64_t k = R.moff[s]; k < R.moff[s + 1]; ++k, ++at) {
-      out[at] = mval[k];
-      if (ord) ord[at] = (uint32_t)rank[r];
+  // Four entries a step (p, p + stride, ...): each stage's loads go out
+  // together (a slot past the end reads entry 0's and drops it).
+  const uint64_t st = stride();
+  for (uint64_t p = first_index(); p < m; p += 4 * st) {
+    uint32_t r4[4], s4[4];
+    uint64_t e4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t q = p + k * st < m ? p + k * st : 0;
+      r4[k] = row_of[q];
+      s4[k] = R.stgt[q];
+      e4[k] = eprefix[q];
+    }
+    uint64_t at4[4], m0[4], m1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t r = r4[k];
+      at4[k] = start[r] + (R.self ? R.moff[r + 1] - R.moff[r] : 0) + e4[k] - eprefix[R.soff[r]];
+      m0[k] = R.moff[s4[k]];
+      m1[k] = R.moff[s4[k] + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (p + k * st >= m || !selected(R, r4[k])) continue;
+      uint64_t at = at4[k];
+      for (uint64_t j = m0[k]; j < m1[k]; ++j, ++at) {
+        out[at] = mval[j];
+        if (ord) ord[at] = (uint32_t)rank[r4[k]];
+      }
     }
   }
 }
