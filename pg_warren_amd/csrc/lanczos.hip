@@ -611,78 +611,77 @@ static std::vector<int> ritz_order(const std::vector<double>& values, int which,
 
 // ---- host: components (components.rs) --------------------------------------------
 
+// ---- components on the device ------------------------------------------------
+// The components of the active nodes over the forward CSR's edges (either
+// direction connects), numbered by their lowest node, each node's colour in a
+// two-colouring (the lowest node false) and whether the component is
+// bipartite: labels start as the node ids and take the smallest label of a
+// neighbour (the library's own u32 min-second products over A and A^T,
+// accumulated by min) and then their label's label, until a pass changes
+// nothing; a breadth-first search from the lowest nodes over both directions
+// gives the depths, whose parity is the colour; an edge whose ends share a
+// colour (or a self loop) makes its component odd. The colour matters only
+// in bipartite components, where it is the unique two-colouring.
+
+__global__ void cc_iota(uint64_t n, uint32_t* __restrict__ label) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+    label[v] = (uint32_t)v;
+}
+
+// y[v] = y[y[v]] (one jump; labels only fall, so racing reads are safe) and
+// *changed = 1 where y differs from x.
+__global__ void cc_jump(uint64_t n, const uint32_t* __restrict__ x, uint32_t* y, unsigned* __restrict__ changed) {
+  unsigned c = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t l = y[v];
+    const uint32_t ll = y[l];
+    if (ll < l) {
+      l = ll;
+      y[v] = l;
+    }
+    c |= l != x[v];
+  }
+  if (__syncthreads_or(c) && threadIdx.x == 0) atomicOr(changed, 1u);
+}
+
+// Roots (active nodes that are their own label) flagged for the scan.
+__global__ void cc_roots(uint64_t n, const uint8_t* __restrict__ active, const uint32_t* __restrict__ label,
+                         uint64_t* __restrict__ flag) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+    flag[v] = active[v] && label[v] == v;
+}
+
+// comp[v] = the number of v's root; roots[number] = the root; colour bytes
+// from the depths; p0 / p1 mark the active nodes of colour 0 / 1.
+__global__ void cc_number(uint64_t n, const uint8_t* __restrict__ active, const uint32_t* __restrict__ label,
+                          const uint64_t* __restrict__ number, const uint32_t* __restrict__ depth,
+                          uint32_t* __restrict__ comp, uint32_t* __restrict__ roots, uint8_t* __restrict__ colour,
+                          uint8_t* __restrict__ p0, uint8_t* __restrict__ p1) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+    const bool act = active[v] != 0;
+    comp[v] = act ? (uint32_t)number[label[v]] : 0;
+    if (act && label[v] == v) roots[number[v]] = (uint32_t)v;
+    const uint8_t c = depth ? (uint8_t)((depth[v] != 0xffffffffu) && (depth[v] & 1u)) : 0;
+    colour[v] = act ? c : 0;
+    p0[v] = act && !c;
+    p1[v] = act && c;
+  }
+}
+
+// odd[comp[v]] = 1 where active v has a neighbour of its own colour.
+__global__ void cc_odd(uint64_t n, const uint8_t* __restrict__ active, const uint32_t* __restrict__ comp,
+                       const uint8_t* __restrict__ colour, const uint8_t* __restrict__ y0,
+                       const uint8_t* __restrict__ y1, uint8_t* __restrict__ odd) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+    if (active[v] && (colour[v] ? y1[v] : y0[v])) odd[comp[v]] = 1;
+}
+
 struct Components {
   std::vector<uint32_t> comp;   // component of each active node (0 elsewhere)
   std::vector<uint8_t> colour;  // two-colouring, the lowest node false
   std::vector<uint8_t> bipartite;
   uint32_t count = 0;
 };
-
-// Union-find with parity over the forward CSR's edges between active nodes
-// (either direction connects), roots the lowest node of each tree,
-// components numbered by their lowest node.
-static Components find_components(const std::vector<uint64_t>& off, const std::vector<uint32_t>& tgt,
-                                  const std::vector<uint8_t>& active) {
-  const uint64_t n = active.size();
-  std::vector<uint32_t> parent(n);
-  std::vector<uint8_t> parity(n, 0), odd(n, 0);
-  std::iota(parent.begin(), parent.end(), 0u);
-  auto find = [&](uint32_t v, bool& colour) {
-    // Colour of v relative to its root, halving the path on the way.
-    uint32_t r = v;
-    bool c = false;
-    while (parent[r] != r) {
-      c ^= parity[r];
-      r = parent[r];
-    }
-    colour = c;
-    // Point every node on the path at the root with its own parity.
-    bool cv = c;
-    uint32_t x = v;
-    while (parent[x] != x) {
-      const uint32_t up = parent[x];
-      const bool px = parity[x];
-      parent[x] = r;
-      parity[x] = cv;
-      cv ^= px;
-      x = up;
-    }
-    return r;
-  };
-  for (uint64_t s = 0; s < n; ++s) {
-    if (!active[s]) continue;
-    for (uint64_t p = off[s]; p < off[s + 1]; ++p) {
-      const uint32_t t = tgt[p];
-      if (!active[t]) continue;
-      bool ca, cb;
-      const uint32_t a = find((uint32_t)s, ca), b = find(t, cb);
-      if (a == b) {
-        odd[a] |= (ca == cb);
-        continue;
-      }
-      const uint32_t low = std::min(a, b), high = std::max(a, b);
-      parent[high] = low;
-      parity[high] = (ca == cb);
-      odd[low] |= odd[high];
-    }
-  }
-  Components c;
-  c.comp.assign(n, 0);
-  c.colour.assign(n, 0);
-  std::vector<uint32_t> number(n, ~0u);
-  for (uint64_t v = 0; v < n; ++v) {
-    if (!active[v]) continue;
-    bool col;
-    const uint32_t r = find((uint32_t)v, col);
-    if (number[r] == ~0u) {
-      number[r] = c.count++;
-      c.bipartite.push_back(!odd[r]);
-    }
-    c.comp[v] = number[r];
-    c.colour[v] = col;
-  }
-  return c;
-}
 
 // ---- host: the solver -----------------------------------------------------------------
 
@@ -841,11 +840,96 @@ struct Eigen {
     return WG_OK;
   }
 
+  // Components (deflation.rs, components.rs) on the device: see cc_iota.
   int prepare_components() {
-    std::vector<uint32_t> tgt(g->nnz);
+    comps = Components{};
+    comps.comp.assign(n, 0);
+    comps.colour.assign(n, 0);
+    if (n == 0) return WG_OK;
+    uint32_t *lx, *ly, *comp, *roots, *dep, *via;
+    uint64_t *flag, *number;
+    uint8_t *colour, *p0, *p1, *y0, *y1, *odd;
+    unsigned* changed;
+    int s;
+    if ((s = get(&lx, n)) || (s = get(&ly, n)) || (s = get(&comp, n)) || (s = get(&roots, n)) ||
+        (s = get(&dep, n)) || (s = get(&via, n)) || (s = get(&flag, n)) || (s = get(&number, n)) ||
+        (s = get(&colour, n)) || (s = get(&p0, n)) || (s = get(&p1, n)) || (s = get(&y0, n)) ||
+        (s = get(&y1, n)) || (s = get(&odd, n)) || (s = get(&changed, 1)))
+      return s;
+    cc_iota<<<L.grid(n), 256, 0, st>>>(n, lx);
+    WG_HIP(ctx, hipGetLastError());
+    wg_product pm{};
+    pm.semiring = WG_MIN_SECOND;
+    pm.weighted = 0;
+    pm.accumulate = WG_MIN;
+    // The plain launch: these products need no tuning.
+    const int saved_variant = ctx->spmv_variant;
+    ctx->spmv_variant = 0;
+    for (;;) {
+      wg_vector X{ctx, WG_U32, n, 1, lx, n * 4}, Y{ctx, WG_U32, n, 1, ly, n * 4};
+      WG_HIP(ctx, hipMemcpyAsync(ly, lx, n * 4, hipMemcpyDeviceToDevice, st));
+      pm.transpose = 0;
+      if ((s = wg_spmv(ctx, g, &pm, &X, &Y))) break;
+      if (!g->rev_is_fwd) {
+        pm.transpose = 1;
+        if ((s = wg_spmv(ctx, g, &pm, &X, &Y))) break;
+      }
+      WG_HIP(ctx, hipMemsetAsync(changed, 0, 4, st));
+      cc_jump<<<L.grid(n), 256, 0, st>>>(n, lx, ly, changed);
+      WG_HIP(ctx, hipGetLastError());
+      unsigned h = 0;
+      WG_HIP(ctx, hipMemcpyAsync(&h, changed, 4, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      std::swap(lx, ly);
+      if (!h) break;
+    }
+    ctx->spmv_variant = saved_variant;
+    if (s) return s;
+    // Number the roots in node order; list them.
+    cc_roots<<<L.grid(n), 256, 0, st>>>(n, active, lx, flag);
+    WG_HIP(ctx, hipGetLastError());
+    if ((s = exclusive_scan_u64(ctx, flag, number, n))) return s;
+    uint64_t last[2] = {0, 0};
+    WG_HIP(ctx, hipMemcpyAsync(&last[0], number + n - 1, 8, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipMemcpyAsync(&last[1], flag + n - 1, 8, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipStreamSynchronize(st));
-    if (g->nnz) WG_HIP(ctx, hipMemcpy(tgt.data(), g->fwd.targets, g->nnz * 4, hipMemcpyDeviceToHost));
-    comps = find_components(g->fwd.h_offsets, tgt, h_active);
+    comps.count = (uint32_t)(last[0] + last[1]);
+    cc_number<<<L.grid(n), 256, 0, st>>>(n, active, lx, number, nullptr, comp, roots, colour, p0, p1);
+    WG_HIP(ctx, hipGetLastError());
+    if (comps.count) {
+      // Depths from the roots over both directions: the colours.
+      std::vector<uint32_t> r32(comps.count);
+      WG_HIP(ctx, hipMemcpyAsync(r32.data(), roots, comps.count * 4, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      std::vector<uint64_t> starts(r32.begin(), r32.end());
+      wg_vector D{ctx, WG_U32, n, 1, dep, n * 4}, V{ctx, WG_U32, n, 1, via, n * 4};
+      if ((s = wg_bfs_levels(ctx, g, WG_BOTH, nullptr, 0, starts.data(), starts.size(), nullptr, -1, 1, &D, &V,
+                             nullptr)))
+        return s;
+      cc_number<<<L.grid(n), 256, 0, st>>>(n, active, lx, number, dep, comp, roots, colour, p0, p1);
+      WG_HIP(ctx, hipGetLastError());
+      // A neighbour of each colour, then odd components.
+      wg_product po{};
+      po.semiring = WG_LOR_LAND;
+      po.weighted = 0;
+      po.accumulate = WG_OP_NONE;
+      wg_vector P0{ctx, WG_BOOL, n, 1, p0, n}, P1{ctx, WG_BOOL, n, 1, p1, n};
+      wg_vector Y0{ctx, WG_BOOL, n, 1, y0, n}, Y1{ctx, WG_BOOL, n, 1, y1, n};
+      ctx->spmv_variant = 0;
+      s = wg_spmv(ctx, g, &po, &P0, &Y0);
+      if (!s) s = wg_spmv(ctx, g, &po, &P1, &Y1);
+      ctx->spmv_variant = saved_variant;
+      if (s) return s;
+      WG_HIP(ctx, hipMemsetAsync(odd, 0, comps.count, st));
+      cc_odd<<<L.grid(n), 256, 0, st>>>(n, active, comp, colour, y0, y1, odd);
+      WG_HIP(ctx, hipGetLastError());
+    }
+    comps.bipartite.assign(comps.count, 0);
+    WG_HIP(ctx, hipMemcpyAsync(comps.comp.data(), comp, n * 4, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipMemcpyAsync(comps.colour.data(), colour, n, hipMemcpyDeviceToHost, st));
+    if (comps.count) WG_HIP(ctx, hipMemcpyAsync(comps.bipartite.data(), odd, comps.count, hipMemcpyDeviceToHost, st));
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    for (auto& b : comps.bipartite) b = !b;
     return WG_OK;
   }
 
