@@ -346,8 +346,7 @@ void free_csr(wg_context* ctx, DevCsr& c) {
   }
   for (auto& t : c.lds_tgt) dfree(ctx, t.second.first, std::max<uint64_t>(c.nnz, 1) * 4);
   if (c.spmm_ready) {
-    dfree(ctx, c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8);
-    dfree(ctx, c.spmm_ticket_of_row, std::max<uint64_t>(c.n, 1) * 4);
+    dfree(ctx, c.spmm_tasks, std::max<uint64_t>(c.spmm_ntasks, 1) * sizeof(SpmmTask));
     dfree(ctx, c.spmm_pbase, (c.spmm_nlong + 1) * 4);
     dfree(ctx, c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4);
   }
@@ -376,34 +375,54 @@ int graph_buffer(wg_context* ctx, wg_graph* g, int slot, size_t bytes, void** ou
 
 int prepare_spmm(wg_context* ctx, DevCsr& c) {
   if (c.spmm_ready) return WG_OK;
-  std::vector<uint32_t> extra, ticket_of_row(c.n, ~0u), pbase, lrows;
+  // Tasks by entry count: a counting pass, then each task placed in its
+  // count's run, counts descending, rows and pieces ascending within one.
+  std::vector<uint64_t> at(kSpmmPiece + 2, 0);
+  std::vector<uint32_t> pbase, lrows;
+  auto piece_count = [](uint64_t deg, uint32_t p) {
+    return (uint32_t)std::min<uint64_t>(kSpmmPiece, deg - (uint64_t)p * kSpmmPiece);
+  };
+  uint64_t ntasks = 0;
+  for (uint64_t r = 0; r < c.n; ++r) {
+    const uint64_t deg = c.h_offsets[r + 1] - c.h_offsets[r];
+    const uint32_t np = deg > kSpmmPiece ? (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece) : 1;
+    for (uint32_t p = 0; p < np; ++p) ++at[np > 1 ? piece_count(deg, p) : deg];
+    ntasks += np;
+  }
+  for (uint64_t k = kSpmmPiece + 1, sum = 0; k-- > 0;) {
+    const uint64_t cnt = at[k];
+    at[k] = sum;
+    sum += cnt;
+  }
+  std::vector<SpmmTask> tasks(ntasks);
   uint32_t np_total = 0;
   for (uint64_t r = 0; r < c.n; ++r) {
-    uint64_t deg = c.h_offsets[r + 1] - c.h_offsets[r];
-    if (deg <= kSpmmPiece) continue;
-    uint32_t np = (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece);
-    ticket_of_row[r] = (uint32_t)pbase.size();
+    const uint64_t rb = c.h_offsets[r], deg = c.h_offsets[r + 1] - rb;
+    if (deg <= kSpmmPiece) {
+      tasks[at[deg]++] = SpmmTask{rb | deg << kSpmmCountShift, (uint32_t)r, ~0u};
+      continue;
+    }
+    const uint32_t np = (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece);
     pbase.push_back(np_total);
     lrows.push_back((uint32_t)r);
-    np_total += np;
-    for (uint32_t p = 1; p < np; ++p) {
-      extra.push_back((uint32_t)r);
-      extra.push_back(p);
+    for (uint32_t p = 0; p < np; ++p) {
+      const uint32_t cnt = piece_count(deg, p);
+      tasks[at[cnt]++] = SpmmTask{(rb + (uint64_t)p * kSpmmPiece) | (uint64_t)cnt << kSpmmCountShift,
+                                  (uint32_t)r, np_total + p};
     }
+    np_total += np;
   }
   pbase.push_back(np_total);
-  c.spmm_nextra = (uint32_t)(extra.size() / 2);
+  c.spmm_ntasks = ntasks;
   c.spmm_nlong = (uint32_t)lrows.size();
   c.spmm_npartials = np_total;
   int s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_extra, std::max<uint32_t>(c.spmm_nextra, 1) * 8))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_ticket_of_row, std::max<uint64_t>(c.n, 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_tasks, std::max<uint64_t>(ntasks, 1) * sizeof(SpmmTask)))) return s;
   if ((s = dalloc(ctx, (void**)&c.spmm_pbase, (c.spmm_nlong + 1) * 4))) return s;
   if ((s = dalloc(ctx, (void**)&c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
-  if (!extra.empty())
-    WG_HIP(ctx, hipMemcpyAsync(c.spmm_extra, extra.data(), extra.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  if (c.n)
-    WG_HIP(ctx, hipMemcpyAsync(c.spmm_ticket_of_row, ticket_of_row.data(), c.n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (ntasks)
+    WG_HIP(ctx, hipMemcpyAsync(c.spmm_tasks, tasks.data(), ntasks * sizeof(SpmmTask), hipMemcpyHostToDevice,
+                               ctx->stream));
   WG_HIP(ctx, hipMemcpyAsync(c.spmm_pbase, pbase.data(), pbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   if (!lrows.empty())
     WG_HIP(ctx, hipMemcpyAsync(c.spmm_long_rows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice, ctx->stream));

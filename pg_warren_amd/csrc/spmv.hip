@@ -426,46 +426,43 @@ __global__ __launch_bounds__(kBlockThreads) void spmv_direct(
   if (e_err) raise_error(err, b.row0, e_err);
 }
 
-// SpMM: LPR lanes per row task, each lane folding columns lane, lane + LPR,
-// ... of its row over the task's entries.
-template <class E, int ADD, int MUL, bool HW>
+// C consecutive elements of a row of x, read as one 16-byte access when
+// C * sizeof(E) == 16 (the caller has checked the alignment).
+template <class E, int C>
+struct alignas(C * sizeof(E)) Span {
+  E v[C];
+};
+
+// SpMM: LPR lanes per task (a short row or a piece of a split row), each
+// lane folding the column groups lane, lane + LPR, ... (C adjacent columns
+// a group, one load an entry) of its row over the task's entries; each
+// column folds the entries in order.
+template <class E, int ADD, int MUL, bool HW, int C>
 __global__ __launch_bounds__(256) void spmm_rows(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ tgt, const E* __restrict__ w,
-    const E* __restrict__ x, E* __restrict__ y, uint64_t width, uint64_t n, uint32_t lpr,
-    const uint32_t* __restrict__ extra, uint32_t nextra, const uint32_t* __restrict__ ticket_of_row,
-    const uint32_t* __restrict__ pbase, typename Fold<E, ADD>::A* __restrict__ partials,
-    WriteArgs wa, unsigned long long* __restrict__ err) {
+    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
+    E* __restrict__ y, uint64_t width, const SpmmTask* __restrict__ tasks, uint64_t ntasks, uint32_t lpr,
+    typename Fold<E, ADD>::A* __restrict__ partials, WriteArgs wa, unsigned long long* __restrict__ err) {
   using F = Fold<E, ADD>;
   using A = typename F::A;
   const uint32_t tasks_per_block = 256 / lpr;
   const uint64_t task = (uint64_t)blockIdx.x * tasks_per_block + threadIdx.x / lpr;
   const uint32_t lane = threadIdx.x % lpr;
-  if (task >= n + nextra) return;
-  uint64_t row;
-  uint32_t piece;
-  if (task < n) {
-    row = task;
-    piece = 0;
-  } else {
-    row = extra[2 * (task - n)];
-    piece = extra[2 * (task - n) + 1];
-  }
-  const uint64_t rb = off[row], re = off[row + 1];
-  const uint64_t deg = re - rb;
-  const uint32_t np = deg > kSpmmPiece ? (uint32_t)((deg + kSpmmPiece - 1) / kSpmmPiece) : 1;
-  const uint64_t e0 = rb + (uint64_t)piece * kSpmmPiece;
-  const uint64_t e1 = e0 + kSpmmPiece < re ? e0 + kSpmmPiece : re;
+  if (task >= ntasks) return;
+  const SpmmTask tk = tasks[task];
+  const uint64_t row = tk.row;
+  const uint64_t e0 = tk.ec & ((1ull << kSpmmCountShift) - 1);
+  const uint64_t e1 = e0 + (tk.ec >> kSpmmCountShift);
   uint32_t e_err = 0;
-  const uint32_t tk = np > 1 ? ticket_of_row[row] : 0;
-  for (uint64_t c = lane; c < width; c += lpr) {
-    A acc{};
+  for (uint64_t c = (uint64_t)lane * C; c < width; c += (uint64_t)lpr * C) {
+    A acc[C];
     int pres = 0;
     // Eight entries at a time, their loads in flight together (a slot past
-    // the end reads row 0's column c and drops it); folded in order.
+    // the end reads row 0's columns and drops them); folded in order.
     constexpr int kU = 8;
     for (uint64_t e = e0; e < e1; e += kU) {
       uint32_t t4[kU];
-      E wk[kU], xv[kU];
+      E wk[kU];
+      Span<E, C> xv[kU];
 #pragma unroll
       for (int k = 0; k < kU; ++k) {
         t4[k] = 0;
@@ -477,22 +474,32 @@ __global__ __launch_bounds__(256) void spmm_rows(
       }
 #pragma unroll
       for (int k = 0; k < kU; ++k) {
-        xv[k] = E();
-        if constexpr (reads_x<MUL>()) xv[k] = x[(uint64_t)t4[k] * width + c];
+        if constexpr (reads_x<MUL>()) {
+          xv[k] = *reinterpret_cast<const Span<E, C>*>(x + (uint64_t)t4[k] * width + c);
+        } else {
+#pragma unroll
+          for (int q = 0; q < C; ++q) xv[k].v[q] = E();
+        }
       }
 #pragma unroll
       for (int k = 0; k < kU; ++k)
         if (e + k < e1) {
-          A v = F::lift(multiply<E, ADD, MUL>(wk[k], xv[k], e_err));
-          acc = pres ? F::combine(acc, v, e_err) : v;
+#pragma unroll
+          for (int q = 0; q < C; ++q) {
+            A v = F::lift(multiply<E, ADD, MUL>(wk[k], xv[k].v[q], e_err));
+            acc[q] = pres ? F::combine(acc[q], v, e_err) : v;
+          }
           pres = 1;
         }
     }
-    if (np == 1) {
-      E res = pres ? F::finish(acc, e_err) : E();
-      write_elem<E, ADD>(wa, row, &y[row * width + c], pres != 0, res, e_err);
-    } else {
-      partials[(uint64_t)(pbase[tk] + piece) * width + c] = acc;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (tk.slot == ~0u) {
+        E res = pres ? F::finish(acc[q], e_err) : E();
+        write_elem<E, ADD>(wa, row, &y[row * width + c + q], pres != 0, res, e_err);
+      } else {
+        partials[(uint64_t)tk.slot * width + c + q] = acc[q];
+      }
     }
   }
   if (e_err) raise_error(err, row, e_err);
@@ -666,20 +673,28 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       if ((s = prepare_spmm(ctx, csr))) return s;
       if ((s = ensure_scratch(ctx, (size_t)std::max<uint32_t>(csr.spmm_npartials, 1) * width * sizeof(A))))
         return s;
+      // A lane takes 16 bytes of columns an entry when the rows of x split
+      // into aligned 16-byte groups; else one column.
+      constexpr int kC = (sizeof(E) == 4 || sizeof(E) == 8) ? (int)(16 / sizeof(E)) : 1;
+      const bool grouped = kC > 1 && width % kC == 0 && (uintptr_t)x % 16 == 0;
+      const uint64_t groups = grouped ? width / kC : width;
       uint32_t lpr = 1;
-      while (lpr < width && lpr < 64) lpr <<= 1;
-      const uint64_t tasks = csr.n + csr.spmm_nextra;
-      const uint64_t grid = (tasks + (256 / lpr) - 1) / (256 / lpr);
+      while (lpr < groups && lpr < 64) lpr <<= 1;
+      const uint64_t grid = (csr.spmm_ntasks + (256 / lpr) - 1) / (256 / lpr);
       if (grid == 0) return WG_OK;
       A* part = (A*)ctx->scratch;
-      if (wv)
-        spmm_rows<E, ADD, MUL, true><<<(unsigned)grid, 256, 0, ctx->stream>>>(
-            csr.offsets, csr.targets, wv, x, y, width, csr.n, lpr, csr.spmm_extra, csr.spmm_nextra,
-            csr.spmm_ticket_of_row, csr.spmm_pbase, part, wa, ctx->d_err);
-      else
-        spmm_rows<E, ADD, MUL, false><<<(unsigned)grid, 256, 0, ctx->stream>>>(
-            csr.offsets, csr.targets, nullptr, x, y, width, csr.n, lpr, csr.spmm_extra,
-            csr.spmm_nextra, csr.spmm_ticket_of_row, csr.spmm_pbase, part, wa, ctx->d_err);
+      auto run = [&](auto hw, auto cols) {
+        spmm_rows<E, ADD, MUL, decltype(hw)::value, decltype(cols)::value>
+            <<<(unsigned)grid, 256, 0, ctx->stream>>>(csr.targets, wv, x, y, width, csr.spmm_tasks,
+                                                      csr.spmm_ntasks, lpr, part, wa, ctx->d_err);
+      };
+      if (wv) {
+        if (grouped) run(std::true_type{}, std::integral_constant<int, kC>{});
+        else run(std::true_type{}, std::integral_constant<int, 1>{});
+      } else {
+        if (grouped) run(std::false_type{}, std::integral_constant<int, kC>{});
+        else run(std::false_type{}, std::integral_constant<int, 1>{});
+      }
       if (csr.spmm_nlong) {
         const uint64_t items = (uint64_t)csr.spmm_nlong * width;
         spmm_long<E, ADD><<<(unsigned)((items + 255) / 256), 256, 0, ctx->stream>>>(

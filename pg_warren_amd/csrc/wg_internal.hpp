@@ -37,6 +37,15 @@ constexpr uint32_t kBlockRows = 512;   // rows a short block may hold
 // A CSR resident in device memory, with its SpMV partition and the edge
 // weights cast once per element type (SemiringElement::from_weight,
 // compute/semiring.rs:243-246); a cast that fails keeps its status.
+// A SpMM task: entries [e0, e0 + cnt) of `row`, packed as e0 | cnt << 55
+// (cnt <= kSpmmPiece); slot ~0 for a whole row, else the partial slot of
+// this piece of a split row.
+struct SpmmTask {
+  uint64_t ec;
+  uint32_t row, slot;
+};
+constexpr int kSpmmCountShift = 55;
+
 struct DevCsr {
   uint64_t n = 0, nnz = 0;
   uint64_t* offsets = nullptr;   // n + 1
@@ -53,10 +62,12 @@ struct DevCsr {
   uint64_t max_degree = 0;
   void* wview[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int wstatus[5] = {-1, -1, -1, -1, -1};  // -1: not cast yet
-  // SpMM: long rows split into pieces of at most kSpmmPiece entries.
-  uint32_t* spmm_extra = nullptr;  // (row, piece) pairs beyond piece 0
-  uint32_t spmm_nextra = 0;
-  uint32_t* spmm_ticket_of_row = nullptr;  // per row: split-row index or ~0
+  // SpMM: long rows split into pieces of at most kSpmmPiece entries; one
+  // task per short row or piece, the tasks ordered by entry count (most
+  // first, then by row and piece) so that the rows a wavefront folds
+  // together are about as long.
+  SpmmTask* spmm_tasks = nullptr;
+  uint64_t spmm_ntasks = 0;
   uint32_t* spmm_pbase = nullptr;          // nlong + 1: partial slots per split row
   uint32_t* spmm_long_rows = nullptr;      // nlong: the row of each split row
   uint32_t spmm_nlong = 0, spmm_npartials = 0;

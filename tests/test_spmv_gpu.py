@@ -250,7 +250,7 @@ def test_float_plus_within_rounding_bound(ctx, oracle, variant):
     W.set_spmv_variant(ctx, -1)
 
 
-@pytest.mark.parametrize("width", [2, 5, 64, 70])
+@pytest.mark.parametrize("width", [2, 4, 5, 16, 64, 70])
 def test_spmm_matches_the_oracle(ctx, oracle, width):
     g = hub_graph(oracle, 7, n=3000, m=20000)
     rng = np.random.default_rng(width)
