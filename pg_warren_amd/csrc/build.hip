@@ -175,24 +175,52 @@ __global__ __launch_bounds__(kThreads) void row_starts(const uint32_t* __restric
   }
 }
 
-// row_of[p] = the row of entry p, for rows of at most kBlockNnz entries
-// (a thread a row); longer rows, the CSR's split rows, go to expand_long.
-__global__ __launch_bounds__(kThreads) void expand_short(const uint64_t* __restrict__ off, uint64_t n,
-                                                         uint32_t* __restrict__ row_of) {
-  for (uint64_t r = first_index(); r < n; r += stride()) {
-    const uint64_t b = off[r], e = off[r + 1];
-    if (e - b > kBlockNnz) continue;
-    for (uint64_t p = b; p < e; ++p) row_of[p] = (uint32_t)r;
+// The row holding entry p: the last r in [lo, n) with off[r] <= p (so
+// off[r] <= p < off[r + 1]), given off[lo] <= p < off[n].
+__device__ __forceinline__ uint64_t row_holding(const uint64_t* __restrict__ off, uint64_t n, uint64_t p,
+                                                uint64_t lo) {
+  uint64_t hi = n;
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (off[mid] <= p) lo = mid;
+    else hi = mid;
   }
+  return lo;
 }
 
-// One workgroup per split row.
-__global__ __launch_bounds__(kThreads) void expand_long(const uint64_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ long_rows,
-                                                        uint32_t* __restrict__ row_of) {
-  const uint32_t r = long_rows[blockIdx.x];
-  const uint64_t e = off[r + 1];
-  for (uint64_t p = off[r] + threadIdx.x; p < e; p += kThreads) row_of[p] = r;
+// row_of[p] = the row of entry p. A thread takes kExpandRun consecutive
+// entries: a binary search finds the first one's row, a row change inside
+// the run searches again from there (empty rows are skipped however many),
+// and the run is stored as 16-byte writes when row_of is so aligned.
+constexpr uint32_t kExpandRun = 16;
+__global__ __launch_bounds__(kThreads) void expand_entries(const uint64_t* __restrict__ off, uint64_t n,
+                                                           uint64_t m, uint32_t* __restrict__ row_of,
+                                                           bool vec) {
+  for (uint64_t t = first_index(); t < (m + kExpandRun - 1) / kExpandRun; t += stride()) {
+    const uint64_t p0 = t * kExpandRun;
+    const uint32_t cnt = (uint32_t)min<uint64_t>(kExpandRun, m - p0);
+    uint64_t r = row_holding(off, n, p0, 0);
+    uint64_t next = off[r + 1];
+    uint32_t v[kExpandRun];
+#pragma unroll
+    for (uint32_t j = 0; j < kExpandRun; ++j) {
+      if (j < cnt && p0 + j >= next && r + 1 < n) {
+        r = row_holding(off, n, p0 + j, r + 1);
+        next = off[r + 1];
+      }
+      v[j] = (uint32_t)r;
+    }
+    if (vec && cnt == kExpandRun) {
+      uint4* dst = reinterpret_cast<uint4*>(row_of + p0);
+#pragma unroll
+      for (uint32_t q = 0; q < kExpandRun / 4; ++q)
+        dst[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < kExpandRun; ++j)
+        if (j < cnt) row_of[p0 + j] = v[j];
+    }
+  }
 }
 
 // Reverse position k holds forward entry perm[k]: its source, its weight.
@@ -400,11 +428,12 @@ int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, 
 
 }  // namespace
 
-// row_of[p] for every entry of a partitioned CSR.
+// row_of[p] for every entry of a CSR.
 int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of) {
-  if (c.nnz == 0) return WG_OK;
-  expand_short<<<grid_of(c.n), kThreads, 0, ctx->stream>>>(c.offsets, c.n, row_of);
-  if (c.nlong) expand_long<<<c.nlong, kThreads, 0, ctx->stream>>>(c.offsets, c.long_rows, row_of);
+  if (c.nnz == 0 || c.n == 0) return WG_OK;
+  const uint64_t runs = (c.nnz + kExpandRun - 1) / kExpandRun;
+  expand_entries<<<grid_of(runs), kThreads, 0, ctx->stream>>>(c.offsets, c.n, c.nnz, row_of,
+                                                              (uintptr_t)row_of % 16 == 0);
   WG_HIP(ctx, hipGetLastError());
   return WG_OK;
 }

@@ -289,8 +289,25 @@ __global__ __launch_bounds__(kLongThreads) void fold_long(const uint32_t* __rest
     __syncthreads();
     if (base + kStage < b) load(base + kStage);
     if (lane == 0) {
+      // The same left fold as fold_into, with nothing but the operation on
+      // the accumulator's chain: the first element outside the loop, and
+      // the first error code kept aside (a failed receiver raises; the
+      // value it then holds is never used).
       const unsigned cnt = (unsigned)(b - base < kStage ? b - base : kStage);
-      for (unsigned t = 0; t < cnt; ++t) fold_into<E, OP>(acc, first, e_err, stage[t]);
+      unsigned t = 0;
+      if (first) {
+        acc = stage[0];
+        first = false;
+        t = 1;
+      }
+      uint32_t le = 0;
+#pragma unroll 8
+      for (; t < cnt; ++t) {
+        uint32_t te = 0;
+        acc = apply<OP>(acc, stage[t], te);
+        le = le ? le : te;
+      }
+      e_err = e_err ? e_err : le;
     }
     __syncthreads();
   }

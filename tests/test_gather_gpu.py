@@ -144,3 +144,25 @@ def test_hubs_and_many_messages(ctx):
     vals = rng.standard_normal(20000)
     for comb in ("sum", "collect", "mode", "max"):
         compare(ctx, g, n, edges, senders, vals, comb)
+
+
+def test_long_segments_fold_in_order_and_raise(ctx):
+    """Receivers with more than 512 messages (the one-lane ordered fold):
+    f64 and i64 sums that overflow part-way raise what the oracle's left
+    fold raises; finite ones, and min / max, equal it bit for bit."""
+    rng = np.random.default_rng(5)
+    n = 2000
+    edges = sorted({(0, s, 3) for s in range(n) if s != 3} |
+                   {(0, int(s), 9) for s in rng.integers(0, n, 900)} |
+                   {(0, int(a), int(b)) for a, b in rng.integers(0, n, (3000, 2))})
+    g = graph_of(ctx, n, edges)
+    senders = np.arange(n)
+    print(f"long-segment gather: n={n} edges={len(edges)}", flush=True)
+    finite = rng.standard_normal(n) * 1e3
+    for comb in ("sum", "min", "max"):
+        compare(ctx, g, n, edges, senders, finite, comb)
+    big = np.where(rng.random(n) < 0.5, 1e306, -1e305)
+    compare(ctx, g, n, edges, senders, big, "sum")
+    ints = rng.integers(2**58, 2**59, n).astype(np.int64)
+    compare(ctx, g, n, edges, senders, ints, "sum")
+    compare(ctx, g, n, edges, senders, -ints, "sum")
