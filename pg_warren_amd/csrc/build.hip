@@ -188,24 +188,41 @@ __device__ __forceinline__ uint64_t row_holding(const uint64_t* __restrict__ off
   return lo;
 }
 
-// row_of[p] = the row of entry p. A thread takes kExpandRun consecutive
-// entries: a binary search finds the first one's row, a row change inside
-// the run searches again from there (empty rows are skipped however many),
-// and the run is stored as 16-byte writes when row_of is so aligned.
+// row_of[p] = the row of entry p, a tile of kExpandTile entries at a time.
+// anchor[k] = the row of entry k * kExpandTile (a thread each, a search over
+// all rows); a tile's rows then lie in [anchor[k], anchor[k + 1]] (the
+// last tile's up to n - 1), so a thread's run of kExpandRun consecutive
+// entries searches only those: the first entry by binary search, a row
+// change inside the run again from there (empty rows are skipped however
+// many). Runs are stored as 16-byte writes when row_of is so aligned.
 constexpr uint32_t kExpandRun = 16;
+constexpr uint64_t kExpandTile = (uint64_t)kThreads * kExpandRun;
+
+__global__ __launch_bounds__(kThreads) void expand_anchors(const uint64_t* __restrict__ off, uint64_t n,
+                                                           uint64_t m, uint32_t* __restrict__ anchor) {
+  const uint64_t tiles = (m + kExpandTile - 1) / kExpandTile;
+  for (uint64_t k = first_index(); k < tiles; k += stride())
+    anchor[k] = (uint32_t)row_holding(off, n, k * kExpandTile, 0);
+}
+
 __global__ __launch_bounds__(kThreads) void expand_entries(const uint64_t* __restrict__ off, uint64_t n,
-                                                           uint64_t m, uint32_t* __restrict__ row_of,
-                                                           bool vec) {
-  for (uint64_t t = first_index(); t < (m + kExpandRun - 1) / kExpandRun; t += stride()) {
-    const uint64_t p0 = t * kExpandRun;
+                                                           uint64_t m, const uint32_t* __restrict__ anchor,
+                                                           uint32_t* __restrict__ row_of, bool vec) {
+  const uint64_t tiles = (m + kExpandTile - 1) / kExpandTile;
+  for (uint64_t k = blockIdx.x; k < tiles; k += gridDim.x) {
+    const uint64_t p0 = k * kExpandTile + (uint64_t)threadIdx.x * kExpandRun;
+    if (p0 >= m) continue;
     const uint32_t cnt = (uint32_t)min<uint64_t>(kExpandRun, m - p0);
-    uint64_t r = row_holding(off, n, p0, 0);
+    // Rows [lo, hi) hold the tile: off[lo] <= p0 < off[hi].
+    const uint64_t lo = anchor[k];
+    const uint64_t hi = k + 1 < tiles ? (uint64_t)anchor[k + 1] + 1 : n;
+    uint64_t r = row_holding(off, hi, p0, lo);
     uint64_t next = off[r + 1];
     uint32_t v[kExpandRun];
 #pragma unroll
     for (uint32_t j = 0; j < kExpandRun; ++j) {
-      if (j < cnt && p0 + j >= next && r + 1 < n) {
-        r = row_holding(off, n, p0 + j, r + 1);
+      if (j < cnt && p0 + j >= next && r + 1 < hi) {
+        r = row_holding(off, hi, p0 + j, r + 1);
         next = off[r + 1];
       }
       v[j] = (uint32_t)r;
@@ -431,9 +448,13 @@ int build_forward(wg_context* ctx, uint64_t n, uint64_t m, const uint32_t* src, 
 // row_of[p] for every entry of a CSR.
 int expand_rows(wg_context* ctx, const DevCsr& c, uint32_t* row_of) {
   if (c.nnz == 0 || c.n == 0) return WG_OK;
-  const uint64_t runs = (c.nnz + kExpandRun - 1) / kExpandRun;
-  expand_entries<<<grid_of(runs), kThreads, 0, ctx->stream>>>(c.offsets, c.n, c.nnz, row_of,
-                                                              (uintptr_t)row_of % 16 == 0);
+  const uint64_t tiles = (c.nnz + kExpandTile - 1) / kExpandTile;
+  Scratch<uint32_t> anchor;
+  int s;
+  if ((s = anchor.alloc(ctx, tiles))) return s;
+  expand_anchors<<<grid_of(tiles), kThreads, 0, ctx->stream>>>(c.offsets, c.n, c.nnz, anchor.p);
+  expand_entries<<<(unsigned)std::min<uint64_t>(tiles, 65536), kThreads, 0, ctx->stream>>>(
+      c.offsets, c.n, c.nnz, anchor.p, row_of, (uintptr_t)row_of % 16 == 0);
   WG_HIP(ctx, hipGetLastError());
   return WG_OK;
 }

@@ -257,10 +257,11 @@ __global__ __launch_bounds__(kThreads) void fold_segments(uint64_t n, const uint
   }
 }
 
-// A long segment, one 64-lane block each: the block stages 256 messages at
-// a time in LDS, the next 256 loading into registers while lane 0 folds
-// the staged ones in order.
-constexpr unsigned kLongThreads = 64, kStage = 256;
+// A long segment, one workgroup each: the workgroup stages 2048 messages at
+// a time in LDS, the next 2048 loading into registers while thread 0 folds
+// the staged ones in order (a chunk's fold outlasts a load's latency, so
+// the serial fold of a hub's segment never waits on memory).
+constexpr unsigned kLongThreads = 256, kStage = 2048;
 
 template <class E, int OP>
 __global__ __launch_bounds__(kLongThreads) void fold_long(const uint32_t* __restrict__ long_list,
