@@ -154,12 +154,17 @@ __global__ __launch_bounds__(kThreads) void receiver_counts(Receiving R, uint64_
 // then each sender entry's run at its prefix position (thread per entry),
 // so every receiver's messages lie in collect order at [start[r],
 // start[r + 1]). ord, if given, gets the receiver's rank at each message.
+// fill_self also leaves base[r], where entry p of receiver r puts its run:
+// base[r] + eprefix[p] (its own messages first, then its entries' in order).
 template <class E>
 __global__ __launch_bounds__(kThreads) void fill_self(Receiving R, uint64_t n, const E* __restrict__ mval,
                                                       const uint64_t* __restrict__ start,
+                                                      const uint64_t* __restrict__ eprefix,
                                                       const uint64_t* __restrict__ rank, E* __restrict__ out,
-                                                      uint32_t* __restrict__ ord) {
+                                                      uint32_t* __restrict__ ord, uint64_t* __restrict__ base) {
   for (uint64_t r = first_index(); r < n; r += stride()) {
+    if (R.soff)
+      base[r] = start[r] + (R.self ? R.moff[r + 1] - R.moff[r] : 0) - eprefix[R.soff[r]];
     if (!R.self || !selected(R, r)) continue;
     uint64_t at = start[r];
     for (uint64_t k = R.moff[r]; k < R.moff[r + 1]; ++k, ++at) {
@@ -173,7 +178,7 @@ template <class E>
 __global__ __launch_bounds__(kThreads) void fill_entries(Receiving R, uint64_t m, const uint32_t* __restrict__ row_of,
                                                          const uint64_t* __restrict__ eprefix,
                                                          const E* __restrict__ mval,
-                                                         const uint64_t* __restrict__ start,
+                                                         const uint64_t* __restrict__ base,
                                                          const uint64_t* __restrict__ rank, E* __restrict__ out,
                                                          uint32_t* __restrict__ ord) {
   // Four entries a step (p, p + stride, ...): each stage's loads go out
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void fill_entries(Receiving R, uint64_t m
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t r = r4[k];
-      at4[k] = start[r] + (R.self ? R.moff[r + 1] - R.moff[r] : 0) + e4[k] - eprefix[R.soff[r]];
+      at4[k] = base[r] + e4[k];
       m0[k] = R.moff[s4[k]];
       m1[k] = R.moff[s4[k] + 1];
     }
@@ -257,11 +262,10 @@ __global__ __launch_bounds__(kThreads) void fold_segments(uint64_t n, const uint
   }
 }
 
-// A long segment, one workgroup each: the workgroup stages 2048 messages at
-// a time in LDS, the next 2048 loading into registers while thread 0 folds
-// the staged ones in order (a chunk's fold outlasts a load's latency, so
-// the serial fold of a hub's segment never waits on memory).
-constexpr unsigned kLongThreads = 256, kStage = 2048;
+// A long segment, one 64-lane block each: the block stages 256 messages at
+// a time in LDS, the next 256 loading into registers while lane 0 folds
+// the staged ones in order.
+constexpr unsigned kLongThreads = 64, kStage = 256;
 
 template <class E, int OP>
 __global__ __launch_bounds__(kLongThreads) void fold_long(const uint32_t* __restrict__ long_list,
@@ -290,25 +294,8 @@ __global__ __launch_bounds__(kLongThreads) void fold_long(const uint32_t* __rest
     __syncthreads();
     if (base + kStage < b) load(base + kStage);
     if (lane == 0) {
-      // The same left fold as fold_into, with nothing but the operation on
-      // the accumulator's chain: the first element outside the loop, and
-      // the first error code kept aside (a failed receiver raises; the
-      // value it then holds is never used).
       const unsigned cnt = (unsigned)(b - base < kStage ? b - base : kStage);
-      unsigned t = 0;
-      if (first) {
-        acc = stage[0];
-        first = false;
-        t = 1;
-      }
-      uint32_t le = 0;
-#pragma unroll 8
-      for (; t < cnt; ++t) {
-        uint32_t te = 0;
-        acc = apply<OP>(acc, stage[t], te);
-        le = le ? le : te;
-      }
-      e_err = e_err ? e_err : le;
+      for (unsigned t = 0; t < cnt; ++t) fold_into<E, OP>(acc, first, e_err, stage[t]);
     }
     __syncthreads();
   }
@@ -608,8 +595,9 @@ static int run_gather(wg_context* ctx, const Receiving& R, const DevCsr& S, uint
   if (combiner == C_MODE && (s = ord.alloc(ctx, total))) return s;
   if ((s = expand_rows(ctx, S, row_of.p))) return s;
   uint32_t* ordp = combiner == C_MODE ? ord.p : nullptr;
-  fill_self<E><<<grid, kThreads, 0, st>>>(R, n, mval, start.p, rank.p, coll.p, ordp);
-  fill_entries<E><<<grid_of(m), kThreads, 0, st>>>(R, m, row_of.p, eprefix.p, mval, start.p, rank.p, coll.p, ordp);
+  // flags (n entries) is free once the ranks are scanned: it holds base.
+  fill_self<E><<<grid, kThreads, 0, st>>>(R, n, mval, start.p, eprefix.p, rank.p, coll.p, ordp, flags.p);
+  fill_entries<E><<<grid_of(m), kThreads, 0, st>>>(R, m, row_of.p, eprefix.p, mval, flags.p, rank.p, coll.p, ordp);
   WG_HIP(ctx, hipGetLastError());
   row_of.reset();
   if (combiner != C_COLLECT && combiner != C_MODE) {
