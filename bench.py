@@ -734,8 +734,16 @@ def main():
 
     head = legs["plus_times_f32"]
     tuned = head.get("tuned_variant", 0)
-    kernel_name = ("spmv_direct" if tuned >= 0 and tuned & 1 else "spmv_blocks") + \
-        "<float, 3, 4, true>"
+    if tuned == 32:
+        # The tiled product: one step is four launches on the context's
+        # stream (hot x staged, the tiled fold, the wave-boundary rows, the
+        # row writes); the step's time, all four, is the figure.
+        kernel_name = "spmv_tiles<float, 3, 4, true>"
+        step_kernels = ["gather_hot<float>", kernel_name, "tile_fixup<float, 3>", "tile_write<float, 3>"]
+    else:
+        kernel_name = ("spmv_direct" if tuned >= 0 and tuned & 1 else "spmv_blocks") + \
+            "<float, 3, 4, true>"
+        step_kernels = [kernel_name, "spmv_long<float, 3>"]
     result = {
         "metric": "GTEPS & HBM GB/s per SpMV/BFS; batched PPR vs plain, same GPU, 1e-9 L1",
         "value": head["gteps"],
@@ -755,6 +763,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": head["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kernel_name), "kernel": kernel_name,
+                     "step_kernels": step_kernels,
                      "traffic_source": "profiles/spmv_rmat22_pmc.json (rocprofv3 FETCH_SIZE, "
                                        "WRITE_SIZE; copy-kernel scaled)",
                      "algorithmic_bytes": head["bytes"]},

@@ -81,12 +81,22 @@ typedef struct wg_vector wg_vector;
  * (compute/memory.rs:31-60) and the Interrupts/Cancellation pair
  * (interrupts.rs:22-70). memory_limit = 0: no limit. */
 int wg_context_create(int device, uint64_t memory_limit, wg_context** out);
-/* Every graph and vector of the context must be destroyed first. */
+/* Releases the context's device memory, including that of every graph,
+ * vector, BFS cursor and gathered result made through it and not yet
+ * destroyed: those are left orphaned, so a later wg_*_destroy of one only
+ * frees its host record, and any other call given one returns
+ * WG_INVALID_ARGUMENT. A graph, vector, cursor or result passed with a
+ * context other than its own is refused the same way. */
 void wg_context_destroy(wg_context* ctx);
 /* Copies the last failure's message; returns its status. */
 int wg_last_error(const wg_context* ctx, char* buf, size_t len);
 /* Cancellation::cancel (interrupts.rs:40-70): the next check returns
- * WG_CANCELLED; wg_cancel_reset clears it. */
+ * WG_CANCELLED; wg_cancel_reset clears it. wg_cancel is the one call that
+ * may be made from another thread while a call on the context runs (as the
+ * reference's flag is raised by the thread waiting for the pool): it sets
+ * an atomic word that calls check before each launch, search level, batch
+ * of PageRank iterations and Lanczos step. Every other call on a context
+ * comes from one thread at a time. */
 void wg_cancel(wg_context* ctx);
 void wg_cancel_reset(wg_context* ctx);
 /* ComputeMemory::in_use (compute/memory.rs:97-120). */
@@ -116,6 +126,9 @@ void* wg_stream(wg_context* ctx);
  * NULL (unweighted: every weight reads as 1). Host pointers; copied. */
 int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* offsets,
                     const uint32_t* targets, const double* weights, wg_graph** out);
+/* Also releases any BFS cursor still reading the graph (the cursor's next
+ * step then returns WG_INVALID_ARGUMENT; wg_bfs_cursor_destroy still frees
+ * it). */
 void wg_graph_destroy(wg_graph* g);
 /* n, nnz and whether the graph carries weights. */
 int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted);

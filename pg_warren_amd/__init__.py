@@ -719,7 +719,8 @@ def msbfs(ctx, graph, sources, *, direction="out", max_depth=None, depths=True, 
 
 def set_spmv_variant(ctx, variant):
     """Forces an SpMV launch variant (0..31: bits direct, non-temporal, XCD order,
-    hot-first gathers, LDS hot-column cache), or -1 to autotune."""
+    hot-first gathers, LDS hot-column cache; 32: the tiled kernel), or -1 to
+    autotune."""
     ctx.lib.wg_set_spmv_variant(ctx.h, variant)
 
 

@@ -1105,12 +1105,32 @@ struct wg_bfs_cursor {
   wg::LevelEmitter e;
 };
 
+namespace wg {
+void release_cursor(wg_bfs_cursor* c) {
+  LevelEmitter& e = c->e;
+  if (!e.ctx) return;
+  (void)hipStreamSynchronize(e.ctx->stream);
+  for (auto& b : e.owned) dfree(e.ctx, b.first, b.second);
+  e.owned.clear();
+  e.ctx = nullptr;
+  e.g = nullptr;
+  e.finished = true;
+}
+
+bool cursor_reads(const wg_bfs_cursor* c, const wg_graph* g) {
+  for (const wg_graph* r = c->e.g; r; r = r->owner)
+    if (r == g) return true;
+  return false;
+}
+}  // namespace wg
+
 extern "C" {
 
 int wg_bfs_levels(wg_context* ctx, const wg_graph* gc, int direction, const uint32_t* relations,
                   uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
                   int64_t max_depth, int pulling, wg_vector* depth, wg_vector* via, wg_traversal_stats* stats) {
   if (!ctx || !gc || !depth || !via) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   wg_graph* g = const_cast<wg_graph*>(gc);
   Traversal t;
   int s = traversal_of(ctx, g, direction, relations, nrelations, false, t);
@@ -1157,6 +1177,7 @@ int wg_bfs(wg_context* ctx, const wg_graph* gc, int direction, const uint32_t* r
            const uint64_t* starts, uint64_t nstarts, const uint8_t* filter, int64_t max_depth, int64_t max_results,
            wg_visit* out, uint64_t* count, wg_traversal_stats* stats) {
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   WG_HIP(ctx, hipSetDevice(ctx->device));
   LevelEmitter e;
   int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, relations, nrelations, starts, nstarts, filter,
@@ -1171,6 +1192,7 @@ int wg_reachable_set(wg_context* ctx, const wg_graph* gc, int direction, const u
                      uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, int64_t max_depth, uint64_t* out,
                      uint64_t* count, wg_traversal_stats* stats) {
   if (!ctx || !gc || !out || !count) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   WG_HIP(ctx, hipSetDevice(ctx->device));
   LevelEmitter e;
   int s = e.init(ctx, const_cast<wg_graph*>(gc), direction, relations, nrelations, starts, nstarts, nullptr,
@@ -1186,6 +1208,7 @@ int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, cons
                          uint32_t nrelations, const uint64_t* starts, uint64_t nstarts, const uint8_t* filter,
                          int64_t max_depth, int64_t max_results, int reachable, wg_bfs_cursor** out) {
   if (!ctx || !g || !out) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, g)) return cs_;
   *out = nullptr;
   if (reachable && (filter || max_results >= 0))
     return fail(ctx, WG_INVALID_ARGUMENT, "a reachable-set cursor takes no filter or result limit");
@@ -1200,12 +1223,16 @@ int wg_bfs_cursor_create(wg_context* ctx, const wg_graph* g, int direction, cons
     delete c;
     return s;
   }
+  ctx->cursors.insert(c);
   *out = c;
   return ok(ctx);
 }
 
 int wg_bfs_cursor_next(wg_context* ctx, wg_bfs_cursor* c, wg_visit* out, uint64_t* count, int* more) {
   if (!ctx || !c || !out || !count || !more) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (c->e.ctx != ctx)
+    return fail(ctx, WG_INVALID_ARGUMENT,
+                c->e.ctx ? "the cursor belongs to another context" : "the cursor's graph or context was destroyed");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   int s = c->e.next(out, count, more);
   return s ? s : ok(ctx);
@@ -1217,6 +1244,11 @@ int wg_bfs_cursor_stats(const wg_bfs_cursor* c, wg_traversal_stats* stats) {
   return WG_OK;
 }
 
-void wg_bfs_cursor_destroy(wg_bfs_cursor* c) { delete c; }
+void wg_bfs_cursor_destroy(wg_bfs_cursor* c) {
+  if (!c) return;
+  if (c->e.ctx) c->e.ctx->cursors.erase(c);
+  release_cursor(c);
+  delete c;
+}
 
 }  // extern "C"

@@ -755,6 +755,7 @@ using namespace wg;
 extern "C" int wg_graph_select(wg_context* ctx, wg_graph* g, const uint32_t* relations,
                                uint32_t nrelations, int per_relation, wg_graph** out) {
   if (!ctx || !g || !out) return WG_INVALID_ARGUMENT;
+  if (int cs_ = check_graph(ctx, g)) return cs_;
   *out = nullptr;
   if (nrelations && !relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
   int s = select_graph(ctx, g, relations, nrelations, per_relation, out);
@@ -857,6 +858,7 @@ extern "C" int wg_graph_build(wg_context* ctx, uint64_t n, uint64_t m, const uin
     return s;
   }
   if (deduplicated) *deduplicated = folded;
+  ctx->graphs.insert(g);
   *out = g;
   return ok(ctx);
 }

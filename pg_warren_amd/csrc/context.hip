@@ -317,15 +317,42 @@ int adopt_device_csr(wg_context* ctx, DevCsr& c) {
   return partition_csr(ctx, c);
 }
 
+void release_graph(wg_graph* g) {
+  if (!g->ctx) return;
+  wg_context* ctx = g->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (wg_graph* r : g->relations) release_graph(r);
+  for (auto& v : g->selections) release_graph(v.second);
+  free_csr(ctx, g->fwd);
+  free_csr(ctx, g->rev);
+  for (auto& w : g->workspace) dfree(ctx, w.first, w.second);
+  g->workspace.clear();
+  for (auto& c : g->cached) dfree(ctx, c.second.first, c.second.second);
+  g->cached.clear();
+  g->ctx = nullptr;
+}
+
 void destroy_graph(wg_graph* g) {
-  (void)hipStreamSynchronize(g->ctx->stream);
+  release_graph(g);
   for (wg_graph* r : g->relations) destroy_graph(r);
   for (auto& v : g->selections) destroy_graph(v.second);
-  free_csr(g->ctx, g->fwd);
-  free_csr(g->ctx, g->rev);
-  for (auto& w : g->workspace) dfree(g->ctx, w.first, w.second);
-  for (auto& c : g->cached) dfree(g->ctx, c.second.first, c.second.second);
   delete g;
+}
+
+int check_graph(wg_context* ctx, const wg_graph* g) {
+  if (!g) return fail(ctx, WG_INVALID_ARGUMENT, "null graph");
+  if (g->ctx != ctx)
+    return fail(ctx, WG_INVALID_ARGUMENT,
+                g->ctx ? "the graph belongs to another context" : "the graph's context was destroyed");
+  return WG_OK;
+}
+
+int check_vector(wg_context* ctx, const wg_vector* v) {
+  if (!v) return fail(ctx, WG_INVALID_ARGUMENT, "null vector");
+  if (v->ctx != ctx)
+    return fail(ctx, WG_INVALID_ARGUMENT,
+                v->ctx ? "the vector belongs to another context" : "the vector's context was destroyed");
+  return WG_OK;
 }
 
 void free_csr(wg_context* ctx, DevCsr& c) {
@@ -345,6 +372,18 @@ void free_csr(wg_context* ctx, DevCsr& c) {
     dfree(ctx, c.hot_rank, std::max<uint64_t>(c.n, 1) * 4);
   }
   for (auto& t : c.lds_tgt) dfree(ctx, t.second.first, std::max<uint64_t>(c.nnz, 1) * 4);
+  for (auto& t : c.tiles) {
+    dfree(ctx, t.second.tagged, c.tile_padded * 4);
+    for (int d = 0; d < 5; ++d)
+      if (t.second.w[d]) dfree(ctx, t.second.w[d], c.tile_padded * dtype_size(d));
+  }
+  if (c.tile_state == 1) {
+    dfree(ctx, c.tile_wave, (c.tile_nwaves + 1) * 4);
+    dfree(ctx, c.tile_heads, std::max<uint32_t>(c.tile_nwaves, 1) * 4);
+    dfree(ctx, c.nz_bits, ((c.n + 63) / 64 + 1) * 8);
+    dfree(ctx, c.nz_pre, ((c.n + 63) / 64 + 1) * 4);
+  }
+  if (c.tile_out) dfree(ctx, c.tile_out, c.tile_out_bytes);
   if (c.spmm_ready) {
     dfree(ctx, c.spmm_tasks, std::max<uint64_t>(c.spmm_ntasks, 1) * sizeof(SpmmTask));
     dfree(ctx, c.spmm_pbase, (c.spmm_nlong + 1) * 4);
@@ -570,7 +609,8 @@ int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
   ctx->device = device;
   ctx->limit = memory_limit;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void**)&ctx->d_err, 8) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_err, 8) != hipSuccess || hipMalloc((void**)&ctx->d_jerr, 8) != hipSuccess ||
+      hipMemset(ctx->d_jerr, 0xff, 8) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_mail, kMailWords * 8, hipHostMallocCoherent | hipHostMallocMapped) !=
           hipSuccess ||
@@ -586,9 +626,24 @@ void wg_context_destroy(wg_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  // Whatever the caller still holds loses its device memory now and keeps
+  // a host record that its own destroy call frees without touching ctx.
+  for (wg_bfs_cursor* c : ctx->cursors) release_cursor(c);
+  for (wg_gathered* r : ctx->gathereds) release_gathered(r);
+  for (wg_vector* v : ctx->vectors) {
+    dfree(ctx, v->data, v->bytes);
+    v->data = nullptr;
+    v->ctx = nullptr;
+  }
+  for (wg_graph* g : ctx->graphs) release_graph(g);
+  ctx->cursors.clear();
+  ctx->gathereds.clear();
+  ctx->vectors.clear();
+  ctx->graphs.clear();
   flush_cache(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->d_jerr) (void)hipFree(ctx->d_jerr);
   if (ctx->h_err) (void)hipHostFree(ctx->h_err);
   if (ctx->h_mail) (void)hipHostFree(ctx->h_mail);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -666,24 +721,32 @@ int wg_vector_create(wg_context* ctx, int dtype, uint64_t id_space, uint64_t wid
     delete v;
     return hip_fail(ctx, e, "vector create");
   }
+  ctx->vectors.insert(v);
   *out = v;
   return ok(ctx);
 }
 
 void wg_vector_destroy(wg_vector* v) {
   if (!v) return;
-  (void)hipStreamSynchronize(v->ctx->stream);
-  dfree(v->ctx, v->data, v->bytes);
+  if (v->ctx) {
+    v->ctx->vectors.erase(v);
+    (void)hipStreamSynchronize(v->ctx->stream);
+    dfree(v->ctx, v->data, v->bytes);
+  }
   delete v;
 }
 
 int wg_vector_write(wg_context* ctx, wg_vector* v, const void* host) {
+  if (!ctx) return WG_INVALID_ARGUMENT;
+  if (int s = check_vector(ctx, v)) return s;
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (v->bytes) WG_HIP(ctx, hipMemcpy(v->data, host, v->bytes, hipMemcpyHostToDevice));
   return ok(ctx);
 }
 
 int wg_vector_read(wg_context* ctx, const wg_vector* v, void* host) {
+  if (!ctx) return WG_INVALID_ARGUMENT;
+  if (int s = check_vector(ctx, v)) return s;
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (v->bytes) WG_HIP(ctx, hipMemcpy(host, v->data, v->bytes, hipMemcpyDeviceToHost));
   return ok(ctx);
@@ -736,6 +799,7 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint
     delete g;
     return s;
   }
+  ctx->graphs.insert(g);
   *out = g;
   return ok(ctx);
 }
@@ -743,6 +807,20 @@ int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* off, const uint
 void wg_graph_destroy(wg_graph* g) {
   // A relation or selection goes with the graph that owns it.
   if (!g || g->owner) return;
+  if (wg_context* ctx = g->ctx) {
+    ctx->graphs.erase(g);
+    // A cursor over this graph can read it no more: its device memory goes
+    // too (it leaves the context's list, orphaned), and its next step
+    // reports the graph gone.
+    for (auto it = ctx->cursors.begin(); it != ctx->cursors.end();) {
+      if (cursor_reads(*it, g)) {
+        release_cursor(*it);
+        it = ctx->cursors.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
   destroy_graph(g);
 }
 
@@ -756,6 +834,7 @@ int wg_graph_info(const wg_graph* g, uint64_t* n, uint64_t* nnz, int* weighted) 
 
 int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* off, uint32_t* tgt, double* w) {
   if (!ctx || !g || !off) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int s = check_graph(ctx, g)) return s;
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   WG_HIP(ctx, hipMemcpy(off, g->fwd.offsets, (g->n + 1) * 8, hipMemcpyDeviceToHost));
   if (g->nnz) {
@@ -768,6 +847,8 @@ int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* off, uin
 
 int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* off, uint32_t* tgt,
                           double* w) {
+  if (!ctx || !g || !off || (g->nnz && !tgt)) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int s = check_graph(ctx, g)) return s;
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   WG_HIP(ctx, hipMemcpy(off, g->rev.offsets, (g->n + 1) * 8, hipMemcpyDeviceToHost));
   if (g->nnz) {
@@ -781,6 +862,7 @@ int wg_graph_read_reverse(wg_context* ctx, const wg_graph* g, uint64_t* off, uin
 int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges, const double* values,
                             uint64_t count) {
   if (!ctx || !g || (count && (!edges || !values))) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int s = check_graph(ctx, g)) return s;
   if (!g->fwd.weights) return fail(ctx, WG_INVALID_ARGUMENT, "the graph has no edge weights");
   for (uint64_t i = 0; i < count; ++i)
     if (edges[i] >= g->nnz) return fail(ctx, WG_INVALID_ARGUMENT, "edge index outside the graph");
@@ -814,14 +896,21 @@ int wg_graph_update_weights(wg_context* ctx, wg_graph* g, const uint64_t* edges,
   if (e != hipSuccess) return hip_fail(ctx, e, "update weights");
   // A weight changed in one direction breaks A = A^T.
   g->rev_is_fwd = false;
-  // Weights cast to other element types are cast again on next use.
-  for (DevCsr* c : {&g->fwd, &g->rev})
+  // Weights cast to other element types, and every copy in the tiled
+  // product's order, are made again on next use.
+  for (DevCsr* c : {&g->fwd, &g->rev}) {
     for (int d = 0; d < 5; ++d) {
       if (d == WG_F64) continue;
       if (c->wview[d]) dfree(ctx, c->wview[d], c->nnz * dtype_size(d));
       c->wview[d] = nullptr;
       c->wstatus[d] = -1;
     }
+    for (auto& t : c->tiles)
+      for (int d = 0; d < 5; ++d) {
+        if (t.second.w[d]) dfree(ctx, t.second.w[d], c->tile_padded * dtype_size(d));
+        t.second.w[d] = nullptr;
+      }
+  }
   return ok(ctx);
 }
 

@@ -684,6 +684,7 @@ static int run_gather(wg_context* ctx, const Receiving& R, const DevCsr& S, uint
 extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, int dtype,
                          uint64_t count, const uint32_t* senders, const void* messages, wg_gathered** out) {
   if (!ctx || !gc || !o || !out) return WG_INVALID_ARGUMENT;
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   *out = nullptr;
   if (count && (!senders || !messages)) return fail(ctx, WG_INVALID_ARGUMENT, "null messages");
   if (o->direction != WG_OUT && o->direction != WG_IN && o->direction != WG_BOTH)
@@ -765,6 +766,7 @@ extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_op
     delete res;
     return s;
   }
+  ctx->gathereds.insert(res);
   *out = res;
   return ok(ctx);
 }
@@ -779,6 +781,7 @@ extern "C" int wg_gathered_info(const wg_gathered* r, uint64_t* receivers, uint6
 extern "C" int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t* receivers, void* values,
                                 uint64_t* offsets) {
   if (!ctx || !r) return WG_INVALID_ARGUMENT;
+  if (r->ctx != ctx) return fail(ctx, WG_INVALID_ARGUMENT, "the result belongs to another or a destroyed context");
   WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (receivers && r->nrecv) {
     std::vector<uint32_t> tmp(r->nrecv);
@@ -791,4 +794,23 @@ extern "C" int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t*
   return ok(ctx);
 }
 
-extern "C" void wg_gathered_destroy(wg_gathered* r) { delete r; }
+namespace wg {
+void release_gathered(wg_gathered* r) {
+  if (!r->ctx) return;
+  (void)hipStreamSynchronize(r->ctx->stream);
+  dfree(r->ctx, r->recv, r->recv_bytes);
+  dfree(r->ctx, r->vals, r->vals_bytes);
+  dfree(r->ctx, r->offs, r->offs_bytes);
+  r->recv = nullptr;
+  r->vals = nullptr;
+  r->offs = nullptr;
+  r->ctx = nullptr;
+}
+}  // namespace wg
+
+extern "C" void wg_gathered_destroy(wg_gathered* r) {
+  if (!r) return;
+  if (r->ctx) r->ctx->gathereds.erase(r);
+  release_gathered(r);
+  delete r;
+}

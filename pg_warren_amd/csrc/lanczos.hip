@@ -1357,6 +1357,7 @@ using namespace wg;
 extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts* o, double* values,
                         double* residuals, wg_vector* vectors, wg_eigen_info* info) {
   if (!ctx || !gc || !o || !values || !residuals) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   if (o->matrix < 0 || o->matrix > 3)
     return fail(ctx, WG_INVALID_ARGUMENT, "unknown matrix: expected adjacency, laplacian, normalized_laplacian or random_walk");
   if (o->which != WG_LARGEST && o->which != WG_SMALLEST)

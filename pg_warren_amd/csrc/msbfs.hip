@@ -212,6 +212,7 @@ extern "C" int wg_msbfs(wg_context* ctx, const wg_graph* gc, int direction, cons
                         uint32_t nsources, int64_t max_depth, wg_vector* depths, wg_vector* reach,
                         wg_vector* farness, uint32_t* levels) {
   if (!ctx || !gc || !sources) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   wg_graph* g = const_cast<wg_graph*>(gc);
   const uint64_t n = g->n;
   if (direction != WG_OUT && direction != WG_IN) return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");

@@ -337,6 +337,7 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
                            const wg_pagerank_opts* o, const wg_vector* seed, wg_vector* result,
                            uint32_t* iterations) {
   if (!ctx || !gc || !o || !result) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   wg_graph* g = const_cast<wg_graph*>(gc);
   const uint64_t n = g->n;
   if (result->dtype != WG_F64 || result->width != 1 || result->id_space != n)
@@ -588,6 +589,7 @@ extern "C" int wg_pagerank_backward(wg_context* ctx, const wg_graph* gc, const w
                                     uint32_t* iterations) {
   if (!ctx || !gc || !o || !ranks || !dl_dranks || !dl_dweights)
     return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   wg_graph* g = const_cast<wg_graph*>(gc);
   const uint64_t n = g->n, m = g->nnz;
   if (iterations) *iterations = 0;

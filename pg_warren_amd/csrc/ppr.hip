@@ -519,6 +519,7 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
                                  const uint64_t* seed_nodes, const double* seed_weights, uint32_t R,
                                  wg_vector* ranks, uint32_t* iterations) {
   if (!ctx || !gc || !o || !seed_offsets || !ranks) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
   wg_graph* g = const_cast<wg_graph*>(gc);
   const uint64_t n = g->n;
   if (R == 0 || R > kPprMaxCols) return fail(ctx, WG_INVALID_ARGUMENT, "between 1 and 256 seed sets");

@@ -315,6 +315,7 @@ extern "C" int wg_graph_load_segments(wg_context* ctx, uint64_t id_space, const 
     destroy_graph(g);
     return s;
   }
+  ctx->graphs.insert(g);
   *out = g;
   return ok(ctx);
 }
@@ -330,6 +331,7 @@ extern "C" int wg_csr_segment_lengths(uint64_t id_space, uint64_t nnz, uint64_t*
 extern "C" int wg_graph_encode_segments(wg_context* ctx, const wg_graph* g, int reverse, uint8_t* offsets,
                                         uint8_t* targets, uint8_t* weights) {
   if (!ctx || !g) return WG_INVALID_ARGUMENT;
+  if (int cs_ = check_graph(ctx, g)) return cs_;
   if (weights && !g->weighted) return fail(ctx, WG_INVALID_ARGUMENT, "the graph has no weights");
   const DevCsr& c = reverse ? g->rev : g->fwd;
   WG_HIP(ctx, hipSetDevice(ctx->device));

@@ -5,8 +5,9 @@
 // and weight streams and a gather of x, then folds each row: short blocks
 // give each row a power-of-two group of lanes, each lane folding a
 // contiguous run of the row and the group combining the runs in order with
-// DPP/bpermute shuffles; rows longer than a block are split into pieces whose
-// partials the last piece to finish (an arrival ticket) folds in piece order.
+// DPP/bpermute shuffles; rows longer than a block are split into pieces that
+// each leave a partial, and a second kernel (spmv_long) folds every split
+// row's partials in piece order.
 // Folding contiguous runs in order keeps the reference's left-to-right
 // result exactly for every monoid but float plus.
 //
@@ -16,6 +17,7 @@
 #include <algorithm>
 
 #include "semiring.hpp"
+#include "spmv_tiles.hpp"
 #include "wg_internal.hpp"
 
 namespace wg {
@@ -57,6 +59,7 @@ enum : int {
   kVarXcd = 4,     // XCD-contiguous block order
   kVarHot = 8,     // gather from x permuted into the hot-first column order
   kVarLds = 16,    // persistent workgroups with the hottest columns' x in LDS
+  kVarTile = 32,   // the tiled kernel (spmv_tiles.hpp), alone
 };
 
 // Targets tagged for the LDS cache: rank | bit 31 for the first nhot ranks.
@@ -528,6 +531,138 @@ __global__ __launch_bounds__(256) void spmm_long(const uint32_t* __restrict__ lo
   if (e_err) raise_error(err, row, e_err);
 }
 
+__global__ void tile_clear(unsigned long long* p) { *p = ~0ull; }
+
+// The tile layout of c for elements of esize bytes (and the part all sizes
+// share), built once. WG_UNSUPPORTED when the layout does not apply (an id
+// space of 2^30 or more, no entries, or no hot-column ranking).
+static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap) {
+  if (c.tile_state < 0) return WG_UNSUPPORTED;
+  if (c.n >= (1ull << 30) || c.nnz == 0 || c.nnz >= (1ull << 40)) {
+    c.tile_state = -1;
+    return WG_UNSUPPORTED;
+  }
+  int s;
+  if (!c.hot_state && (s = prepare_hot_columns(ctx, c))) return s;
+  if (c.hot_state != 1) return WG_UNSUPPORTED;
+  hipStream_t st = ctx->stream;
+  if (c.tile_state == 0) {
+    int cus = 0;
+    WG_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const uint32_t W = (uint32_t)std::max(cus, 1) * kTileWaves;
+    const uint64_t ntiles = (c.nnz + kTileEntries - 1) / kTileEntries;
+    const uint64_t words = (c.n + 63) / 64;
+    std::vector<uint64_t> bits(words + 1, 0);
+    std::vector<uint32_t> pre(words + 1, 0);
+    std::vector<uint32_t> wave(W + 1), heads(W);
+    const std::vector<uint64_t>& off = c.h_offsets;
+    for (uint64_t r = 0; r < c.n; ++r)
+      if (off[r + 1] > off[r]) bits[r >> 6] |= 1ull << (r & 63);
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i < words; ++i) {
+      pre[i] = (uint32_t)acc;
+      acc += __builtin_popcountll(bits[i]);
+    }
+    // Each wave's tiles, and the row starts before its first entry.
+    uint64_t r = 0, starts = 0;
+    for (uint32_t g = 0; g <= W; ++g) {
+      wave[g] = (uint32_t)(ntiles * g / W);
+      if (g == W) break;
+      const uint64_t pos = (uint64_t)wave[g] * kTileEntries;
+      for (; r < c.n && off[r] < pos; ++r)
+        if (off[r + 1] > off[r]) ++starts;
+      heads[g] = (uint32_t)starts;
+    }
+    uint32_t *dw = nullptr, *dh = nullptr, *dp = nullptr;
+    uint64_t* db = nullptr;
+    if ((s = dalloc(ctx, (void**)&dw, (W + 1) * 4)) || (s = dalloc(ctx, (void**)&dh, W * 4)) ||
+        (s = dalloc(ctx, (void**)&db, (words + 1) * 8)) || (s = dalloc(ctx, (void**)&dp, (words + 1) * 4))) {
+      dfree(ctx, dw, (W + 1) * 4);
+      dfree(ctx, dh, W * 4);
+      dfree(ctx, db, (words + 1) * 8);
+      return s;
+    }
+    WG_HIP(ctx, hipMemcpyAsync(dw, wave.data(), (W + 1) * 4, hipMemcpyHostToDevice, st));
+    WG_HIP(ctx, hipMemcpyAsync(dh, heads.data(), W * 4, hipMemcpyHostToDevice, st));
+    WG_HIP(ctx, hipMemcpyAsync(db, bits.data(), (words + 1) * 8, hipMemcpyHostToDevice, st));
+    WG_HIP(ctx, hipMemcpyAsync(dp, pre.data(), (words + 1) * 4, hipMemcpyHostToDevice, st));
+    // The copies read host vectors that go out of scope here.
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    c.tile_wave = dw;
+    c.tile_heads = dh;
+    c.nz_bits = db;
+    c.nz_pre = dp;
+    c.nz_rows = acc;
+    c.tile_nwaves = W;
+    c.tile_padded = ntiles * kTileEntries;
+    c.tile_state = 1;
+  }
+  if (!c.tiles.count((int)esize)) {
+    DevCsr::Tiles t;
+    if ((s = dalloc(ctx, (void**)&t.tagged, c.tile_padded * 4))) return s;
+    t.nhot = (uint32_t)std::min<uint64_t>(c.hot_n, hotcap);
+    const unsigned grid = (unsigned)std::min<uint64_t>((c.tile_padded + 255) / 256, 16384);
+    tile_tag<<<grid, 256, 0, st>>>(c.targets, c.hot_rank, t.nhot, c.nnz, c.tile_padded, t.tagged);
+    tile_heads<<<(unsigned)std::min<uint64_t>((c.n + 255) / 256, 16384), 256, 0, st>>>(c.offsets, c.n, t.tagged);
+    WG_HIP(ctx, hipGetLastError());
+    c.tiles[(int)esize] = t;
+  }
+  return WG_OK;
+}
+
+// y = A (+.x) x through the tiled kernel; wv: the weights of this element
+// type in CSR order, or null (every weight 1).
+template <class E, int ADD, int MUL>
+static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* y, const WriteArgs& wa) {
+  using A = typename Fold<E, ADD>::A;
+  constexpr bool can_fail = std::is_same<E, int64_t>::value || std::is_same<E, uint32_t>::value;
+  hipStream_t st = ctx->stream;
+  DevCsr::Tiles& t = c.tiles[(int)sizeof(E)];
+  int s;
+  const E* wt = nullptr;
+  if (wv) {
+    const int d = DtypeOf<E>::v;
+    if (!t.w[d]) {
+      void* p = nullptr;
+      if ((s = dalloc(ctx, &p, c.tile_padded * sizeof(E)))) return s;
+      tile_weights<E><<<(unsigned)std::min<uint64_t>((c.tile_padded + 255) / 256, 16384), 256, 0, st>>>(
+          wv, c.nnz, c.tile_padded, (E*)p);
+      t.w[d] = p;
+    }
+    wt = (const E*)t.w[d];
+  }
+  const size_t ybytes = ((c.nz_rows * sizeof(A)) + 255) & ~(size_t)255;
+  const size_t need = ybytes + (size_t)c.tile_nwaves * sizeof(TileRec<A>);
+  if (need > c.tile_out_bytes) {
+    if (c.tile_out) {
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      dfree(ctx, c.tile_out, c.tile_out_bytes);
+      c.tile_out = nullptr;
+      c.tile_out_bytes = 0;
+    }
+    if ((s = dalloc(ctx, &c.tile_out, need))) return s;
+    c.tile_out_bytes = need;
+  }
+  A* ycomp = (A*)c.tile_out;
+  TileRec<A>* recs = (TileRec<A>*)((char*)c.tile_out + ybytes);
+  if (can_fail) tile_clear<<<1, 1, 0, st>>>(ctx->d_jerr);
+  constexpr bool rx = reads_x<MUL>();
+  if (rx && t.nhot)
+    gather_hot<E><<<(t.nhot + 255) / 256, 256, 0, st>>>(x, c.hot_inv, t.nhot, (E*)c.hot_x);
+  const unsigned grid = c.tile_nwaves / kTileWaves;
+  if (wt)
+    spmv_tiles<E, ADD, MUL, true><<<grid, 1024, 0, st>>>(t.tagged, wt, x, (const E*)c.hot_x, t.nhot, c.tile_wave,
+                                                         c.tile_heads, c.nnz, ycomp, recs, ctx->d_jerr);
+  else
+    spmv_tiles<E, ADD, MUL, false><<<grid, 1024, 0, st>>>(t.tagged, nullptr, x, (const E*)c.hot_x, t.nhot,
+                                                          c.tile_wave, c.tile_heads, c.nnz, ycomp, recs, ctx->d_jerr);
+  tile_fixup<E, ADD><<<(c.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, c.tile_nwaves, ycomp);
+  tile_write<E, ADD><<<(unsigned)((c.n + 255) / 256), 256, 0, st>>>(c.nz_bits, c.nz_pre, c.n, ycomp, y, wa,
+                                                                     ctx->d_jerr, ctx->d_err);
+  WG_HIP(ctx, hipGetLastError());
+  return WG_OK;
+}
+
 template <class E, int ADD, int MUL>
 int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y,
                    uint64_t width, const WriteArgs& wa) {
@@ -583,7 +718,20 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
         return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(csr.nblocks, (uint64_t)std::max(per_cu, 1) * std::max(cus, 1)));
       };
+      // The tiled kernel's layout for this element size (built once); a
+      // budget that cannot hold it leaves the other kernels.
+      auto tile_ok = [&]() -> bool {
+        const int st = prepare_tiles(ctx, csr, sizeof(E), tile_hot_cap<E>());
+        if (st == WG_OK) return true;
+        (void)ok(ctx);
+        return false;
+      };
+      int tile_status = WG_OK;
       auto launch = [&](int variant, E* out) {
+        if (variant & kVarTile) {
+          tile_status = launch_tiles<E, ADD, MUL>(ctx, csr, wv, x, out, wa);
+          return;
+        }
         const uint32_t* tg = csr.targets;
         const E* xs = x;
         if (variant & kVarLds) {
@@ -627,10 +775,16 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       const int key = ((DtypeOf<E>::v * 16 + ADD) * 16 + MUL) * 2 + (wv ? 1 : 0);
       if (ctx->spmv_variant >= 0) {
         variant = ctx->spmv_variant;
+        if (variant & kVarTile) variant = tile_ok() ? kVarTile : (variant & ~kVarTile);
         if ((variant & kVarHot) && !hot_ok()) variant &= ~kVarHot;
         if ((variant & kVarLds) && !lds_ok()) variant &= ~kVarLds;
       } else if (csr.tuned.count(key)) {
         variant = csr.tuned[key];
+      } else if (is_float_v<E> && ADD == PLUS && csr.nnz >= kTuneMinNnz) {
+        // Float plus: the tiled kernel on large graphs, chosen by size, not
+        // by timing (its sums group differently from the blocked kernel's).
+        variant = tile_ok() ? kVarTile : 0;
+        csr.tuned[key] = variant;
       } else if (csr.nnz >= kTuneMinNnz) {
         // Autotune on this graph: time every eligible variant into a scratch
         // y (the caller's y may be accumulated into), keep the fastest.
@@ -642,12 +796,15 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
         float best = 1e30f;
         const bool hot = hot_ok();
         const bool lds = hot && lds_ok();
-        for (int v = 0; v < 32; ++v) {
+        const bool tile = tile_ok();
+        for (int v = 0; v <= kVarTile; ++v) {
+          if (v == kVarTile && !tile) continue;
           if ((v & kVarDirect) && !direct_tunable) continue;
           if ((v & kVarHot) && !hot) continue;
           // The LDS kernel is persistent, LDS-staged and reads x as given.
           if ((v & kVarLds) && (!lds || (v & (kVarDirect | kVarXcd | kVarHot)))) continue;
           launch(v, (E*)ys);
+          if (tile_status) return tile_status;
           WG_HIP(ctx, hipEventRecord(e0, ctx->stream));
           for (int r = 0; r < 3; ++r) launch(v, (E*)ys);
           WG_HIP(ctx, hipEventRecord(e1, ctx->stream));
@@ -666,7 +823,8 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
         if ((s = clear_error(ctx))) return s;
       }
       launch(variant, y);
-      if (csr.nlong)
+      if (tile_status) return tile_status;
+      if (csr.nlong && !(variant & kVarTile))
         spmv_long<E, ADD><<<(csr.nlong + 255) / 256, 256, 0, ctx->stream>>>(
             csr.long_rows, csr.long_pbase, csr.nlong, part, y, wa, ctx->d_err);
     } else {
@@ -746,6 +904,10 @@ int launch_plus_times_f64(wg_context* ctx, wg_graph* g, bool transpose, bool wei
 static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
                    wg_vector* y, bool spmv) {
   if (!ctx || !g || !p || !x || !y) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  int s;
+  if ((s = check_graph(ctx, g)) || (s = check_vector(ctx, x)) || (s = check_vector(ctx, y)) ||
+      (p->mask && (s = check_vector(ctx, p->mask))))
+    return s;
   if (spmv) {
     for (uint64_t wd : {x->width, y->width})
       if (wd != 1) return fail(ctx, WG_WIDTH_MISMATCH, "width mismatch: expected 1, found " + std::to_string(wd));
@@ -767,7 +929,6 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
   if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
   WG_HIP(ctx, hipSetDevice(ctx->device));
   wg_graph* gm = const_cast<wg_graph*>(g);
-  int s;
   if (p->nrelations && !p->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
   if ((p->relations || p->per_relation) &&
       (s = select_graph(ctx, gm, p->relations, p->nrelations, p->per_relation, &gm)))
@@ -799,7 +960,7 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
 extern "C" {
 
 void wg_set_spmv_variant(wg_context* ctx, int variant) {
-  if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 31);
+  if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 63);
 }
 
 int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted) {

@@ -1,0 +1,385 @@
+// The tiled SpMV (pull over one CSR, width 1) for large graphs: persistent
+// 1024-thread workgroups, one per CU, each holding in 128 KB of LDS the x
+// values of the most gathered columns.
+//
+// Layout (built once per CSR and element size, kept with the CSR):
+//  * the entries cut into tiles of kTileEntries = 512; lane l of a wave owns
+//    entries 8l .. 8l+7 of a tile, stored interleaved (entry q of lane l at
+//    ((q / 4) * 64 + l) * 4 + q % 4) so that each 16-byte load of a wave
+//    reads 1 KB contiguously;
+//  * tagged targets: bit 31 set for a hot column (the low bits its LDS slot,
+//    its rank by gather count), else the column; bit 30 marks the first entry
+//    of a row. Edge weights in the same interleaved order, per element type.
+//  * per wave a run of tiles and the number of row starts before it; per
+//    64 rows a bitmap of the non-empty ones and their running count.
+//
+// A wave folds its tiles in order: each lane folds its eight entries left to
+// right (a row start begins a new run), a segmented scan over the lanes in
+// lane order carries the open run into the next lane, and the run open at
+// the end of a tile is carried into the next tile. A finished row's value
+// goes to slot j (the row's rank among non-empty rows) of a compact array;
+// the run a wave begins inside a row and the run it leaves open at its end
+// go to the wave's record, which tile_fixup joins in wave order. tile_write
+// then writes every row (empty ones too) through the product's mask and
+// accumulate rules. Every join combines a run with the run that follows it,
+// so every monoid but float plus gives the reference's left fold exactly
+// (compute/product/kernel.rs:100-109); float plus is grouped by runs of up
+// to eight entries and their lane-order scan.
+#pragma once
+
+#include "semiring.hpp"
+
+namespace wg {
+
+constexpr uint32_t kTileEntries = 512;
+constexpr uint32_t kTileWaves = 16;
+constexpr uint32_t kTileHotBytes = 128 * 1024;
+constexpr uint32_t kTileHot = 0x80000000u, kTileHead = 0x40000000u, kTileCol = 0x3fffffffu;
+
+template <class E>
+constexpr uint32_t tile_hot_cap() {
+  return kTileHotBytes / sizeof(E);
+}
+
+template <class A>
+struct TileRec {
+  A P;  // the run before the wave's first row start (or the whole wave)
+  A S;  // the run from the wave's last row start to its end
+  uint32_t Sj, flags;  // flags: 1 P present, 2 the wave holds a row start
+};
+
+template <class T>
+__device__ __forceinline__ T shfl_up_any(T v, unsigned delta) {
+  constexpr int W = (sizeof(T) + 3) / 4;
+  union U {
+    T t;
+    int w[W];
+    __device__ U() {}
+  } in, out;
+  in.t = v;
+#pragma unroll
+  for (int i = 0; i < W; ++i) out.w[i] = __shfl_up(in.w[i], delta, 64);
+  return out.t;
+}
+
+template <class T>
+__device__ __forceinline__ T shfl_any(T v, int src) {
+  constexpr int W = (sizeof(T) + 3) / 4;
+  union U {
+    T t;
+    int w[W];
+    __device__ U() {}
+  } in, out;
+  in.t = v;
+#pragma unroll
+  for (int i = 0; i < W; ++i) out.w[i] = __shfl(in.w[i], src, 64);
+  return out.t;
+}
+
+// Slot of entry q of lane l within its tile.
+__host__ __device__ __forceinline__ uint32_t tile_slot(uint32_t lane, uint32_t q) {
+  return ((q >> 2) * 64 + lane) * 4 + (q & 3);
+}
+
+// A lane's eight weights (16-byte loads where the element allows).
+template <class E>
+__device__ __forceinline__ void load_weights(const E* __restrict__ w, uint64_t tb, uint32_t lane, E (&wq)[8]) {
+  if constexpr (sizeof(E) == 4) {
+    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+    const v4 a = __builtin_nontemporal_load((const v4*)(w + tb + tile_slot(lane, 0)));
+    const v4 b = __builtin_nontemporal_load((const v4*)(w + tb + tile_slot(lane, 4)));
+    const uint32_t u[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) __builtin_memcpy(&wq[q], &u[q], 4);
+  } else if constexpr (sizeof(E) == 8) {
+    typedef uint64_t v2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const v2 a = __builtin_nontemporal_load((const v2*)(w + tb + tile_slot(lane, 4 * h)));
+      const v2 b = __builtin_nontemporal_load((const v2*)(w + tb + tile_slot(lane, 4 * h) + 2));
+      const uint64_t u[4] = {a.x, a.y, b.x, b.y};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) __builtin_memcpy(&wq[4 * h + q], &u[q], 8);
+    }
+  } else {
+    typedef uint32_t v1;
+    const v1 a = __builtin_nontemporal_load((const v1*)(w + tb + tile_slot(lane, 0)));
+    const v1 b = __builtin_nontemporal_load((const v1*)(w + tb + tile_slot(lane, 4)));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      wq[q] = (E)((a >> (8 * q)) & 0xff);
+      wq[4 + q] = (E)((b >> (8 * q)) & 0xff);
+    }
+  }
+}
+
+template <class E, int ADD, int MUL, bool HW>
+__global__ __launch_bounds__(1024) void spmv_tiles(
+    const uint32_t* __restrict__ tagged, const E* __restrict__ w, const E* __restrict__ x,
+    const E* __restrict__ xhot, uint32_t nhot, const uint32_t* __restrict__ wave_tiles,
+    const uint32_t* __restrict__ wave_heads, uint64_t nnz, typename Fold<E, ADD>::A* __restrict__ ycomp,
+    TileRec<typename Fold<E, ADD>::A>* __restrict__ recs, unsigned long long* __restrict__ jerr) {
+  using F = Fold<E, ADD>;
+  using A = typename F::A;
+  constexpr uint32_t HOTN = tile_hot_cap<E>();
+  constexpr bool RX = MUL != FIRST && MUL != PAIR;  // the product reads x
+  __shared__ E hot[HOTN];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if constexpr (RX) {
+    for (uint32_t i = tid; i < HOTN; i += 1024) hot[i] = i < nhot ? xhot[i] : E();
+    __syncthreads();
+  }
+  const uint32_t gw = blockIdx.x * kTileWaves + wave;
+  const uint32_t t0 = wave_tiles[gw], t1 = wave_tiles[gw + 1];
+  const uint32_t Gw = wave_heads[gw];
+  if (t0 >= t1) {
+    if (lane == 0) {
+      TileRec<A> r;
+      r.P = A{};
+      r.S = A{};
+      r.Sj = 0;
+      r.flags = 0;
+      recs[gw] = r;
+    }
+    return;
+  }
+  const uint64_t lt_mask = lane ? ((1ull << lane) - 1) : 0;
+  uint32_t G = Gw;  // row starts before the current tile
+  A C{};            // the run open at the end of the previous tile
+  bool Cp = false;
+  A Pw{};
+  bool Pwp = false;
+  uint32_t jmin = ~0u, jcode = 0;  // the smallest row (by slot) whose product failed
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint64_t tb = (uint64_t)t * kTileEntries;
+    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+    const v4 ta = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 0)));
+    const v4 tb4 = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 4)));
+    const uint32_t tq[8] = {ta.x, ta.y, ta.z, ta.w, tb4.x, tb4.y, tb4.z, tb4.w};
+    E wq[8];
+    if constexpr (HW) {
+      load_weights<E>(w, tb, lane, wq);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) wq[q] = E(1);
+    }
+    // Gathers: a hot entry reads column 0 (and drops it) so that every lane
+    // issues the same loads, all in flight together.
+    E gq[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      gq[q] = E();
+      if constexpr (RX) gq[q] = x[(tq[q] & kTileHot) ? 0u : (tq[q] & kTileCol)];
+    }
+    const uint64_t e = tb + (uint64_t)lane * 8;
+    uint32_t vm = 0xffu;
+    if (e + 8 > nnz) vm = e >= nnz ? 0u : ((1u << (uint32_t)(nnz - e)) - 1);
+    uint32_t hm = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) hm |= ((tq[q] >> 30) & 1u) << q;
+    hm &= vm;
+    const uint32_t nh = __popc(hm);
+    uint32_t Gl = G, Gt = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+      const uint64_t b = __ballot((nh >> bit) & 1);
+      Gl += __popcll(b & lt_mask) << bit;
+      Gt += __popcll(b) << bit;
+    }
+    // Products and the lane's runs, left to right.
+    A sv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      E xv = gq[q];
+      if constexpr (RX) {
+        const E hv = hot[tq[q] & (HOTN - 1)];
+        xv = (tq[q] & kTileHot) ? hv : xv;
+      }
+      uint32_t ee = 0;
+      const A pv = F::lift(multiply<E, ADD, MUL>(wq[q], xv, ee));
+      if (ee && ((vm >> q) & 1)) {
+        // The failing entry's row: the last row start at or before it.
+        const uint32_t upto = hm & ((2u << q) - 1);
+        const uint32_t j = upto ? Gl + __popc(upto) - 1 : Gl - 1;
+        if (j < jmin) {
+          jmin = j;
+          jcode = ee;
+        }
+      }
+      uint32_t ce = 0;
+      sv[q] = (q == 0 || ((hm >> q) & 1)) ? pv : F::combine(sv[q > 0 ? q - 1 : 0], pv, ce);
+    }
+    // The lane's open run: from its last row start (or its first entry) to
+    // its last entry.
+    const uint32_t nvl = vm ? 32 - __clz(vm) : 0;
+    A T = sv[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      if (nvl == (uint32_t)q + 1) T = sv[q];
+    bool Tp = nvl != 0;
+    const bool Fh = hm != 0;
+    uint32_t ce = 0;
+    if (lane == 0 && !Fh && Cp) {
+      T = Tp ? F::combine(C, T, ce) : C;
+      Tp = true;
+    }
+    // Segmented inclusive scan over lanes: flag = the lane holds a row start.
+    A X = T;
+    uint32_t fl = (Tp ? 1u : 0u) | (Fh ? 2u : 0u);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const A ov = shfl_up_any(X, d);
+      const uint32_t of = __shfl_up(fl, d, 64);
+      if (lane >= (uint32_t)d && !(fl & 2)) {
+        if (of & 1) {
+          X = (fl & 1) ? F::combine(ov, X, ce) : ov;
+          fl |= 1;
+        }
+        fl |= of & 2;
+      }
+    }
+    A Xin = shfl_up_any(X, 1);
+    bool Xinp = (__shfl_up(fl, 1, 64) & 1) != 0;
+    if (lane == 0) {
+      Xin = C;
+      Xinp = Cp;
+    }
+    if (hm) {
+      // The row ending just before the lane's first row start.
+      const int f = __ffs(hm) - 1;
+      A pre = sv[0];
+#pragma unroll
+      for (int q = 1; q < 7; ++q)
+        if (q < f) pre = sv[q];
+      A v = Xin;
+      bool vp = Xinp;
+      if (f > 0) {
+        v = vp ? F::combine(v, pre, ce) : pre;
+        vp = true;
+      }
+      if (Gl == Gw) {
+        Pw = v;
+        Pwp = vp;
+      } else if (vp) {
+        ycomp[Gl - 1] = v;
+      }
+      // Rows that start and end inside the lane.
+      const uint32_t em = (hm & (hm - 1)) >> 1;
+#pragma unroll
+      for (int q = 0; q < 7; ++q)
+        if ((em >> q) & 1) ycomp[Gl + __popc(hm & ((2u << q) - 1)) - 1] = sv[q];
+    }
+    C = shfl_any(X, 63);
+    Cp = (__shfl(fl, 63, 64) & 1) != 0;
+    G += Gt;
+  }
+  if (jmin != ~0u) raise_error(jerr, jmin, jcode);
+  const uint64_t pb = __ballot(Pwp);
+  const bool has_head = G != Gw;
+  A P{};
+  bool Pp = false;
+  if (has_head) {
+    if (pb) {
+      P = shfl_any(Pw, __ffsll((unsigned long long)pb) - 1);
+      Pp = true;
+    }
+  } else {
+    P = C;
+    Pp = Cp;
+  }
+  if (lane == 0) {
+    TileRec<A> r;
+    r.P = P;
+    r.S = has_head ? C : A{};
+    r.Sj = has_head ? G - 1 : 0u;
+    r.flags = (Pp ? 1u : 0u) | (has_head ? 2u : 0u);
+    recs[gw] = r;
+  }
+}
+
+// The rows a wave leaves open at its end, joined with the following waves'
+// leading runs in wave order.
+template <class E, int ADD>
+__global__ __launch_bounds__(256) void tile_fixup(const TileRec<typename Fold<E, ADD>::A>* __restrict__ recs,
+                                                  uint32_t nwaves, typename Fold<E, ADD>::A* __restrict__ ycomp) {
+  using F = Fold<E, ADD>;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nwaves) return;
+  const TileRec<typename F::A> r = recs[i];
+  if (!(r.flags & 2)) return;
+  typename F::A v = r.S;
+  uint32_t ce = 0;
+  for (uint32_t k = i + 1; k < nwaves; ++k) {
+    const uint32_t fk = recs[k].flags;
+    if (fk & 1) v = F::combine(v, recs[k].P, ce);
+    if (fk & 2) break;
+  }
+  ycomp[r.Sj] = v;
+}
+
+// Every row of y: a non-empty row's value from its slot, an empty row as
+// "no entry", through the mask / replace / accumulate rules.
+template <class E, int ADD>
+__global__ __launch_bounds__(256) void tile_write(const uint64_t* __restrict__ nz_bits,
+                                                  const uint32_t* __restrict__ nz_pre, uint64_t n,
+                                                  const typename Fold<E, ADD>::A* __restrict__ ycomp, E* __restrict__ y,
+                                                  WriteArgs wa, const unsigned long long* __restrict__ jerr,
+                                                  unsigned long long* __restrict__ err) {
+  using F = Fold<E, ADD>;
+  const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t wd = nz_bits[r >> 6];
+  const uint32_t b = r & 63;
+  const bool present = (wd >> b) & 1;
+  uint32_t e_err = 0;
+  E res = E();
+  if (present) {
+    const uint32_t j = nz_pre[r >> 6] + __popcll(wd & ((1ull << b) - 1));
+    res = F::finish(ycomp[j], e_err);
+    const unsigned long long je = *jerr;
+    if (je != ~0ull && (je >> 8) == j) e_err = (uint32_t)(je & 0xff);
+  }
+  write_elem<E, ADD>(wa, r, &y[r], present, res, e_err);
+  if (e_err) raise_error(err, r, e_err);
+}
+
+// ---- layout builders ----------------------------------------------------
+
+// tagged[slot of p] = p's target, tagged hot, for every entry; padding 0.
+__global__ __launch_bounds__(256) void tile_tag(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ rank,
+                                                uint32_t nhot, uint64_t nnz, uint64_t padded,
+                                                uint32_t* __restrict__ tagged) {
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
+    uint32_t v = 0;
+    if (p < nnz) {
+      const uint32_t c = tgt[p], k = rank[c];
+      v = k < nhot ? (kTileHot | k) : c;
+    }
+    const uint32_t li = (uint32_t)(p % kTileEntries);
+    tagged[p - li + tile_slot(li >> 3, li & 7)] = v;
+  }
+}
+
+// Marks each non-empty row's first entry.
+__global__ __launch_bounds__(256) void tile_heads(const uint64_t* __restrict__ off, uint64_t n,
+                                                  uint32_t* __restrict__ tagged) {
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
+    const uint64_t p = off[r];
+    if (off[r + 1] > p) {
+      const uint32_t li = (uint32_t)(p % kTileEntries);
+      tagged[p - li + tile_slot(li >> 3, li & 7)] |= kTileHead;
+    }
+  }
+}
+
+// Weights of one element type in the tile order; padding 0.
+template <class E>
+__global__ __launch_bounds__(256) void tile_weights(const E* __restrict__ w, uint64_t nnz, uint64_t padded,
+                                                    E* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
+    const uint32_t li = (uint32_t)(p % kTileEntries);
+    out[p - li + tile_slot(li >> 3, li & 7)] = p < nnz ? w[p] : E();
+  }
+}
+
+}  // namespace wg

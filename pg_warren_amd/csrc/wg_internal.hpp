@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "../../include/warren_gpu.h"
@@ -17,14 +18,15 @@ namespace wg {
 
 // A row block of the SpMV partition: either up to kBlockRows whole rows
 // whose entries total at most kBlockNnz (nrows > 0), or one piece of a row
-// too long for a block (nrows == 0, piece < npieces). Rows split into
-// several pieces combine their partials in piece order through `ticket`.
+// too long for a block (nrows == 0, piece < npieces). A split row's pieces
+// each write a partial to slot pbase + piece; spmv_long folds them in piece
+// order.
 struct Block {
   uint32_t row0;
   uint32_t nrows;
   uint32_t piece;
   uint32_t npieces;
-  uint32_t ticket;  // long rows: index of the row's arrival counter
+  uint32_t lrow;    // long rows: index of the row among the split rows
   uint32_t pbase;   // long rows: first partial slot of the row
   uint32_t pad0, pad1;
 };
@@ -88,6 +90,28 @@ struct DevCsr {
   // Targets tagged for the LDS hot-column cache, per element size: bit 31
   // and the rank for the first `nhot` ranks, the column id otherwise.
   std::map<int, std::pair<uint32_t*, uint32_t>> lds_tgt;
+  // The tiled SpMV's layout (spmv_tiles.hpp), built on first use: per
+  // element size the tagged targets in tile order and the weights of each
+  // element type in that order; per CSR the waves' tile runs and row starts
+  // before them, the non-empty-row bitmap with a count per 64 rows, and room
+  // for the finished rows and the waves' records. tile_state: 0 not built,
+  // 1 ready, -1 not available (id space of 2^30 or more, or the budget).
+  struct Tiles {
+    uint32_t* tagged = nullptr;
+    uint32_t nhot = 0;
+    void* w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  };
+  std::map<int, Tiles> tiles;
+  uint64_t tile_padded = 0;        // entries including the last tile's padding
+  uint32_t* tile_wave = nullptr;   // tile_nwaves + 1
+  uint32_t* tile_heads = nullptr;  // tile_nwaves
+  uint32_t tile_nwaves = 0;
+  uint64_t* nz_bits = nullptr;     // (n + 63) / 64 words
+  uint32_t* nz_pre = nullptr;      // (n + 63) / 64
+  uint64_t nz_rows = 0;
+  void* tile_out = nullptr;        // finished rows and wave records
+  size_t tile_out_bytes = 0;
+  int tile_state = 0;
 };
 
 constexpr uint32_t kSpmmPiece = 256;
@@ -104,6 +128,7 @@ struct wg_context {
   int last_status = 0;
   std::string last_msg;
   unsigned long long* d_err = nullptr;  // (index << 8 | status), ~0 when clear
+  unsigned long long* d_jerr = nullptr; // the tiled SpMV's error word, by row slot
   unsigned long long* h_err = nullptr;  // pinned mirror
   // Coherent pinned words a kernel posts a few results into: mail[0] the
   // sequence number, then the values; the host polls (wait_mail) instead of
@@ -118,6 +143,13 @@ struct wg_context {
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
+  // The live objects handed out through this context. wg_context_destroy
+  // releases their device memory and orphans them (their ctx becomes null),
+  // so destroying one of them afterwards only frees its host record.
+  std::set<wg_graph*> graphs;
+  std::set<wg_vector*> vectors;
+  std::set<wg_bfs_cursor*> cursors;
+  std::set<wg_gathered*> gathereds;
 };
 
 struct wg_vector {
@@ -188,8 +220,21 @@ int reverse_position_map(wg_context* ctx, const wg_graph* g, uint64_t* pos);
 // relations, or a merged selection kept with g); rels NULL: every relation.
 int select_graph(wg_context* ctx, wg_graph* g, const uint32_t* rels, uint32_t k, int per_relation,
                  wg_graph** out);
-// Frees a graph, its relations and selections.
+// Frees a graph, its relations and selections (an orphaned graph: only
+// their host records).
 void destroy_graph(wg_graph* g);
+// Frees the device memory of a graph, its relations and selections and
+// orphans them (ctx null); their host records stay for destroy_graph.
+void release_graph(wg_graph* g);
+// The same for a BFS cursor and a gathered result (bfs.hip, gather.hip).
+void release_cursor(wg_bfs_cursor* c);
+void release_gathered(wg_gathered* r);
+// Whether a cursor reads graph g (or a relation or selection of g).
+bool cursor_reads(const wg_bfs_cursor* c, const wg_graph* g);
+// WG_INVALID_ARGUMENT unless g is a live graph of ctx (a graph of another
+// context, or one orphaned by wg_context_destroy, is refused).
+int check_graph(wg_context* ctx, const wg_graph* g);
+int check_vector(wg_context* ctx, const wg_vector* v);
 // Whether two CSRs hold equal arrays (on the device).
 int same_csr(wg_context* ctx, const DevCsr& a, const DevCsr& b, bool* same);
 // row_of[p] = the row of entry p of a partitioned CSR.

@@ -174,7 +174,7 @@ ADD_OF = {"plus_times": "plus", "min_plus": "min", "max_plus": "max", "max_times
           "max_second": "max"}
 
 
-@pytest.mark.parametrize("variant", list(range(16)) + [16, 18])
+@pytest.mark.parametrize("variant", list(range(16)) + [16, 18, 32])
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool", "u32"])
 def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype, variant):
     """Integer-valued inputs: every partial sum is exact, so even float plus
@@ -203,7 +203,7 @@ def test_every_semiring_matches_the_oracle_bit_for_bit(ctx, oracle, dtype, varia
     W.set_spmv_variant(ctx, -1)
 
 
-@pytest.mark.parametrize("variant", [0, 3, 7, 8, 15, 16, 18])
+@pytest.mark.parametrize("variant", [0, 3, 7, 8, 15, 16, 18, 32])
 @pytest.mark.parametrize("dtype", ["f32", "f64", "i64", "bool"])
 def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype, variant):
     W.set_spmv_variant(ctx, variant)
@@ -230,7 +230,7 @@ def test_masks_replace_accumulate_match_the_oracle(ctx, oracle, dtype, variant):
     W.set_spmv_variant(ctx, -1)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 6, 7, 9, 14, 18])
+@pytest.mark.parametrize("variant", [0, 1, 6, 7, 9, 14, 18, 32])
 def test_float_plus_within_rounding_bound(ctx, oracle, variant):
     W.set_spmv_variant(ctx, variant)
     g = hub_graph(oracle, 5)
@@ -305,7 +305,7 @@ def test_autotuner_choice_is_a_valid_variant(ctx, oracle):
     xv = W.Vector(ctx, x)
     tuned = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
     choice = W.spmv_variant(g, "plus_times", "f32")
-    assert 0 <= choice < 32
+    assert choice == 32  # float plus above 2^22 entries: the tiled kernel
     W.set_spmv_variant(ctx, 0)
     plain = W.spmv(ctx, g, W.Product(), xv, W.Vector(ctx, np.zeros_like(x))).numpy()
     W.set_spmv_variant(ctx, -1)
@@ -313,13 +313,16 @@ def test_autotuner_choice_is_a_valid_variant(ctx, oracle):
 
 
 def test_tuned_float_plus_has_the_default_bits(ctx, oracle):
-    """Random (not integer-valued) f32 and f64 input: whatever variant the
-    tuner keeps, the result is the default launch's bits (the direct
-    kernel, whose float-plus grouping differs, is not a candidate)."""
+    """Random (not integer-valued) f32 and f64 input above 2^22 entries:
+    float plus takes the tiled kernel by size (no timing decides), so the
+    result is the forced tiled launch's bits, within the rounding bound of
+    the oracle's left fold."""
     off, tgt = W.rmat(19, 16, seed=4)
     n = len(off) - 1
     w = W.edge_weights(off, tgt, "uniform01", seed=6)
     rng = np.random.default_rng(8)
+    ref = oracle.Csr(n, off, tgt, w)
+    rev = ref.transpose()
     for dtype in ("f32", "f64"):
         g = W.Graph(ctx, n, off, tgt, w)
         x = (rng.random(n) * 2 - 1).astype(W.NP_DTYPES[dtype])
@@ -328,11 +331,49 @@ def test_tuned_float_plus_has_the_default_bits(ctx, oracle):
             prod = W.Product("plus_times", transpose=transpose)
             tuned = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
             choice = W.spmv_variant(g, "plus_times", dtype, transpose=transpose)
-            assert 0 <= choice < 32 and not choice & 1
-            W.set_spmv_variant(ctx, 0)
-            plain = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+            assert choice == 32, (dtype, transpose, choice)
+            W.set_spmv_variant(ctx, 32)
+            forced = W.spmv(ctx, g, prod, xv, W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
             W.set_spmv_variant(ctx, -1)
-            assert tuned.tobytes() == plain.tobytes(), (dtype, transpose, choice)
+            assert tuned.tobytes() == forced.tobytes(), (dtype, transpose)
+            m = rev if transpose else ref
+            want = run_oracle(oracle, ref, x, "plus_times", dtype, transpose=transpose).astype(np.float64)
+            abssum = oracle.spmv(m, np.abs(x.astype(np.float64)), "plus_times", "f64")
+            deg = np.diff(m.offsets.astype(np.int64))
+            assert np.all(np.abs(tuned.astype(np.float64) - want) <= 2 * deg * EPS[dtype] * abssum + 1e-300)
+        g.close()
+
+
+def test_tiled_kernel_across_waves_matches_the_oracle(ctx, oracle):
+    """R-MAT scale 19 (8.4M entries, about four 512-entry tiles per wave, so
+    rows cross tiles and waves, and hub rows span many waves): the tiled
+    kernel forced for every exact monoid matches the oracle bit for bit,
+    i64 plus with its prefix-overflow rule included."""
+    off, tgt = W.rmat(19, 16, seed=21)
+    n = len(off) - 1
+    w = W.edge_weights(off, tgt, "int", seed=22, p0=1, p1=9)
+    ref = oracle.Csr(n, off, tgt, w)
+    g = W.Graph(ctx, n, off, tgt, w)
+    rng = np.random.default_rng(23)
+    W.set_spmv_variant(ctx, 32)
+    try:
+        for sr, dtype in [("min_plus", "u32"), ("min_plus", "i64"), ("lor_land", "bool"),
+                          ("plus_times", "i64"), ("max_times", "f64"), ("plus_times", "f32"),
+                          ("any_pair", "i64"), ("min_second", "f32"), ("plus_pair", "u32")]:
+            x = values(dtype, n, rng, ADD_OF[sr])
+            for transpose in (False, True):
+                want = run_oracle(oracle, ref, x, sr, dtype, transpose=transpose)
+                got = W.spmv(ctx, g, W.Product(sr, transpose=transpose), W.Vector(ctx, x, dtype=dtype),
+                             W.Vector(ctx, dtype=dtype, id_space=n)).numpy()
+                assert got.tobytes() == want.tobytes(), (sr, dtype, transpose)
+        # A row whose i64 prefix sum overflows raises as the reference does.
+        x = np.zeros(n, np.int64)
+        x[tgt[off[0]:off[1]]] = I64MAX // 2
+        with pytest.raises(W.WarrenError) as e:
+            W.spmv(ctx, g, W.Product("plus_times"), W.Vector(ctx, x), W.Vector(ctx, np.zeros(n, np.int64)))
+        assert e.value.name == "bigint_out_of_range"
+    finally:
+        W.set_spmv_variant(ctx, -1)
         g.close()
 
 
@@ -344,7 +385,7 @@ def test_repeated_launches_are_bit_identical(ctx, oracle):
         assert run_gpu(ctx, g, x, "plus_times", "f32", transpose=True).tobytes() == first.tobytes()
 
 
-@pytest.mark.parametrize("variant", [8, 16, 18])
+@pytest.mark.parametrize("variant", [8, 16, 18, 32])
 def test_hot_column_variants_on_a_graph_wider_than_the_cache(ctx, oracle, variant):
     """R-MAT scale 16 has more gathered columns than the LDS cache holds, so
     the cached variant reads hot columns from LDS and the rest from x; every
