@@ -250,6 +250,24 @@ int wg_spmv(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_ve
 int wg_spmm(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
             wg_vector* y);
 
+/* wg_spmv without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, checks, errors and results as wg_spmv's default launch (the
+ * blocked kernel of graphs under 2^22 entries, whose bits it gives for every
+ * semiring and dtype); it runs that launch on any graph, never tunes and
+ * never builds a layout. Weights are read as cached for the vector's type by
+ * an earlier wg_spmv, else the f64 weights are cast per entry. The partials
+ * of rows longer than 2048 entries live in a fixed 1 MiB device array, one
+ * per GPU for the process's life, shared by every wg_spmv_noalloc call on
+ * that GPU: a graph whose split rows need more (over 131,072 row pieces
+ * of f64 partials; fewer for wider accumulators, 21,845 for i64 plus) is
+ * WG_MEMORY_LIMIT, and calls that overlap on one GPU (two
+ * contexts' streams) race. A relation selection (relations, per_relation) is
+ * WG_UNSUPPORTED: building one allocates; wg_spmv builds it. */
+int wg_spmv_noalloc(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+                    wg_vector* y);
+
 /* ---- SpMV autotuner: no counterpart in the reference (whose launch shape is
  * fixed at 4096-row tasks, compute/product/kernel.rs:15). By default (-1) the
  * first product over a graph orientation of at least 2^22 entries, for each

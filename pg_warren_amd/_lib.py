@@ -117,6 +117,7 @@ SIGNATURES = [
     ("wg_vector_info", I32, [P, P, P, P]),
     ("wg_vector_device_ptr", P, [P]),
     ("wg_spmv", I32, [P, P, P, P, P]),
+    ("wg_spmv_noalloc", I32, [P, P, P, P, P]),
     ("wg_spmm", I32, [P, P, P, P, P]),
     ("wg_set_spmv_variant", None, [P, I32]),
     ("wg_spmv_variant", I32, [P, I32, I32, I32, I32]),

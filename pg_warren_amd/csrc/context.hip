@@ -614,7 +614,8 @@ int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
       hipHostMalloc((void**)&ctx->h_err, 8, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_mail, kMailWords * 8, hipHostMallocCoherent | hipHostMallocMapped) !=
           hipSuccess ||
-      hipHostGetDevicePointer((void**)&ctx->d_mail, ctx->h_mail, 0) != hipSuccess) {
+      hipHostGetDevicePointer((void**)&ctx->d_mail, ctx->h_mail, 0) != hipSuccess ||
+      static_scratch(device, &ctx->static_scratch) != WG_OK) {
     delete ctx;
     return WG_DEVICE_ERROR;
   }

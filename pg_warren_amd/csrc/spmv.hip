@@ -15,6 +15,7 @@
 // each lane folding its columns over the row's entries in ascending order
 // (the reference's fold order); long rows split as above.
 #include <algorithm>
+#include <mutex>
 
 #include "semiring.hpp"
 #include "spmv_tiles.hpp"
@@ -108,9 +109,17 @@ struct BlockLds {
 
 // One row block: its products staged in LDS (x read through `gather`),
 // each row folded in order, split rows' pieces leaving partials.
-template <class E, int ADD, int MUL, bool HW, class Gather>
+// A stored weight as the element: the cached cast (WT = E), or the f64
+// weight cast per entry (compute/product/kernel.rs:40-47).
+template <class E, class WT>
+__device__ __forceinline__ E weight_as(WT v, uint32_t& err) {
+  if constexpr (std::is_same<WT, E>::value) return v;
+  else return weight_cast<E>(v, err);
+}
+
+template <class E, int ADD, int MUL, bool HW, class Gather, class WT = E>
 __device__ __forceinline__ void block_body(const Block& b, const uint64_t* __restrict__ off,
-                                           const uint32_t* __restrict__ tgt, const E* __restrict__ w,
+                                           const uint32_t* __restrict__ tgt, const WT* __restrict__ w,
                                            const Gather& gather, E* __restrict__ y, const WriteArgs& wa,
                                            typename Fold<E, ADD>::A* __restrict__ partials,
                                            unsigned long long* __restrict__ err, bool nt,
@@ -149,7 +158,7 @@ __device__ __forceinline__ void block_body(const Block& b, const uint64_t* __res
       wv[k] = E(1);
       if (i < cnt) {
         if constexpr (reads_x<MUL>()) col[k] = ld_stream(tgt + e0 + i, nt);
-        if constexpr (HW) wv[k] = ld_stream(w + e0 + i, nt);
+        if constexpr (HW) wv[k] = weight_as<E, WT>(ld_stream(w + e0 + i, nt), e_err);
       }
     }
     E xv[kPer];
@@ -246,10 +255,10 @@ __device__ __forceinline__ void block_body(const Block& b, const uint64_t* __res
   if (e_err) raise_error(err, b.row0, e_err);
 }
 
-template <class E, int ADD, int MUL, bool HW>
+template <class E, int ADD, int MUL, bool HW, class WT = E>
 __global__ __launch_bounds__(kBlockThreads) void spmv_blocks(
     const Block* __restrict__ blocks, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ tgt, const E* __restrict__ w, const E* __restrict__ x,
+    const uint32_t* __restrict__ tgt, const WT* __restrict__ w, const E* __restrict__ x,
     E* __restrict__ y, WriteArgs wa, typename Fold<E, ADD>::A* __restrict__ partials,
     unsigned long long* __restrict__ err, int variant) {
   __shared__ BlockLds<E, ADD> L;
@@ -768,7 +777,7 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
                 csr.blocks, csr.offsets, tg, wv, xs, out, wa, part, ctx->d_err, variant);
           else
             spmv_blocks<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, ctx->stream>>>(
-                csr.blocks, csr.offsets, tg, nullptr, xs, out, wa, part, ctx->d_err, variant);
+                csr.blocks, csr.offsets, tg, (const E*)nullptr, xs, out, wa, part, ctx->d_err, variant);
         }
       };
       int variant = 0;
@@ -901,8 +910,8 @@ int launch_plus_times_f64(wg_context* ctx, wg_graph* g, bool transpose, bool wei
 }
 
 // The checks of spmv / spmm / multiply (compute/product/mod.rs:77-153).
-static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
-                   wg_vector* y, bool spmv) {
+static int product_checks(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+                          wg_vector* y, bool spmv) {
   if (!ctx || !g || !p || !x || !y) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   int s;
   if ((s = check_graph(ctx, g)) || (s = check_vector(ctx, x)) || (s = check_vector(ctx, y)) ||
@@ -927,9 +936,16 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
   if (p->semiring < 0 || p->semiring > 12) return fail(ctx, WG_INVALID_ARGUMENT, "unknown semiring");
   if (p->accumulate < -1 || p->accumulate > 9) return fail(ctx, WG_INVALID_ARGUMENT, "unknown binary op");
   if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
+  if (p->nrelations && !p->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
+  return WG_OK;
+}
+
+static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+                   wg_vector* y, bool spmv) {
+  int s = product_checks(ctx, g, p, x, y, spmv);
+  if (s) return s;
   WG_HIP(ctx, hipSetDevice(ctx->device));
   wg_graph* gm = const_cast<wg_graph*>(g);
-  if (p->nrelations && !p->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
   if ((p->relations || p->per_relation) &&
       (s = select_graph(ctx, gm, p->relations, p->nrelations, p->per_relation, &gm)))
     return s;
@@ -955,9 +971,145 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
   return ok(ctx);
 }
 
+// ---- the product without allocation (wg_spmv_noalloc) -------------------
+// The split rows' partials of wg_spmv_noalloc live in this array: one copy
+// per device, held until the process ends, shared by every such call on
+// that device. Host code reaches it only through the address
+// hipGetSymbolAddress gives for the device (static_scratch below).
+constexpr size_t kStaticScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device; called when a context is created.
+int static_scratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return WG_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return WG_DEVICE_ERROR;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return WG_OK;
+}
+
+// The blocked kernel (launch variant 0: wg_spmv's default below 2^22
+// entries) over the CSR's own buffers; partials in the static scratch;
+// weights as cached for this element type, or the f64 weights cast per
+// entry. Sets *collect when the call can raise an error on the device.
+template <class E, int ADD, int MUL>
+static int launch_noalloc(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y, const WriteArgs& wa,
+                          bool* collect) {
+  if constexpr (std::is_same<E, uint32_t>::value && ADD == MAX) {
+    return fail(ctx, WG_UNSUPPORTED, "u32 has no image of the max identity (i64::MIN)");
+  } else {
+    using A = typename Fold<E, ADD>::A;
+    constexpr bool reads_w = MUL != SECOND && MUL != PAIR;
+    *collect = std::is_same<E, int64_t>::value || std::is_same<E, uint32_t>::value;
+    if (csr.nblocks == 0) return WG_OK;
+    const size_t need = (size_t)csr.npartials * sizeof(A);
+    if (need > kStaticScratchBytes)
+      return fail(ctx, WG_MEMORY_LIMIT,
+                  "wg_spmv_noalloc: the graph's split rows need " + std::to_string(need) +
+                      " bytes of partials, above its fixed scratch of " + std::to_string(kStaticScratchBytes));
+    A* part = (A*)ctx->static_scratch;
+    hipStream_t st = ctx->stream;
+    const int dt = DtypeOf<E>::v;
+    const bool hw = reads_w && weighted && csr.weights;
+    if (!hw) {
+      spmv_blocks<E, ADD, MUL, false><<<csr.nblocks, kBlockThreads, 0, st>>>(
+          csr.blocks, csr.offsets, csr.targets, (const E*)nullptr, x, y, wa, part, ctx->d_err, 0);
+    } else if (std::is_same<E, double>::value || (csr.wstatus[dt] == WG_OK && csr.wview[dt])) {
+      const E* w = std::is_same<E, double>::value ? (const E*)csr.weights : (const E*)csr.wview[dt];
+      spmv_blocks<E, ADD, MUL, true><<<csr.nblocks, kBlockThreads, 0, st>>>(
+          csr.blocks, csr.offsets, csr.targets, w, x, y, wa, part, ctx->d_err, 0);
+    } else if (csr.wstatus[dt] > 0) {
+      return fail(ctx, csr.wstatus[dt], "edge weight cast failed");
+    } else {
+      if constexpr (!std::is_same<E, double>::value) {
+        spmv_blocks<E, ADD, MUL, true, double><<<csr.nblocks, kBlockThreads, 0, st>>>(
+            csr.blocks, csr.offsets, csr.targets, csr.weights, x, y, wa, part, ctx->d_err, 0);
+        *collect = true;  // the cast can fail (f32 range, i64 range)
+      }
+    }
+    if (csr.nlong)
+      spmv_long<E, ADD><<<(csr.nlong + 255) / 256, 256, 0, st>>>(csr.long_rows, csr.long_pbase, csr.nlong, part,
+                                                                 y, wa, ctx->d_err);
+    WG_HIP(ctx, hipGetLastError());
+    return WG_OK;
+  }
+}
+
+template <class E>
+static int dispatch_noalloc(wg_context* ctx, DevCsr& csr, int semiring, bool weighted, const E* x, E* y,
+                            const WriteArgs& wa, bool* collect) {
+  switch (semiring) {
+    case WG_PLUS_TIMES: return launch_noalloc<E, PLUS, TIMES>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MIN_PLUS: return launch_noalloc<E, MIN, PLUS>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MAX_PLUS: return launch_noalloc<E, MAX, PLUS>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MAX_TIMES: return launch_noalloc<E, MAX, TIMES>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MIN_TIMES: return launch_noalloc<E, MIN, TIMES>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_LOR_LAND: return launch_noalloc<E, LOR, LAND>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_ANY_PAIR: return launch_noalloc<E, ANY, PAIR>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_PLUS_PAIR: return launch_noalloc<E, PLUS, PAIR>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_PLUS_FIRST: return launch_noalloc<E, PLUS, FIRST>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_PLUS_SECOND: return launch_noalloc<E, PLUS, SECOND>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MIN_FIRST: return launch_noalloc<E, MIN, FIRST>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MIN_SECOND: return launch_noalloc<E, MIN, SECOND>(ctx, csr, weighted, x, y, wa, collect);
+    case WG_MAX_SECOND: return launch_noalloc<E, MAX, SECOND>(ctx, csr, weighted, x, y, wa, collect);
+  }
+  return fail(ctx, WG_INVALID_ARGUMENT, "unknown semiring");
+}
+
+static int product_noalloc(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+                           wg_vector* y) {
+  int s = product_checks(ctx, g, p, x, y, true);
+  if (s) return s;
+  if (p->relations || p->nrelations || p->per_relation)
+    return fail(ctx, WG_UNSUPPORTED, "wg_spmv_noalloc reads the graph's own CSR; a relation selection is built "
+                                     "(allocated) by wg_spmv");
+  if (!ctx->static_scratch) return fail(ctx, WG_DEVICE_ERROR, "no static scratch on this device");
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  const DevCsr& csr = p->transpose ? g->rev : g->fwd;
+  const bool weighted = (p->weighted < 0 || p->weighted != 0) && g->weighted;
+  WriteArgs wa{p->mask ? p->mask->data : nullptr, p->mask ? p->mask->dtype : 0, p->mask_complement, p->replace,
+               p->accumulate};
+  DevCsr& c = const_cast<DevCsr&>(csr);
+  bool collect = false;
+  if ((s = clear_error(ctx))) return s;
+  switch (x->dtype) {
+    case WG_F32: s = dispatch_noalloc<float>(ctx, c, p->semiring, weighted, (const float*)x->data, (float*)y->data, wa, &collect); break;
+    case WG_F64: s = dispatch_noalloc<double>(ctx, c, p->semiring, weighted, (const double*)x->data, (double*)y->data, wa, &collect); break;
+    case WG_I64: s = dispatch_noalloc<int64_t>(ctx, c, p->semiring, weighted, (const int64_t*)x->data, (int64_t*)y->data, wa, &collect); break;
+    case WG_BOOL: s = dispatch_noalloc<uint8_t>(ctx, c, p->semiring, weighted, (const uint8_t*)x->data, (uint8_t*)y->data, wa, &collect); break;
+    case WG_U32: s = dispatch_noalloc<uint32_t>(ctx, c, p->semiring, weighted, (const uint32_t*)x->data, (uint32_t*)y->data, wa, &collect); break;
+    default: return fail(ctx, WG_INVALID_ARGUMENT, "unknown dtype");
+  }
+  if (s) return s;
+  if (collect) return collect_error(ctx, "spmv");
+  if (ctx->blocking) WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ok(ctx);
+}
+
 }  // namespace wg
 
 extern "C" {
+
+int wg_spmv_noalloc(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
+                    wg_vector* y) {
+  return wg::product_noalloc(ctx, g, p, x, y);
+}
 
 void wg_set_spmv_variant(wg_context* ctx, int variant) {
   if (ctx) ctx->spmv_variant = variant < 0 ? -1 : (variant & 63);

@@ -143,6 +143,7 @@ struct wg_context {
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
+  void* static_scratch = nullptr;  // this device's copy of wg_spmv_noalloc's partials
   // The live objects handed out through this context. wg_context_destroy
   // releases their device memory and orphans them (their ctx becomes null),
   // so destroying one of them afterwards only frees its host record.
@@ -277,6 +278,9 @@ int clear_error(wg_context* ctx);
 // Waits for the stream and turns a raised error word into a status.
 int collect_error(wg_context* ctx, const char* what);
 bool is_cancelled(wg_context* ctx);
+// The device address of wg_spmv_noalloc's static partials on `device` (the
+// current device), looked up once per device (spmv.hip).
+int static_scratch(int device, void** out);
 // The weights of `csr` cast to `dtype` (cached); returns the cast status.
 int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out);
 int prepare_spmm(wg_context* ctx, DevCsr& csr);
