@@ -1,8 +1,10 @@
 """Builds libwarren_gpu.so for gfx950 from csrc/ with hipcc (in-tree).
 
 Every HIP source is compiled with packed-FP32 instructions disabled
-(-Xclang -target-feature -Xclang -packed-fp32-ops); the objects are linked
-into pg_warren_amd/libwarren_gpu.so.
+(-Xclang -target-feature -Xclang -packed-fp32-ops) and without contracting
+a * b + c into one fused multiply-add (-ffp-contract=off: hipcc fuses by
+default), so each f64 update rounds twice, as the reference's Rust code
+computes it; the objects are linked into pg_warren_amd/libwarren_gpu.so.
 """
 from __future__ import annotations
 
@@ -18,7 +20,7 @@ LIB = os.path.join(HERE, "libwarren_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
+FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off", "-Wall",
          "-Wno-unused-function", "-Wno-unused-variable",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]

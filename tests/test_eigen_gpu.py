@@ -162,3 +162,44 @@ def test_rmat_normalized_laplacian(ctx, oracle):
     want = S.spectrum(csr, "normalized_laplacian", 12, "smallest")
     np.testing.assert_allclose(vals, want, atol=1e-9)
     assert np.all(res < 1e-8)
+
+
+def largest_component(oracle, off, tgt):
+    """The largest connected component of a symmetric CSR, relabelled 0.. in
+    id order (its rows stay ascending)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    n = len(off) - 1
+    a = sp.csr_matrix((np.ones(len(tgt)), tgt.astype(np.int64), off.astype(np.int64)), shape=(n, n))
+    _, lab = connected_components(a, directed=False)
+    keep = np.flatnonzero(lab == np.bincount(lab).argmax())
+    sub = a[keep][:, keep].tocsr()
+    sub.sort_indices()
+    return oracle.Csr(len(keep), sub.indptr.astype(np.uint64), sub.indices.astype(np.uint32))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_config_shape_32_smallest_with_a_basis_of_128(ctx, oracle, precision):
+    """The eigensolver workload's shape at scale 12: the 32 smallest
+    eigenpairs of the normalised Laplacian of symmetrised R-MAT's largest
+    component with a Krylov basis of 128 (one exact pair from the deflation,
+    31 from Lanczos, thick restarts), f64 and f32 basis storage, against the
+    dense operator's eigvalsh; eigenvectors checked as the reference checks
+    them (residual, orthonormality, sign)."""
+    off, tgt = W.rmat(12, 16, seed=5, symmetrize=True)
+    csr = largest_component(oracle, off, tgt)
+    g = upload(ctx, csr)
+    tol = 1e-10 if precision == "f64" else 1e-5
+    vals, res, vec, info = W.eigen(ctx, g, 32, matrix="normalized_laplacian", which="smallest",
+                                   precision=precision, tol=tol, basis_size=128)
+    assert info["basis_size"] == 128 and info["converged"]
+    assert info["exact"] == 1 and info["restarts"] > 0
+    want = S.spectrum(csr, "normalized_laplacian", 32, "smallest")
+    scale = 2.0  # the normalised Laplacian's spectrum lies in [0, 2]
+    atol = (1e-9 if precision == "f64" else 2e-5) * scale
+    np.testing.assert_allclose(vals, want, atol=atol)
+    if precision == "f64":
+        assert np.all(res <= 1e-8 * np.maximum(1.0, np.abs(vals)))
+        check_vectors(vec, vals, csr, "normalized_laplacian", weighted=False)
+    g.close()

@@ -147,3 +147,66 @@ def test_errors(ctx, oracle):
         wg.update_weights([3], [1.0])
     with pytest.raises(W.WarrenError):
         W.pagerank_backward(ctx, wg, x, np.ones(5))
+
+
+def test_config_shape_rmat_with_hub_rows(ctx, oracle):
+    """The differentiable-PageRank workload's shape at scale 16: R-MAT with
+    log-normal(0, 1) weights, row-normalised; PageRank personalised from 32
+    seeds; loss = sum of squared ranks over 1,000 targets; 200 iterations
+    each way. Its hub rows exceed 2048 entries in both directions, so the
+    forward and adjoint products fold split rows piece by piece. Checked
+    against the oracle's FP64 adjoint and against central differences of the
+    device's own forward run (h = 1e-6) on 20 edges, a hub row's among them."""
+    off, tgt = W.rmat(16, 16, seed=4)
+    n, m = len(off) - 1, len(tgt)
+    deg = np.diff(off.astype(np.int64))
+    indeg = np.bincount(tgt, minlength=n)
+    assert deg.max() > 2048 and indeg.max() > 2048
+    w = W.edge_weights(off, tgt, "lognormal", seed=4, p0=0.0, p1=1.0)
+    rows = np.repeat(np.arange(n), deg)
+    w = w / np.bincount(rows, weights=w, minlength=n)[rows]
+    csr = oracle.Csr(n, off, tgt, w)
+    g = W.Graph(ctx, n, off, tgt, w)
+    rng = np.random.default_rng(20)
+    seed = np.zeros(n)
+    seed[rng.choice(np.flatnonzero(deg > 0), 32, replace=False)] = 1.0
+    targets = rng.choice(n, 1000, replace=False)
+    sv = W.Vector(ctx, seed)
+    out = W.Vector(ctx, dtype="f64", id_space=n)
+
+    def forward():
+        W.pagerank(ctx, g, tol=None, max_iter=200, seed=sv, out=out)
+        xs = out.numpy()
+        return float(np.sum(xs[targets] ** 2)), xs
+
+    _, x = forward()
+    ox, oit = oracle.pagerank(csr, tol=None, max_iter=200, seed=seed)
+    assert oit == 200
+    assert np.abs(x - ox).sum() <= 1e-12
+    gx = np.zeros(n)
+    gx[targets] = 2.0 * x[targets]
+    dwv, it = W.pagerank_backward(ctx, g, x, gx, tol=None, max_iter=200, seed=seed)
+    assert it == 200
+    dw = dwv.numpy()
+    odw, _, oit = oracle.pagerank_backward(csr, x, gx, tol=None, max_iter=200, seed=seed)
+    assert oit == 200
+    scale = np.abs(odw).max()
+    assert scale > 0
+    assert np.abs(dw - odw).max() <= 1e-9 * scale
+    hub = int(np.argmax(deg))
+    hub_edges = np.arange(int(off[hub]), int(off[hub + 1]), dtype=np.int64)
+    edges = np.concatenate([hub_edges[np.argsort(-np.abs(dw[hub_edges]))[:5]],
+                            np.argsort(-np.abs(dw))[:10], rng.choice(m, 5, replace=False)])
+    h = 1e-6
+    for e in edges:
+        w0 = float(w[e])
+        g.update_weights([e], [w0 + h])
+        lp, _ = forward()
+        g.update_weights([e], [w0 - h])
+        lm, _ = forward()
+        g.update_weights([e], [w0])
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - dw[e]) <= 1e-7 * scale, (int(e), fd, dw[e])
+    for v in (sv, out, dwv):
+        v.close()
+    g.close()
