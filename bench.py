@@ -82,7 +82,10 @@ def timed(timer, fn, steps):
 
 def cpu_baseline(off, tgt, wf, x, budget_s=15.0):
     """The oracle's row-parallel pull (ROWS_PER_TASK 4096, f32 fold in
-    ascending column order) on every host core, repeated within a budget."""
+    ascending column order, the semiring fixed at compile time as
+    warren-core monomorphises it, the weight cast per entry) on this host's
+    CPU share (16 threads) within a time budget, and on one thread over a
+    few products for the per-thread rate."""
     import oracle
 
     oracle.build()
@@ -97,10 +100,93 @@ def cpu_baseline(off, tgt, wf, x, budget_s=15.0):
         el = time.perf_counter() - t0
         if el > budget_s or reps >= 50:
             break
-    return {"value": round(len(tgt) / (el / reps) / 1e9, 4), "unit": "GTEPS", "cores": threads,
+    reps1, t1 = 0, time.perf_counter()
+    while True:
+        oracle.spmv(g, x, "plus_times", "f32", y=y, threads=1)
+        reps1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 > budget_s / 3 or reps1 >= 5:
+            break
+    m = len(tgt)
+    return {"value": round(m / (el / reps) / 1e9, 4), "unit": "GTEPS", "cores": threads,
             "kind": "port",
-            "sample": f"{reps} plus_times f32 SpMVs over the full scale-22 graph, "
-                      f"{el / reps * 1e3:.1f} ms each"}
+            "sample": f"{reps} plus_times f32 SpMVs over the full scale-22 graph on {threads} threads, "
+                      f"{el / reps * 1e3:.1f} ms each; {reps1} on one thread, {el1 / reps1 * 1e3:.0f} ms each",
+            "gteps_one_thread": round(m / (el1 / reps1) / 1e9, 4),
+            "gteps_per_thread_at_full_share": round(m / (el / reps) / 1e9 / threads, 4),
+            "host_cpu_count": os.cpu_count()}
+
+
+def bench_config1(W, ctx, cpu=True, scale=16, seeds=16):
+    """The plain method's CPU configuration: R-MAT scale 16 (self loops and
+    duplicates removed), PageRank (damping 0.85, tol 1e-10 with the
+    reference's per-node threshold, at most 100 iterations, f64), then BFS
+    to depth 3 from 16 uniformly drawn seeds. Timed on the GPU and, on the
+    host, through the oracle (16 threads and one)."""
+    off, tgt = W.rmat(scale, 16, seed=1)
+    n = len(off) - 1
+    deg = np.diff(off.astype(np.int64))
+    starts = np.random.default_rng(16).choice(np.flatnonzero(deg > 0), seeds, replace=False)
+    g = W.Graph(ctx, n, off, tgt)
+    out = ctx.host_array(W._lib.Visit, n)
+    cnt, stt = C.c_uint64(), W._lib.TraversalStats()
+
+    def gpu_bfs(s):
+        st_arr = np.array([s], np.uint64)
+        ctx.check(ctx.lib.wg_bfs(ctx.h, g.h, 0, None, 0, st_arr.ctypes.data, 1, None, 3, -1, C.addressof(out),
+                                 C.byref(cnt), C.byref(stt)))
+        return cnt.value
+
+    W.pagerank(ctx, g, tol=1e-10, max_iter=100)[0].close()  # warm
+    gpu_bfs(int(starts[0]))
+    t0 = time.perf_counter()
+    ranks, it = W.pagerank(ctx, g, tol=1e-10, max_iter=100)
+    t_pr = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    visits = 0
+    for s in starts:
+        visits += gpu_bfs(int(s))
+    t_bfs = time.perf_counter() - t0
+    res = {"scale": scale, "n": n, "m": len(tgt), "pagerank_iterations": it,
+           "gpu_ms_pagerank": round(t_pr * 1e3, 2), "gpu_ms_bfs_16_depth3": round(t_bfs * 1e3, 2),
+           "bfs_visits": int(visits)}
+    gpu_ranks = ranks.numpy()
+    ranks.close()
+    g.close()
+    if cpu:
+        import oracle
+
+        ref = oracle.Csr(n, off, tgt)
+        for threads in (16, 1):
+            t0 = time.perf_counter()
+            pr, oit = oracle.pagerank(ref, tol=1e-10, max_iter=100, threads=threads)
+            tp = time.perf_counter() - t0
+            res[f"cpu_{threads}t_ms_pagerank"] = round(tp * 1e3, 2)
+        # The oracle's search is one thread (warren.bfs is too): the C call
+        # alone, its visit rows left in the ctypes buffer.
+        vis = (oracle._Visit * n)()
+        ocnt = np.zeros(1, np.uint64)
+        ostats = oracle._Stats()
+        rp = (C.POINTER(oracle._Csr) * 1)(C.pointer(ref._c))
+        t0 = time.perf_counter()
+        ov = 0
+        for s in starts:
+            st_arr = np.array([s], np.uint64)
+            code = oracle.lib().or_bfs_rows(n, C.cast(rp, C.c_void_p), 1, None, 0, st_arr.ctypes.data_as(C.c_void_p),
+                                            1, None, 3, -1, C.addressof(vis), ocnt.ctypes.data_as(C.c_void_p),
+                                            C.byref(ostats))
+            assert code == 0
+            ov += int(ocnt[0])
+        tb = time.perf_counter() - t0
+        res["cpu_ms_bfs_16_depth3"] = round(tb * 1e3, 2)
+        res["pagerank_iterations_oracle"] = oit
+        res["pagerank_l1_vs_oracle"] = float(np.abs(gpu_ranks - pr).sum())
+        res["bfs_visits_oracle"] = int(ov)
+    log(f"config 1 (s{scale}): GPU PageRank {res['gpu_ms_pagerank']} ms ({it} iterations), "
+        f"BFS x{seeds} depth 3 {res['gpu_ms_bfs_16_depth3']} ms"
+        + (f"; CPU 16 threads {res.get('cpu_16t_ms_pagerank')} ms + {res.get('cpu_ms_bfs_16_depth3')} ms"
+           if cpu else ""))
+    return res
 
 
 def power_law_seed_sets(rng, nodes, count, smax=64, alpha=2.0):
@@ -597,6 +683,10 @@ def main():
     copy_ms = timed(timer, cp, 10)
     copy_gbs = 2 * nbytes / copy_ms / 1e6
     log(f"copy kernel: {copy_gbs:.0f} GB/s")
+    # Each leg against both ceilings: the spec peak and the measured copy.
+    for leg in ("plus_times_f32", "min_plus_u32", "or_and_bool"):
+        legs[leg]["frac_of_peak"] = round(legs[leg]["gbs"] / HBM_PEAK_GBS, 4)
+        legs[leg]["frac_of_copy"] = round(legs[leg]["gbs"] / copy_gbs, 4)
     ctx.set_blocking(True)
     for name, g, sr, dt in [("plus_times_f32", gf, "plus_times", "f32"),
                             ("min_plus_u32", gi, "min_plus", "u32"),
@@ -722,6 +812,7 @@ def main():
                             "what": "host CSR (u64 offsets, u32 targets, f64 weights) copied in, "
                                     "reverse CSR transposed on the device, both partitioned"}
     legs["gather_in_f64"] = bench_gather(W, ctx, gf, n)
+    legs["config1_scale16_pagerank_bfs"] = bench_config1(W, ctx, cpu=not args.no_cpu)
     build_state = adj_state = lz_state = None
     if not args.skip_build:
         legs["csr_build"], build_state = bench_build(W, ctx, scale=args.scale)
@@ -762,6 +853,7 @@ def main():
                    "graph": "rmat", "scale": args.scale, "edge_factor": 16, "parallelism": "none"},
         "roofline": {"bound": "hbm", "achieved": head["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["gbs"] / HBM_PEAK_GBS, 4),
+                     "frac_of_measured_copy": head.get("frac_of_copy"),
                      "traffic": measured_traffic(kernel_name), "kernel": kernel_name,
                      "step_kernels": step_kernels,
                      "traffic_source": "profiles/spmv_rmat22_pmc.json (rocprofv3 FETCH_SIZE, "

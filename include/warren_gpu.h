@@ -532,8 +532,9 @@ int wg_eigen(wg_context* ctx, const wg_graph* g, const wg_eigen_opts* opts, doub
              double* residuals, wg_vector* vectors, wg_eigen_info* info);
 
 /* ---- measurement ---------------------------------------------------------
- * No counterpart in the reference: a streaming copy kernel (16-byte loads and
- * stores, grid-stride) of `bytes` from src to dst, both device buffers of
+ * No counterpart in the reference: a streaming copy kernel (16-byte
+ * non-temporal loads and stores, four loads in flight per thread, 32
+ * workgroups per CU, grid-stride) of `bytes` from src to dst, both device buffers of
  * at least that size, on the context stream. Calibrates the achievable HBM
  * bandwidth and the profiler's FETCH_SIZE / WRITE_SIZE counters. */
 int wg_device_copy(wg_context* ctx, void* dst, const void* src, uint64_t bytes);

@@ -370,13 +370,56 @@ int push(const or_csr* m, const ProductArgs& a, const E* x, E* y) {
   return OR_OK;
 }
 
+// pull_with's SpMV path (width 1, kernel.rs:100-109) with the semiring fixed
+// at compile time, as warren-core monomorphises each semiring and element
+// type (kernel.rs:51-70): the same operations in the same order as pull,
+// the weight still cast per entry (kernel.rs:40-47), without a run-time
+// dispatch or a heap buffer per element.
+template <class E, int ADD, int MUL>
+int pull_mono(const or_csr* m, const ProductArgs& a, const E* x, E* y, int threads) {
+  const uint64_t n = m->n;
+  const uint64_t tasks = (n + ROWS_PER_TASK - 1) / ROWS_PER_TASK;
+  constexpr bool reads_w = MUL != SECOND && MUL != PAIR;
+  const double* wts = a.weighted ? m->weights : nullptr;
+  return parallel_tasks(tasks, threads, [&](uint64_t task) -> int {
+    Err e;
+    const uint64_t r1 = std::min(n, (task + 1) * ROWS_PER_TASK);
+    for (uint64_t row = task * ROWS_PER_TASK; row < r1; ++row) {
+      const uint64_t p0 = m->offsets[row], p1 = m->offsets[row + 1];
+      E sum{};
+      for (uint64_t p = p0; p < p1; ++p) {
+        E w = E();
+        if (reads_w) w = Elem<E>::from_weight(wts ? wts[p] : 1.0, e);
+        const E prod = Elem<E>::multiply(ADD, MUL, w, x[m->targets[p]], e);
+        sum = p == p0 ? prod : Elem<E>::apply(ADD, sum, prod, e);
+        if (e.code) return e.code;
+      }
+      write_row<E>(a, row, y + row, p1 > p0 ? &sum : nullptr, e);
+      if (e.code) return e.code;
+    }
+    return OR_OK;
+  });
+}
+
+template <class E>
+int pull_width1(const or_csr* m, const ProductArgs& a, const E* x, E* y, int threads) {
+  if (a.width != 1) return pull<E>(m, a, x, y, threads);
+#define OR_MONO(A_, M_) \
+  if (a.add == A_ && a.mul == M_) return pull_mono<E, A_, M_>(m, a, x, y, threads);
+  OR_MONO(PLUS, TIMES) OR_MONO(MIN, PLUS) OR_MONO(MAX, PLUS) OR_MONO(MAX, TIMES) OR_MONO(MIN, TIMES)
+  OR_MONO(LOR, LAND) OR_MONO(ANY, PAIR) OR_MONO(PLUS, PAIR) OR_MONO(PLUS, FIRST) OR_MONO(PLUS, SECOND)
+  OR_MONO(MIN, FIRST) OR_MONO(MIN, SECOND) OR_MONO(MAX, SECOND)
+#undef OR_MONO
+  return pull<E>(m, a, x, y, threads);
+}
+
 template <class E>
 int spmv_typed(const or_csr* fwd, const or_csr* rev, int transpose, const ProductArgs& a,
                const void* x, void* y, int threads) {
   const E* xs = (const E*)x;
   E* ys = (E*)y;
-  if (!transpose) return pull<E>(fwd, a, xs, ys, threads);
-  if (rev) return pull<E>(rev, a, xs, ys, threads);
+  const or_csr* m = !transpose ? fwd : rev;
+  if (m) return a.width == 1 ? pull_width1<E>(m, a, xs, ys, threads) : pull<E>(m, a, xs, ys, threads);
   return push<E>(fwd, a, xs, ys);
 }
 
@@ -602,7 +645,7 @@ int or_pagerank(const or_csr* fwd, const or_csr* rev, const uint8_t* tomb, doubl
   // out = PlusFirst(A) over the forward CSR: the out-weight of each row.
   std::vector<double> out(n, 0.0), msg(n, 0.0), y(n, 0.0), shares;
   ProductArgs pf{PLUS, FIRST, 1, nullptr, 0, 0, 0, -1, 1};
-  int c = pull<double>(fwd, pf, x, out.data(), threads);
+  int c = pull_width1<double>(fwd, pf, x, out.data(), threads);
   if (c) return c;
   std::vector<double> tr;  // transposed matrix if no reverse given
   std::vector<uint64_t> roff;
@@ -664,7 +707,7 @@ int or_pagerank(const or_csr* fwd, const or_csr* rev, const uint8_t* tomb, doubl
     if (c) return c;
     double dangling = 0.0;
     for (uint64_t ch = 0; ch < chunks; ++ch) dangling += partial[ch];
-    c = pull<double>(rev, pt, msg.data(), y.data(), threads);
+    c = pull_width1<double>(rev, pt, msg.data(), y.data(), threads);
     if (c) return c;
     // advance (mod.rs:245-282)
     c = parallel_tasks(chunks, threads, [&](uint64_t ch) -> int {
@@ -882,7 +925,7 @@ int or_pagerank_backward(const or_csr* fwd, const uint8_t* tomb, double damping,
   if (e.code) return e.code;
   std::vector<double> ones(n, 1.0), out(n, 0.0), p(n, 0.0), lt(n, 0.0), u(n, 0.0);
   ProductArgs pf{PLUS, FIRST, 1, nullptr, 0, 0, 0, -1, 1};
-  int c = pull<double>(fwd, pf, ones.data(), out.data(), threads);
+  int c = pull_width1<double>(fwd, pf, ones.data(), out.data(), threads);
   if (c) return c;
   if (seed) {
     double sum = 0.0;
@@ -906,7 +949,7 @@ int or_pagerank_backward(const or_csr* fwd, const uint8_t* tomb, double damping,
   uint32_t it = 0;
   while (it < max_iter) {
     ++it;
-    c = pull<double>(fwd, pt, lt.data(), u.data(), threads);
+    c = pull_width1<double>(fwd, pt, lt.data(), u.data(), threads);
     if (c) return c;
     double change = 0.0, pin = 0.0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -920,7 +963,7 @@ int or_pagerank_backward(const or_csr* fwd, const uint8_t* tomb, double damping,
     pi = pin;
     if (has_tol && sql_cmp_f(change, bound) < 0) break;
   }
-  c = pull<double>(fwd, pt, lt.data(), u.data(), threads);
+  c = pull_width1<double>(fwd, pt, lt.data(), u.data(), threads);
   if (c) return c;
   for (uint64_t a = 0; a < n; ++a) {
     const bool flows = sql_cmp_f(out[a], 0.0) > 0;
