@@ -374,6 +374,8 @@ void free_csr(wg_context* ctx, DevCsr& c) {
   for (auto& t : c.lds_tgt) dfree(ctx, t.second.first, std::max<uint64_t>(c.nnz, 1) * 4);
   for (auto& t : c.tiles) {
     dfree(ctx, t.second.tagged, c.tile_padded * 4);
+    dfree(ctx, t.second.warm_ids, (uint64_t)t.second.nwarm * 4);
+    dfree(ctx, t.second.warm_x, (uint64_t)t.second.nwarm * t.first);
     for (int d = 0; d < 5; ++d)
       if (t.second.w[d]) dfree(ctx, t.second.w[d], c.tile_padded * dtype_size(d));
   }

@@ -81,6 +81,23 @@ __global__ __launch_bounds__(256) void gather_hot(const E* __restrict__ x, const
     xh[i] = x[inv[i]];
 }
 
+// The tiled kernel's two x copies in one launch: the hot run in rank order
+// (xh[r] = x[inv[r]]) and the warm run in ascending column order
+// (xw[j] = x[warm[j]]).
+template <class E>
+__global__ __launch_bounds__(256) void gather_tiers(const E* __restrict__ x, const uint32_t* __restrict__ inv,
+                                                    uint32_t nh, E* __restrict__ xh,
+                                                    const uint32_t* __restrict__ warm, uint32_t nw,
+                                                    E* __restrict__ xw) {
+  const uint64_t total = (uint64_t)nh + nw;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+    if (i < nh)
+      xh[i] = x[inv[i]];
+    else
+      xw[i - nh] = x[warm[i - nh]];
+  }
+}
+
 // A streamed load, non-temporal when nt (the row streams are read once and
 // should not evict the gathered x from L2 and the infinity cache).
 template <class T>
@@ -545,7 +562,7 @@ __global__ void tile_clear(unsigned long long* p) { *p = ~0ull; }
 // The tile layout of c for elements of esize bytes (and the part all sizes
 // share), built once. WG_UNSUPPORTED when the layout does not apply (an id
 // space of 2^30 or more, no entries, or no hot-column ranking).
-static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap) {
+static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap, uint32_t warmcap) {
   if (c.tile_state < 0) return WG_UNSUPPORTED;
   if (c.n >= (1ull << 30) || c.nnz == 0 || c.nnz >= (1ull << 40)) {
     c.tile_state = -1;
@@ -608,10 +625,34 @@ static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotc
   }
   if (!c.tiles.count((int)esize)) {
     DevCsr::Tiles t;
-    if ((s = dalloc(ctx, (void**)&t.tagged, c.tile_padded * 4))) return s;
     t.nhot = (uint32_t)std::min<uint64_t>(c.hot_n, hotcap);
+    // The warm run: the next ranks, while their slots (n + slot) fit the
+    // tag's column bits.
+    uint64_t nw = std::min<uint64_t>(c.hot_n - t.nhot, warmcap);
+    if (c.n + nw >= (1ull << 30)) nw = 0;
+    t.nwarm = (uint32_t)nw;
+    if (nw) {
+      std::vector<uint32_t> ids(nw);
+      WG_HIP(ctx, hipMemcpyAsync(ids.data(), c.hot_inv + t.nhot, nw * 4, hipMemcpyDeviceToHost, st));
+      WG_HIP(ctx, hipStreamSynchronize(st));
+      std::sort(ids.begin(), ids.end());
+      if ((s = dalloc(ctx, (void**)&t.warm_ids, nw * 4))) return s;
+      if ((s = dalloc(ctx, &t.warm_x, nw * esize))) {
+        dfree(ctx, t.warm_ids, nw * 4);
+        return s;
+      }
+      WG_HIP(ctx, hipMemcpyAsync(t.warm_ids, ids.data(), nw * 4, hipMemcpyHostToDevice, st));
+      // The copy reads a host vector that goes out of scope here.
+      WG_HIP(ctx, hipStreamSynchronize(st));
+    }
+    if ((s = dalloc(ctx, (void**)&t.tagged, c.tile_padded * 4))) {
+      dfree(ctx, t.warm_ids, nw * 4);
+      dfree(ctx, t.warm_x, nw * esize);
+      return s;
+    }
     const unsigned grid = (unsigned)std::min<uint64_t>((c.tile_padded + 255) / 256, 16384);
-    tile_tag<<<grid, 256, 0, st>>>(c.targets, c.hot_rank, t.nhot, c.nnz, c.tile_padded, t.tagged);
+    tile_tag<<<grid, 256, 0, st>>>(c.targets, c.hot_rank, t.nhot, t.warm_ids, t.nwarm, (uint32_t)c.n, c.nnz,
+                                   c.tile_padded, t.tagged);
     tile_heads<<<(unsigned)std::min<uint64_t>((c.n + 255) / 256, 16384), 256, 0, st>>>(c.offsets, c.n, t.tagged);
     WG_HIP(ctx, hipGetLastError());
     c.tiles[(int)esize] = t;
@@ -656,15 +697,19 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
   TileRec<A>* recs = (TileRec<A>*)((char*)c.tile_out + ybytes);
   if (can_fail) tile_clear<<<1, 1, 0, st>>>(ctx->d_jerr);
   constexpr bool rx = reads_x<MUL>();
-  if (rx && t.nhot)
-    gather_hot<E><<<(t.nhot + 255) / 256, 256, 0, st>>>(x, c.hot_inv, t.nhot, (E*)c.hot_x);
+  if (rx && (t.nhot || t.nwarm))
+    gather_tiers<E><<<(unsigned)std::min<uint64_t>(((uint64_t)t.nhot + t.nwarm + 255) / 256, 16384), 256, 0, st>>>(
+        x, c.hot_inv, t.nhot, (E*)c.hot_x, t.warm_ids, t.nwarm, (E*)t.warm_x);
   const unsigned grid = c.tile_nwaves / kTileWaves;
+  const uint32_t ncols = (uint32_t)c.n;
   if (wt)
-    spmv_tiles<E, ADD, MUL, true><<<grid, 1024, 0, st>>>(t.tagged, wt, x, (const E*)c.hot_x, t.nhot, c.tile_wave,
-                                                         c.tile_heads, c.nnz, ycomp, recs, ctx->d_jerr);
+    spmv_tiles<E, ADD, MUL, true><<<grid, 1024, 0, st>>>(t.tagged, wt, x, (const E*)c.hot_x, t.nhot,
+                                                         (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
+                                                         c.nnz, ycomp, recs, ctx->d_jerr);
   else
     spmv_tiles<E, ADD, MUL, false><<<grid, 1024, 0, st>>>(t.tagged, nullptr, x, (const E*)c.hot_x, t.nhot,
-                                                          c.tile_wave, c.tile_heads, c.nnz, ycomp, recs, ctx->d_jerr);
+                                                          (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
+                                                          c.nnz, ycomp, recs, ctx->d_jerr);
   tile_fixup<E, ADD><<<(c.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, c.tile_nwaves, ycomp);
   tile_write<E, ADD><<<(unsigned)((c.n + 255) / 256), 256, 0, st>>>(c.nz_bits, c.nz_pre, c.n, ycomp, y, wa,
                                                                      ctx->d_jerr, ctx->d_err);
@@ -730,7 +775,7 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       // The tiled kernel's layout for this element size (built once); a
       // budget that cannot hold it leaves the other kernels.
       auto tile_ok = [&]() -> bool {
-        const int st = prepare_tiles(ctx, csr, sizeof(E), tile_hot_cap<E>());
+        const int st = prepare_tiles(ctx, csr, sizeof(E), tile_hot_cap<E>(), tile_warm_cap<E>());
         if (st == WG_OK) return true;
         (void)ok(ctx);
         return false;

@@ -8,10 +8,17 @@
 //    ((q / 4) * 64 + l) * 4 + q % 4) so that each 16-byte load of a wave
 //    reads 1 KB contiguously;
 //  * tagged targets: bit 31 set for a hot column (the low bits its LDS slot,
-//    its rank by gather count), else the column; bit 30 marks the first entry
+//    its rank by gather count), else the column (or its warm slot, below);
+//    bit 30 marks the first entry
 //    of a row. Edge weights in the same interleaved order, per element type.
 //  * per wave a run of tiles and the number of row starts before it; per
 //    64 rows a bitmap of the non-empty ones and their running count.
+//
+//  * columns ranked just below the hot ones (by gather count) make up the
+//    warm run: their x copied, in ascending column order, into one buffer of
+//    kTileWarmBytes that an XCD's L2 holds beside the streams, and their
+//    entries tagged n + the column's slot in that run. The remaining columns
+//    are read from x itself.
 //
 // A wave folds its tiles in order: each lane folds its eight entries left to
 // right (a row start begins a new run), a segmented scan over the lanes in
@@ -34,11 +41,20 @@ namespace wg {
 constexpr uint32_t kTileEntries = 512;
 constexpr uint32_t kTileWaves = 16;
 constexpr uint32_t kTileHotBytes = 128 * 1024;
+#ifndef WG_TILE_WARM_BYTES
+#define WG_TILE_WARM_BYTES (2u << 20)
+#endif
+constexpr uint32_t kTileWarmBytes = WG_TILE_WARM_BYTES;
 constexpr uint32_t kTileHot = 0x80000000u, kTileHead = 0x40000000u, kTileCol = 0x3fffffffu;
 
 template <class E>
 constexpr uint32_t tile_hot_cap() {
   return kTileHotBytes / sizeof(E);
+}
+
+template <class E>
+constexpr uint32_t tile_warm_cap() {
+  return kTileWarmBytes / sizeof(E);
 }
 
 template <class A>
@@ -116,7 +132,8 @@ __device__ __forceinline__ void load_weights(const E* __restrict__ w, uint64_t t
 template <class E, int ADD, int MUL, bool HW>
 __global__ __launch_bounds__(1024) void spmv_tiles(
     const uint32_t* __restrict__ tagged, const E* __restrict__ w, const E* __restrict__ x,
-    const E* __restrict__ xhot, uint32_t nhot, const uint32_t* __restrict__ wave_tiles,
+    const E* __restrict__ xhot, uint32_t nhot, const E* __restrict__ xwarm, uint32_t ncols,
+    const uint32_t* __restrict__ wave_tiles,
     const uint32_t* __restrict__ wave_heads, uint64_t nnz, typename Fold<E, ADD>::A* __restrict__ ycomp,
     TileRec<typename Fold<E, ADD>::A>* __restrict__ recs, unsigned long long* __restrict__ jerr) {
   using F = Fold<E, ADD>;
@@ -150,128 +167,156 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
   A Pw{};
   bool Pwp = false;
   uint32_t jmin = ~0u, jcode = 0;  // the smallest row (by slot) whose product failed
-  for (uint32_t t = t0; t < t1; ++t) {
-    const uint64_t tb = (uint64_t)t * kTileEntries;
-    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
-    const v4 ta = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 0)));
-    const v4 tb4 = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 4)));
-    const uint32_t tq[8] = {ta.x, ta.y, ta.z, ta.w, tb4.x, tb4.y, tb4.z, tb4.w};
-    E wq[8];
-    if constexpr (HW) {
-      load_weights<E>(w, tb, lane, wq);
-    } else {
+  // The loads of U tiles are issued before the first of them is folded, so
+  // that their latencies overlap (a wave with fewer tiles left loads its
+  // last tile again and drops the copy).
+  constexpr uint32_t U = sizeof(E) <= 4 ? 2 : 1;
+  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+  for (uint32_t t = t0; t < t1; t += U) {
+    uint32_t TQ[U][8];
+    E WQ[U][8];
+    E GQ[U][8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) wq[q] = E(1);
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint64_t tb = (uint64_t)min(t + u, t1 - 1) * kTileEntries;
+      const v4 ta = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 0)));
+      const v4 tb4 = __builtin_nontemporal_load((const v4*)(tagged + tb + tile_slot(lane, 4)));
+      TQ[u][0] = ta.x, TQ[u][1] = ta.y, TQ[u][2] = ta.z, TQ[u][3] = ta.w;
+      TQ[u][4] = tb4.x, TQ[u][5] = tb4.y, TQ[u][6] = tb4.z, TQ[u][7] = tb4.w;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if constexpr (HW) {
+        load_weights<E>(w, (uint64_t)min(t + u, t1 - 1) * kTileEntries, lane, WQ[u]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) WQ[u][q] = E(1);
+      }
     }
     // Gathers: a hot entry reads column 0 (and drops it) so that every lane
-    // issues the same loads, all in flight together.
-    E gq[8];
+    // issues the same loads, all in flight together; a warm one reads its
+    // slot of the warm run.
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      gq[q] = E();
-      if constexpr (RX) gq[q] = x[(tq[q] & kTileHot) ? 0u : (tq[q] & kTileCol)];
-    }
-    const uint64_t e = tb + (uint64_t)lane * 8;
-    uint32_t vm = 0xffu;
-    if (e + 8 > nnz) vm = e >= nnz ? 0u : ((1u << (uint32_t)(nnz - e)) - 1);
-    uint32_t hm = 0;
+    for (uint32_t u = 0; u < U; ++u) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) hm |= ((tq[q] >> 30) & 1u) << q;
-    hm &= vm;
-    const uint32_t nh = __popc(hm);
-    uint32_t Gl = G, Gt = 0;
-#pragma unroll
-    for (int bit = 0; bit < 4; ++bit) {
-      const uint64_t b = __ballot((nh >> bit) & 1);
-      Gl += __popcll(b & lt_mask) << bit;
-      Gt += __popcll(b) << bit;
-    }
-    // Products and the lane's runs, left to right.
-    A sv[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      E xv = gq[q];
-      if constexpr (RX) {
-        const E hv = hot[tq[q] & (HOTN - 1)];
-        xv = (tq[q] & kTileHot) ? hv : xv;
-      }
-      uint32_t ee = 0;
-      const A pv = F::lift(multiply<E, ADD, MUL>(wq[q], xv, ee));
-      if (ee && ((vm >> q) & 1)) {
-        // The failing entry's row: the last row start at or before it.
-        const uint32_t upto = hm & ((2u << q) - 1);
-        const uint32_t j = upto ? Gl + __popc(upto) - 1 : Gl - 1;
-        if (j < jmin) {
-          jmin = j;
-          jcode = ee;
+      for (int q = 0; q < 8; ++q) {
+        GQ[u][q] = E();
+        if constexpr (RX) {
+          const uint32_t tg = TQ[u][q], col = tg & kTileCol;
+          const E* src = (tg & kTileHot) ? x : (col >= ncols ? xwarm + (col - ncols) : x + col);
+          GQ[u][q] = *src;
         }
       }
-      uint32_t ce = 0;
-      sv[q] = (q == 0 || ((hm >> q) & 1)) ? pv : F::combine(sv[q > 0 ? q - 1 : 0], pv, ce);
     }
-    // The lane's open run: from its last row start (or its first entry) to
-    // its last entry.
-    const uint32_t nvl = vm ? 32 - __clz(vm) : 0;
-    A T = sv[7];
 #pragma unroll
-    for (int q = 0; q < 7; ++q)
-      if (nvl == (uint32_t)q + 1) T = sv[q];
-    bool Tp = nvl != 0;
-    const bool Fh = hm != 0;
-    uint32_t ce = 0;
-    if (lane == 0 && !Fh && Cp) {
-      T = Tp ? F::combine(C, T, ce) : C;
-      Tp = true;
-    }
-    // Segmented inclusive scan over lanes: flag = the lane holds a row start.
-    A X = T;
-    uint32_t fl = (Tp ? 1u : 0u) | (Fh ? 2u : 0u);
+    for (uint32_t u = 0; u < U; ++u) {
+      if (u > 0 && t + u >= t1) break;
+      const uint64_t tb = (uint64_t)(t + u) * kTileEntries;
+      const uint32_t(&tq)[8] = TQ[u];
+      const E(&wq)[8] = WQ[u];
+      const E(&gq)[8] = GQ[u];
+      const uint64_t e = tb + (uint64_t)lane * 8;
+      uint32_t vm = 0xffu;
+      if (e + 8 > nnz) vm = e >= nnz ? 0u : ((1u << (uint32_t)(nnz - e)) - 1);
+      uint32_t hm = 0;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const A ov = shfl_up_any(X, d);
-      const uint32_t of = __shfl_up(fl, d, 64);
-      if (lane >= (uint32_t)d && !(fl & 2)) {
-        if (of & 1) {
-          X = (fl & 1) ? F::combine(ov, X, ce) : ov;
-          fl |= 1;
+      for (int q = 0; q < 8; ++q) hm |= ((tq[q] >> 30) & 1u) << q;
+      hm &= vm;
+      const uint32_t nh = __popc(hm);
+      uint32_t Gl = G, Gt = 0;
+#pragma unroll
+      for (int bit = 0; bit < 4; ++bit) {
+        const uint64_t b = __ballot((nh >> bit) & 1);
+        Gl += __popcll(b & lt_mask) << bit;
+        Gt += __popcll(b) << bit;
+      }
+      // Products and the lane's runs, left to right.
+      A sv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        E xv = gq[q];
+        if constexpr (RX) {
+          const E hv = hot[tq[q] & (HOTN - 1)];
+          xv = (tq[q] & kTileHot) ? hv : xv;
         }
-        fl |= of & 2;
+        uint32_t ee = 0;
+        const A pv = F::lift(multiply<E, ADD, MUL>(wq[q], xv, ee));
+        if (ee && ((vm >> q) & 1)) {
+          // The failing entry's row: the last row start at or before it.
+          const uint32_t upto = hm & ((2u << q) - 1);
+          const uint32_t j = upto ? Gl + __popc(upto) - 1 : Gl - 1;
+          if (j < jmin) {
+            jmin = j;
+            jcode = ee;
+          }
+        }
+        uint32_t ce = 0;
+        sv[q] = (q == 0 || ((hm >> q) & 1)) ? pv : F::combine(sv[q > 0 ? q - 1 : 0], pv, ce);
       }
-    }
-    A Xin = shfl_up_any(X, 1);
-    bool Xinp = (__shfl_up(fl, 1, 64) & 1) != 0;
-    if (lane == 0) {
-      Xin = C;
-      Xinp = Cp;
-    }
-    if (hm) {
-      // The row ending just before the lane's first row start.
-      const int f = __ffs(hm) - 1;
-      A pre = sv[0];
-#pragma unroll
-      for (int q = 1; q < 7; ++q)
-        if (q < f) pre = sv[q];
-      A v = Xin;
-      bool vp = Xinp;
-      if (f > 0) {
-        v = vp ? F::combine(v, pre, ce) : pre;
-        vp = true;
-      }
-      if (Gl == Gw) {
-        Pw = v;
-        Pwp = vp;
-      } else if (vp) {
-        ycomp[Gl - 1] = v;
-      }
-      // Rows that start and end inside the lane.
-      const uint32_t em = (hm & (hm - 1)) >> 1;
+      // The lane's open run: from its last row start (or its first entry) to
+      // its last entry.
+      const uint32_t nvl = vm ? 32 - __clz(vm) : 0;
+      A T = sv[7];
 #pragma unroll
       for (int q = 0; q < 7; ++q)
-        if ((em >> q) & 1) ycomp[Gl + __popc(hm & ((2u << q) - 1)) - 1] = sv[q];
+        if (nvl == (uint32_t)q + 1) T = sv[q];
+      bool Tp = nvl != 0;
+      const bool Fh = hm != 0;
+      uint32_t ce = 0;
+      if (lane == 0 && !Fh && Cp) {
+        T = Tp ? F::combine(C, T, ce) : C;
+        Tp = true;
+      }
+      // Segmented inclusive scan over lanes: flag = the lane holds a row start.
+      A X = T;
+      uint32_t fl = (Tp ? 1u : 0u) | (Fh ? 2u : 0u);
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const A ov = shfl_up_any(X, d);
+        const uint32_t of = __shfl_up(fl, d, 64);
+        if (lane >= (uint32_t)d && !(fl & 2)) {
+          if (of & 1) {
+            X = (fl & 1) ? F::combine(ov, X, ce) : ov;
+            fl |= 1;
+          }
+          fl |= of & 2;
+        }
+      }
+      A Xin = shfl_up_any(X, 1);
+      bool Xinp = (__shfl_up(fl, 1, 64) & 1) != 0;
+      if (lane == 0) {
+        Xin = C;
+        Xinp = Cp;
+      }
+      if (hm) {
+        // The row ending just before the lane's first row start.
+        const int f = __ffs(hm) - 1;
+        A pre = sv[0];
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+          if (q < f) pre = sv[q];
+        A v = Xin;
+        bool vp = Xinp;
+        if (f > 0) {
+          v = vp ? F::combine(v, pre, ce) : pre;
+          vp = true;
+        }
+        if (Gl == Gw) {
+          Pw = v;
+          Pwp = vp;
+        } else if (vp) {
+          ycomp[Gl - 1] = v;
+        }
+        // Rows that start and end inside the lane.
+        const uint32_t em = (hm & (hm - 1)) >> 1;
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+          if ((em >> q) & 1) ycomp[Gl + __popc(hm & ((2u << q) - 1)) - 1] = sv[q];
+      }
+      C = shfl_any(X, 63);
+      Cp = (__shfl(fl, 63, 64) & 1) != 0;
+      G += Gt;
     }
-    C = shfl_any(X, 63);
-    Cp = (__shfl(fl, 63, 64) & 1) != 0;
-    G += Gt;
   }
   if (jmin != ~0u) raise_error(jerr, jmin, jcode);
   const uint64_t pb = __ballot(Pwp);
@@ -345,15 +390,32 @@ __global__ __launch_bounds__(256) void tile_write(const uint64_t* __restrict__ n
 
 // ---- layout builders ----------------------------------------------------
 
-// tagged[slot of p] = p's target, tagged hot, for every entry; padding 0.
+// tagged[slot of p] = p's target, tagged hot or warm, for every entry;
+// padding 0. warm: the warm run's columns, ascending.
 __global__ __launch_bounds__(256) void tile_tag(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ rank,
-                                                uint32_t nhot, uint64_t nnz, uint64_t padded,
+                                                uint32_t nhot, const uint32_t* __restrict__ warm, uint32_t nwarm,
+                                                uint32_t ncols, uint64_t nnz, uint64_t padded,
                                                 uint32_t* __restrict__ tagged) {
   for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
     uint32_t v = 0;
     if (p < nnz) {
       const uint32_t c = tgt[p], k = rank[c];
-      v = k < nhot ? (kTileHot | k) : c;
+      if (k < nhot) {
+        v = kTileHot | k;
+      } else if (k - nhot < nwarm) {
+        // The column's slot in the ascending warm run.
+        uint32_t lo = 0, hi = nwarm;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (warm[mid] < c)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        v = ncols + lo;
+      } else {
+        v = c;
+      }
     }
     const uint32_t li = (uint32_t)(p % kTileEntries);
     tagged[p - li + tile_slot(li >> 3, li & 7)] = v;

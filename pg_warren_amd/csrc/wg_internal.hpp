@@ -99,6 +99,9 @@ struct DevCsr {
   struct Tiles {
     uint32_t* tagged = nullptr;
     uint32_t nhot = 0;
+    uint32_t nwarm = 0;           // columns in the warm run
+    uint32_t* warm_ids = nullptr;  // nwarm columns, ascending
+    void* warm_x = nullptr;        // their x, nwarm elements
     void* w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   };
   std::map<int, Tiles> tiles;
