@@ -101,6 +101,11 @@ void wg_cancel(wg_context* ctx);
 void wg_cancel_reset(wg_context* ctx);
 /* ComputeMemory::in_use (compute/memory.rs:97-120). */
 uint64_t wg_memory_in_use(const wg_context* ctx);
+/* A new budget for the calls that follow, as a fresh ComputeMemory::new
+ * (compute/memory.rs:47-60) would give them: later allocations are charged
+ * against it (0: no limit); bytes already held stay held, and a limit
+ * below them refuses every new allocation with WG_MEMORY_LIMIT. */
+int wg_set_memory_limit(wg_context* ctx, uint64_t limit);
 /* Waits for the context's stream and reports a deferred kernel error. */
 int wg_synchronize(wg_context* ctx);
 /* 1 (default): every call waits and reports its own errors. 0: calls whose
@@ -505,7 +510,15 @@ typedef struct {
 
 /* Eigen (lanczos.rs:174-198): steps taken, the connected components of the
  * active nodes, whether every pair met tol, how many pairs came exactly
- * from the components (deflation.rs), and the thick restarts made. */
+ * from the components (deflation.rs), and the thick restarts made.
+ * working_set (WorkingSet::bytes, working_set.rs:114-128): the bytes the
+ * call holds at its peak beyond what was in use when it began (the
+ * operator's and the components' buffers, the basis and its rotation
+ * coefficients); also set when the call is refused WG_MEMORY_LIMIT at
+ * admission. It leaves out the caches the first product over a graph
+ * builds and keeps (the context's partials scratch, the graph's layouts):
+ * once those exist, a budget of in-use + working_set bytes admits the same
+ * call again and one byte less is refused before the first step. */
 typedef struct {
   uint64_t iterations;
   uint64_t components;
@@ -513,6 +526,7 @@ typedef struct {
   uint32_t exact;
   uint32_t restarts;
   uint32_t basis_size;
+  uint64_t working_set;
 } wg_eigen_info;
 
 /* Thick-restart Lanczos with full reorthogonalisation (lanczos.rs:412-743).

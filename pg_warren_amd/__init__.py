@@ -101,6 +101,10 @@ class Context:
     def memory_in_use(self):
         return self.lib.wg_memory_in_use(self.h)
 
+    def set_memory_limit(self, limit):
+        """A new budget for later calls (0: none), as ComputeMemory::new."""
+        self.check(self.lib.wg_set_memory_limit(self.h, int(limit)))
+
     def synchronize(self):
         self.check(self.lib.wg_synchronize(self.h))
 
@@ -692,11 +696,17 @@ def eigen(ctx, graph, k, *, matrix="adjacency", which="largest", weighted=None, 
     res = np.zeros(max(int(k), 1), np.float64)
     vec = Vector(ctx, dtype=precision, id_space=graph.n, width=int(k)) if vectors and k > 0 else None
     info = _lib.EigenInfo()
-    ctx.check(ctx.lib.wg_eigen(ctx.h, graph.h, C.byref(opts), _p(vals), _p(res),
-                               None if vec is None else vec.h, C.byref(info)))
+    code = ctx.lib.wg_eigen(ctx.h, graph.h, C.byref(opts), _p(vals), _p(res),
+                            None if vec is None else vec.h, C.byref(info))
+    try:
+        ctx.check(code)
+    except WarrenError as err:
+        err.working_set = info.working_set  # set when refused at admission
+        raise
     return vals[:k], res[:k], vec, {"iterations": info.iterations, "components": info.components,
                                     "converged": bool(info.converged), "exact": info.exact,
-                                    "restarts": info.restarts, "basis_size": info.basis_size}
+                                    "restarts": info.restarts, "basis_size": info.basis_size,
+                                    "working_set": info.working_set}
 
 
 def msbfs(ctx, graph, sources, *, direction="out", max_depth=None, depths=True, reach=True,

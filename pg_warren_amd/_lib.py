@@ -74,7 +74,7 @@ class EigenInfo(C.Structure):
     """wg_eigen_info: Eigen (spectral/lanczos.rs:174-198)."""
     _fields_ = [("iterations", C.c_uint64), ("components", C.c_uint64),
                 ("converged", C.c_uint32), ("exact", C.c_uint32), ("restarts", C.c_uint32),
-                ("basis_size", C.c_uint32)]
+                ("basis_size", C.c_uint32), ("working_set", C.c_uint64)]
 
 
 # (name, restype, argtypes) of every exported entry point.
@@ -85,6 +85,7 @@ SIGNATURES = [
     ("wg_cancel", None, [P]),
     ("wg_cancel_reset", None, [P]),
     ("wg_memory_in_use", U64, [P]),
+    ("wg_set_memory_limit", I32, [P, U64]),
     ("wg_synchronize", I32, [P]),
     ("wg_set_blocking", None, [P, I32]),
     ("wg_stream", P, [P]),

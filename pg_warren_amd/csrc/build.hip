@@ -606,15 +606,27 @@ int sort_u32_keyed(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const u
   return m ? sort_pairs(ctx, kin, kout, vin, vout, m, bits) : WG_OK;
 }
 
-int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m) {
+size_t exclusive_scan_u64_bytes(uint64_t m) {
+  size_t tb = 0;
+  if (m && rocprim::exclusive_scan(nullptr, tb, (const uint64_t*)nullptr, (uint64_t*)nullptr, (uint64_t)0,
+                                   (size_t)m, rocprim::plus<uint64_t>(), (hipStream_t)0) != hipSuccess)
+    return 0;
+  return tb;
+}
+
+int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m, void* temp,
+                       size_t temp_bytes) {
   if (!m) return WG_OK;
   size_t tb = 0;
   WG_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, in, out, (uint64_t)0, (size_t)m, rocprim::plus<uint64_t>(),
                                       ctx->stream));
   Scratch<char> tmp;
-  if (int s = tmp.alloc(ctx, reusable_size(tb))) return s;
-  WG_HIP(ctx, rocprim::exclusive_scan((void*)tmp.p, tb, in, out, (uint64_t)0, (size_t)m,
-                                      rocprim::plus<uint64_t>(), ctx->stream));
+  if (!temp || temp_bytes < tb) {
+    if (int s = tmp.alloc(ctx, reusable_size(tb))) return s;
+    temp = (void*)tmp.p;
+  }
+  WG_HIP(ctx, rocprim::exclusive_scan(temp, tb, in, out, (uint64_t)0, (size_t)m, rocprim::plus<uint64_t>(),
+                                      ctx->stream));
   return WG_OK;
 }
 

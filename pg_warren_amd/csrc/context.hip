@@ -687,6 +687,12 @@ void wg_cancel_reset(wg_context* ctx) {
 }
 uint64_t wg_memory_in_use(const wg_context* ctx) { return ctx ? ctx->in_use.load() : 0; }
 
+int wg_set_memory_limit(wg_context* ctx, uint64_t limit) {
+  if (!ctx) return WG_INVALID_ARGUMENT;
+  ctx->limit = limit;
+  return wg::ok(ctx);
+}
+
 int wg_synchronize(wg_context* ctx) {
   if (!ctx) return WG_INVALID_ARGUMENT;
   return collect_error(ctx, "deferred kernel");

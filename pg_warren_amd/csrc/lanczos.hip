@@ -888,7 +888,12 @@ struct Eigen {
     // Number the roots in node order; list them.
     cc_roots<<<L.grid(n), 256, 0, st>>>(n, active, lx, flag);
     WG_HIP(ctx, hipGetLastError());
-    if ((s = exclusive_scan_u64(ctx, flag, number, n))) return s;
+    // The scan's scratch is held with the rest, so that the components pass
+    // makes no allocation the working set does not count.
+    const size_t scan_bytes = exclusive_scan_u64_bytes(n);
+    char* scan_tmp = nullptr;
+    if ((s = get(&scan_tmp, scan_bytes))) return s;
+    if ((s = exclusive_scan_u64(ctx, flag, number, n, scan_tmp, scan_bytes))) return s;
     uint64_t last[2] = {0, 0};
     WG_HIP(ctx, hipMemcpyAsync(&last[0], number + n - 1, 8, hipMemcpyDeviceToHost, st));
     WG_HIP(ctx, hipMemcpyAsync(&last[1], flag + n - 1, 8, hipMemcpyDeviceToHost, st));
@@ -1371,6 +1376,8 @@ extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts
   if (info) memset(info, 0, sizeof *info);
   if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
   WG_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t in_use_at_start = ctx->in_use.load();
+  uint64_t working_set = 0;
   Eigen e(ctx, g, *o);
   int s;
   if ((s = e.setup())) return s;
@@ -1392,9 +1399,15 @@ extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts
     if (size > kMaxBasis) return fail(ctx, WG_INVALID_ARGUMENT, "basis size above 256");
     if (size < std::min<uint64_t>(dim, (uint64_t)wanted + 1))
       return fail(ctx, WG_INVALID_ARGUMENT, "basis size must exceed the pairs Lanczos computes");
-    // Admission: the basis and its rotation coefficients, before the first step.
-    const uint64_t need = n * size * (uint64_t)(o->precision == WG_F32 ? 4 : 8) + (uint64_t)size * size * 8;
-    if (ctx->limit && ctx->in_use.load() + need > ctx->limit)
+    // Admission (working_set.rs:114-128): what the operator and the
+    // components already hold plus the basis and its rotation coefficients
+    // (as Solver::init allocates them), before the first step.
+    const uint64_t need = std::max<uint64_t>(std::max<uint64_t>(n, 1) * size * (o->precision == WG_F32 ? 4 : 8), 8) +
+                          std::max<uint64_t>((uint64_t)size * size * 8, 8);
+    const uint64_t held = ctx->in_use.load();
+    working_set = held - std::min(held, in_use_at_start) + need;
+    if (info) info->working_set = working_set;
+    if (ctx->limit && held + need > ctx->limit)
       return fail(ctx, WG_MEMORY_LIMIT, "compute memory limit exceeded: the eigenproblem needs " +
                                             std::to_string(need) + " more bytes");
   }
@@ -1486,6 +1499,7 @@ extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts
     info->exact = exact_count;
     info->restarts = restarts;
     info->basis_size = size;
+    info->working_set = working_set;
   }
   return ok(ctx);
 }

@@ -252,7 +252,11 @@ int sort_u64_keyed(wg_context* ctx, const uint64_t* kin, uint64_t* kout, const u
 int sort_u32_keyed(wg_context* ctx, const uint32_t* kin, uint32_t* kout, const uint64_t* vin, uint64_t* vout,
                    uint64_t m, unsigned bits);
 // out[i] = in[0] + ... + in[i-1] (out[0] = 0), on the context stream.
-int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m);
+// temp: device scratch of temp_bytes for the scan, or null (allocated and
+// released inside); exclusive_scan_u64_bytes(m) is the size it needs.
+int exclusive_scan_u64(wg_context* ctx, const uint64_t* in, uint64_t* out, uint64_t m, void* temp = nullptr,
+                       size_t temp_bytes = 0);
+size_t exclusive_scan_u64_bytes(uint64_t m);
 // A CSR's entries as (row, column) pairs (nnz of each).
 int csr_edges(wg_context* ctx, const DevCsr& c, uint32_t* src, uint32_t* dst);
 // The unweighted, partitioned CSR of a device edge list: parallel edges
