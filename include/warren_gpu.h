@@ -99,6 +99,25 @@ int wg_last_error(const wg_context* ctx, char* buf, size_t len);
  * comes from one thread at a time. */
 void wg_cancel(wg_context* ctx);
 void wg_cancel_reset(wg_context* ctx);
+
+/* The progress port (progress.rs:9-16): long-running work reports the phase
+ * it is in (total units, or -1 when unknown), the units done and named
+ * counters, on the thread that made the call, where it checks for
+ * cancellation. wg_eigen reports as spectral/lanczos.rs:502-629 does: phase
+ * "lanczos" with total = the step limit, then after every step and every
+ * Rayleigh-Ritz step done = steps taken and the details "converged" (pairs
+ * converged, the components' exact pairs included), "restarts" and, once
+ * estimated, "residual" (the largest wanted residual estimate over the
+ * largest Ritz value's magnitude). Reporting never changes a result. A
+ * null callback is skipped. */
+typedef struct wg_progress {
+  void (*phase)(void* user, const char* name, int64_t total);
+  void (*done)(void* user, uint64_t done);
+  void (*detail)(void* user, const char* name, double value);
+  void* user;
+} wg_progress;
+/* Sets the context's progress port (copied); null reports nowhere. */
+void wg_set_progress(wg_context* ctx, const wg_progress* port);
 /* ComputeMemory::in_use (compute/memory.rs:97-120). */
 uint64_t wg_memory_in_use(const wg_context* ctx);
 /* A new budget for the calls that follow, as a fresh ComputeMemory::new
@@ -544,6 +563,16 @@ typedef struct {
  * first step), WG_INVALID_ARGUMENT. */
 int wg_eigen(wg_context* ctx, const wg_graph* g, const wg_eigen_opts* opts, double* values,
              double* residuals, wg_vector* vectors, wg_eigen_info* info);
+
+/* The eigenpairs of a symmetric m x m matrix (m <= 256), solved on the
+ * device in f64: the projected problem of each Rayleigh-Ritz step, where
+ * the reference calls nalgebra's SymmetricEigen (spectral/ritz.rs:34-52).
+ * One workgroup reduces a to tridiagonal form (Householder) and runs
+ * implicit QR steps with the Wilkinson shift. a: row-major, host memory;
+ * values[l] and vectors[i * m + l] (column l the unit eigenvector of
+ * values[l]) are written to host memory, in the solver's order (unsorted).
+ * WG_NOT_FINITE if a value is not finite or the steps do not settle. */
+int wg_sym_eigen(wg_context* ctx, uint32_t m, const double* a, double* values, double* vectors);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference: a streaming copy kernel (16-byte

@@ -98,6 +98,21 @@ class Context:
     def cancel_reset(self):
         self.lib.wg_cancel_reset(self.h)
 
+    def set_progress(self, port):
+        """Reports of long work go to port.phase(name, total or None),
+        port.done(units) and port.detail(name, value) (progress.rs:9-16);
+        None reports nowhere."""
+        if port is None:
+            self._progress = None
+            self.lib.wg_set_progress(self.h, None)
+            return
+        c = _lib.ProgressPort(
+            _lib.PHASE_FN(lambda _u, name, total: port.phase(name.decode(), None if total < 0 else total)),
+            _lib.DONE_FN(lambda _u, done: port.done(done)),
+            _lib.DETAIL_FN(lambda _u, name, value: port.detail(name.decode(), value)), None)
+        self._progress = c  # the callbacks live as long as the port is set
+        self.lib.wg_set_progress(self.h, C.byref(c))
+
     def memory_in_use(self):
         return self.lib.wg_memory_in_use(self.h)
 
@@ -707,6 +722,21 @@ def eigen(ctx, graph, k, *, matrix="adjacency", which="largest", weighted=None, 
                                     "converged": bool(info.converged), "exact": info.exact,
                                     "restarts": info.restarts, "basis_size": info.basis_size,
                                     "working_set": info.working_set}
+
+
+def sym_eigen(ctx, a):
+    """The eigenpairs of the symmetric matrix a (m x m, m <= 256) solved on the
+    device, as each Rayleigh-Ritz step of eigen solves its projected problem
+    (spectral/ritz.rs:34-52): returns (values, vectors), vectors[:, l] the unit
+    eigenvector of values[l], in the solver's order."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    m = a.shape[0]
+    if a.shape != (m, m):
+        raise WarrenError(11, "a must be square")
+    vals = np.zeros(max(m, 1), np.float64)
+    vecs = np.zeros((max(m, 1), max(m, 1)), np.float64)
+    ctx.check(ctx.lib.wg_sym_eigen(ctx.h, m, _p(a), _p(vals), _p(vecs)))
+    return vals[:m], vecs[:m, :m]
 
 
 def msbfs(ctx, graph, sources, *, direction="out", max_depth=None, depths=True, reach=True,

@@ -77,6 +77,16 @@ class EigenInfo(C.Structure):
                 ("basis_size", C.c_uint32), ("working_set", C.c_uint64)]
 
 
+PHASE_FN = C.CFUNCTYPE(None, P, C.c_char_p, C.c_int64)
+DONE_FN = C.CFUNCTYPE(None, P, C.c_uint64)
+DETAIL_FN = C.CFUNCTYPE(None, P, C.c_char_p, C.c_double)
+
+
+class ProgressPort(C.Structure):
+    """wg_progress: the Progress port (progress.rs:9-16)."""
+    _fields_ = [("phase", PHASE_FN), ("done", DONE_FN), ("detail", DETAIL_FN), ("user", P)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("wg_context_create", I32, [I32, U64, P]),
@@ -84,6 +94,7 @@ SIGNATURES = [
     ("wg_last_error", I32, [P, C.c_char_p, C.c_size_t]),
     ("wg_cancel", None, [P]),
     ("wg_cancel_reset", None, [P]),
+    ("wg_set_progress", None, [P, P]),
     ("wg_memory_in_use", U64, [P]),
     ("wg_set_memory_limit", I32, [P, U64]),
     ("wg_synchronize", I32, [P]),
@@ -126,6 +137,7 @@ SIGNATURES = [
     ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
     ("wg_pagerank_backward", I32, [P, P, P, P, P, P, P, P, P, P]),
     ("wg_eigen", I32, [P, P, P, P, P, P, P]),
+    ("wg_sym_eigen", I32, [P, U32, P, P, P]),
     ("wg_bfs_levels", I32, [P, P, I32, P, U32, P, U64, P, I64, I32, P, P, P]),
     ("wg_bfs", I32, [P, P, I32, P, U32, P, U64, P, I64, I64, P, P, P]),
     ("wg_reachable_set", I32, [P, P, I32, P, U32, P, U64, I64, P, P, P]),

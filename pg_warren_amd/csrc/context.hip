@@ -203,7 +203,10 @@ int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out) {
   void* p = nullptr;
   int s = dalloc(ctx, &p, csr.nnz * es);
   if (s) return s;
-  if ((s = clear_error(ctx))) return s;
+  if ((s = clear_error(ctx))) {
+    dfree(ctx, p, csr.nnz * es);
+    return s;
+  }
   uint64_t m = csr.nnz;
   unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 65536);
   if (grid == 0) grid = 1;
@@ -213,8 +216,17 @@ int weight_view(wg_context* ctx, DevCsr& csr, int dtype, const void** out) {
     case WG_BOOL: cast_weights<uint8_t><<<grid, 256, 0, ctx->stream>>>(csr.weights, (uint8_t*)p, m, ctx->d_err); break;
     case WG_U32: cast_weights<uint32_t><<<grid, 256, 0, ctx->stream>>>(csr.weights, (uint32_t*)p, m, ctx->d_err); break;
   }
-  WG_HIP(ctx, hipGetLastError());
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    dfree(ctx, p, csr.nnz * es);
+    return hip_fail(ctx, le, "cast_weights launch");
+  }
   s = collect_error(ctx, "edge weight cast");
+  if (s == WG_DEVICE_ERROR || s == WG_OUT_OF_MEMORY) {
+    // A runtime failure, not the cast's own verdict: nothing is cached.
+    dfree(ctx, p, csr.nnz * es);
+    return s;
+  }
   csr.wview[dtype] = p;
   csr.wstatus[dtype] = s;
   *out = p;
@@ -458,9 +470,17 @@ int prepare_spmm(wg_context* ctx, DevCsr& c) {
   c.spmm_nlong = (uint32_t)lrows.size();
   c.spmm_npartials = np_total;
   int s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_tasks, std::max<uint64_t>(ntasks, 1) * sizeof(SpmmTask)))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_pbase, (c.spmm_nlong + 1) * 4))) return s;
-  if ((s = dalloc(ctx, (void**)&c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) return s;
+  if ((s = dalloc(ctx, (void**)&c.spmm_tasks, std::max<uint64_t>(ntasks, 1) * sizeof(SpmmTask))) ||
+      (s = dalloc(ctx, (void**)&c.spmm_pbase, (c.spmm_nlong + 1) * 4)) ||
+      (s = dalloc(ctx, (void**)&c.spmm_long_rows, std::max<uint32_t>(c.spmm_nlong, 1) * 4))) {
+    // Nothing stays charged from a failed preparation (dfree skips null).
+    dfree(ctx, c.spmm_tasks, std::max<uint64_t>(ntasks, 1) * sizeof(SpmmTask));
+    dfree(ctx, c.spmm_pbase, (c.spmm_nlong + 1) * 4);
+    c.spmm_tasks = nullptr;
+    c.spmm_pbase = nullptr;
+    c.spmm_long_rows = nullptr;
+    return s;
+  }
   if (ntasks)
     WG_HIP(ctx, hipMemcpyAsync(c.spmm_tasks, tasks.data(), ntasks * sizeof(SpmmTask), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -621,6 +641,7 @@ int wg_context_create(int device, uint64_t memory_limit, wg_context** out) {
     delete ctx;
     return WG_DEVICE_ERROR;
   }
+  memset(ctx->h_mail, 0, kMailWords * 8);  // the cancel word starts lowered
   *out = ctx;
   return WG_OK;
 }
@@ -680,10 +701,19 @@ void wg_host_free(void* p) {
 }
 
 void wg_cancel(wg_context* ctx) {
-  if (ctx) ctx->cancelled = 1;
+  if (!ctx) return;
+  ctx->cancelled = 1;
+  if (ctx->h_mail) __atomic_store_n(&ctx->h_mail[kMailCancel], 1ull, __ATOMIC_RELEASE);
 }
 void wg_cancel_reset(wg_context* ctx) {
-  if (ctx) ctx->cancelled = 0;
+  if (!ctx) return;
+  ctx->cancelled = 0;
+  if (ctx->h_mail) __atomic_store_n(&ctx->h_mail[kMailCancel], 0ull, __ATOMIC_RELEASE);
+}
+
+void wg_set_progress(wg_context* ctx, const wg_progress* port) {
+  if (!ctx) return;
+  ctx->progress = port ? *port : wg_progress{};
 }
 uint64_t wg_memory_in_use(const wg_context* ctx) { return ctx ? ctx->in_use.load() : 0; }
 

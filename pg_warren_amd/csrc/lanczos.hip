@@ -463,116 +463,211 @@ __global__ __launch_bounds__(256) void store_column(const double* __restrict__ x
     out[i * width + col] = round_to<E>(x[i]);
 }
 
-// ---- host: the projected eigenproblem -------------------------------------------
+// ---- the projected eigenproblem, on the device ---------------------------------
 
-// Eigenpairs of the symmetric m x m matrix a (row-major) in f64:
-// values[l] and vectors[i * m + l] (column l). Householder reduction to a
-// tridiagonal T = Q^T A Q, then implicit QR steps with the Wilkinson shift
-// on T (Golub & Van Loan, algorithms 8.3.1 and 8.3.3), the Givens
-// rotations accumulated into Q.
-static bool sym_eigen(std::vector<double> a, int m, std::vector<double>& values, std::vector<double>& vectors) {
+// Eigenpairs of the symmetric m x m matrix held row-major in a (f64, m <=
+// kMaxBasis), by one workgroup (SymmetricEigen of ritz.rs:34-52): Householder
+// reduction to a tridiagonal T = Q^T A Q, then implicit QR steps with the
+// Wilkinson shift on T (Golub & Van Loan, algorithms 8.3.1 and 8.3.3), the
+// Givens rotations accumulated into Q. a is overwritten; qt receives Q^T
+// (row l = the eigenvector of values[l]); *status 0 on success, 1 when a
+// value is not finite or the QR steps do not settle. Reductions are in a
+// fixed order, so the result does not vary from run to run.
+constexpr int kEigThreads = 256;
+
+// Doubles of device workspace device_sym_eigen needs for an m x m matrix.
+constexpr uint64_t eigen_workspace(uint64_t m) { return 2 * m * m + m + 1; }
+
+__device__ double eig_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+  __syncthreads();
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kEigThreads / 64; ++k) t += sh[k];
+  return t;
+}
+
+__global__ __launch_bounds__(kEigThreads) void sym_eigen_dev(double* __restrict__ a, double* __restrict__ qt,
+                                                             int m, double* __restrict__ values,
+                                                             int* __restrict__ status) {
+  __shared__ double v[kMaxBasis], p[kMaxBasis], w[kMaxBasis], t[kMaxBasis];
+  __shared__ double d[kMaxBasis], e[kMaxBasis], cs[kMaxBasis], sn[kMaxBasis];
+  __shared__ double sh[kEigThreads / 64];
+  __shared__ int ctl[4];  // l, h, done, failed
+  const int tid = threadIdx.x;
   auto A = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
-  // The orthogonal factor is kept transposed (row k of qt = column k of
-  // Q), so the reflections and rotations sweep contiguous rows.
-  std::vector<double> qt((size_t)m * m, 0.0);
   auto Qt = [&](int k, int r) -> double& { return qt[(size_t)k * m + r]; };
-  for (int i = 0; i < m; ++i) Qt(i, i) = 1.0;
-  for (double v : a)
-    if (!std::isfinite(v)) return false;
-  std::vector<double> v(m), p(m), w(m), t(m);
+  int bad = 0;
+  for (int idx = tid; idx < m * m; idx += kEigThreads) {
+    Qt(idx / m, idx % m) = (idx / m == idx % m) ? 1.0 : 0.0;
+    if (!isfinite(a[idx])) bad = 1;
+  }
+  if (tid == 0) ctl[3] = 0;
+  __syncthreads();
+  if (bad) ctl[3] = 1;
+  __syncthreads();
+  if (ctl[3]) {
+    if (tid == 0) *status = 1;
+    return;
+  }
   for (int k = 0; k + 2 < m; ++k) {
     const int len = m - k - 1;
-    double norm2 = 0.0;
-    for (int i = 0; i < len; ++i) {
+    double q = 0.0;
+    for (int i = tid; i < len; i += kEigThreads) {
       v[i] = A(k + 1 + i, k);
-      norm2 += v[i] * v[i];
+      q += v[i] * v[i];
     }
+    const double norm2 = eig_sum(q, sh);
     if (norm2 == 0.0) continue;
-    const double alpha = v[0] > 0.0 ? -std::sqrt(norm2) : std::sqrt(norm2);
-    v[0] -= alpha;
-    double vv = 0.0;
-    for (int i = 0; i < len; ++i) vv += v[i] * v[i];
+    const double alpha = v[0] > 0.0 ? -sqrt(norm2) : sqrt(norm2);
+    __syncthreads();
+    if (tid == 0) v[0] -= alpha;
+    __syncthreads();
+    q = 0.0;
+    for (int i = tid; i < len; i += kEigThreads) q += v[i] * v[i];
+    const double vv = eig_sum(q, sh);
     if (vv == 0.0) continue;
     const double beta = 2.0 / vv;
-    // A22 -= v w^T + w v^T with p = beta A22 v, w = p - (beta p.v / 2) v.
-    double pv = 0.0;
-    for (int i = 0; i < len; ++i) {
-      const double* row = &A(k + 1 + i, k + 1);
+    // p = beta A22 v; A22 is symmetric, so thread i walks column i (each
+    // row of A22 read contiguously across the threads).
+    q = 0.0;
+    for (int i = tid; i < len; i += kEigThreads) {
       double sum = 0.0;
-      for (int j = 0; j < len; ++j) sum += row[j] * v[j];
+      for (int j = 0; j < len; ++j) sum += A(k + 1 + j, k + 1 + i) * v[j];
       p[i] = beta * sum;
-      pv += p[i] * v[i];
+      q += p[i] * v[i];
     }
+    const double pv = eig_sum(q, sh);
     const double kk = beta * pv / 2.0;
-    for (int i = 0; i < len; ++i) w[i] = p[i] - kk * v[i];
-    for (int i = 0; i < len; ++i) {
-      double* row = &A(k + 1 + i, k + 1);
-      const double vi = v[i], wi = w[i];
-      for (int j = 0; j < len; ++j) row[j] -= vi * w[j] + wi * v[j];
+    for (int i = tid; i < len; i += kEigThreads) w[i] = p[i] - kk * v[i];
+    __syncthreads();
+    // A22 -= v w^T + w v^T (stays exactly symmetric).
+    for (int idx = tid; idx < len * len; idx += kEigThreads) {
+      const int i = idx / len, j = idx % len;
+      A(k + 1 + i, k + 1 + j) -= v[i] * w[j] + w[i] * v[j];
     }
-    A(k + 1, k) = A(k, k + 1) = alpha;
-    for (int i = 1; i < len; ++i) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
-    // Q[:, k+1:] (I - beta v v^T), on the rows of qt.
-    std::fill(t.begin(), t.end(), 0.0);
-    for (int j = 0; j < len; ++j) {
-      const double* row = &Qt(k + 1 + j, 0);
-      const double vj = v[j];
-      for (int r = 0; r < m; ++r) t[r] += row[r] * vj;
+    for (int i = tid; i < len; i += kEigThreads) {
+      const double val = i == 0 ? alpha : 0.0;
+      A(k + 1 + i, k) = val;
+      A(k, k + 1 + i) = val;
     }
-    for (int j = 0; j < len; ++j) {
-      double* row = &Qt(k + 1 + j, 0);
-      const double bv = beta * v[j];
-      for (int r = 0; r < m; ++r) row[r] -= bv * t[r];
+    // Q[:, k+1:] (I - beta v v^T), on the rows of qt: t = rows^T v.
+    for (int r = tid; r < m; r += kEigThreads) {
+      double sum = 0.0;
+      for (int j = 0; j < len; ++j) sum += Qt(k + 1 + j, r) * v[j];
+      t[r] = sum;
     }
+    __syncthreads();
+    for (int idx = tid; idx < len * m; idx += kEigThreads) {
+      const int j = idx / m, r = idx % m;
+      Qt(k + 1 + j, r) -= (beta * v[j]) * t[r];
+    }
+    __syncthreads();
   }
-  std::vector<double> d(m), e(std::max(m - 1, 1), 0.0);
-  for (int i = 0; i < m; ++i) d[i] = A(i, i);
-  for (int i = 0; i + 1 < m; ++i) e[i] = A(i, i + 1);
-  const double eps = std::numeric_limits<double>::epsilon();
-  int h = m - 1, steps = 0;
-  while (h > 0) {
-    if (std::fabs(e[h - 1]) <= eps * (std::fabs(d[h - 1]) + std::fabs(d[h]))) {
-      e[h - 1] = 0.0;
-      --h;
-      continue;
-    }
-    int l = h - 1;
-    while (l > 0 && std::fabs(e[l - 1]) > eps * (std::fabs(d[l - 1]) + std::fabs(d[l]))) --l;
-    if (l > 0) e[l - 1] = 0.0;
-    if (++steps > 60 * m) return false;
-    const double delta = (d[h - 1] - d[h]) / 2.0;
-    const double eh = e[h - 1];
-    const double mu = d[h] - eh * eh / (delta + (delta >= 0.0 ? 1.0 : -1.0) * std::hypot(delta, eh));
-    double x = d[l] - mu, z = e[l];
-    for (int k = l; k < h; ++k) {
-      const double r = std::hypot(x, z);
-      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
-      if (k > l) e[k - 1] = r;
-      const double dk = d[k], ek = e[k], dk1 = d[k + 1];
-      d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
-      d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
-      e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
-      if (k + 1 < h) {
-        z = s * e[k + 1];
-        e[k + 1] = c * e[k + 1];
-        x = e[k];
-      }
-      double* qa = &Qt(k, 0);
-      double* qb = &Qt(k + 1, 0);
-      for (int r2 = 0; r2 < m; ++r2) {
-        const double u0 = qa[r2], u1 = qb[r2];
-        qa[r2] = c * u0 + s * u1;
-        qb[r2] = -s * u0 + c * u1;
-      }
-    }
+  for (int i = tid; i < m; i += kEigThreads) {
+    d[i] = A(i, i);
+    e[i] = i + 1 < m ? A(i, i + 1) : 0.0;
   }
-  values = d;
-  for (double val : values)
-    if (!std::isfinite(val)) return false;
-  vectors.resize((size_t)m * m);
+  if (tid == 0) {
+    ctl[1] = m - 1;
+    ctl[2] = 0;
+  }
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  int steps = 0;
+  for (;;) {
+    if (tid == 0) {
+      int h = ctl[1];
+      while (h > 0 && fabs(e[h - 1]) <= eps * (fabs(d[h - 1]) + fabs(d[h]))) {
+        e[h - 1] = 0.0;
+        --h;
+      }
+      ctl[1] = h;
+      if (h == 0) {
+        ctl[2] = 1;
+      } else if (++steps > 60 * m) {
+        ctl[2] = 1;
+        ctl[3] = 1;
+      } else {
+        int l = h - 1;
+        while (l > 0 && fabs(e[l - 1]) > eps * (fabs(d[l - 1]) + fabs(d[l]))) --l;
+        if (l > 0) e[l - 1] = 0.0;
+        ctl[0] = l;
+        const double delta = (d[h - 1] - d[h]) / 2.0;
+        const double eh = e[h - 1];
+        const double mu = d[h] - eh * eh / (delta + (delta >= 0.0 ? 1.0 : -1.0) * hypot(delta, eh));
+        double x = d[l] - mu, z = e[l];
+        for (int k = l; k < h; ++k) {
+          const double r = hypot(x, z);
+          const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
+          if (k > l) e[k - 1] = r;
+          const double dk = d[k], ek = e[k], dk1 = d[k + 1];
+          d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
+          d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
+          e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
+          if (k + 1 < h) {
+            z = s * e[k + 1];
+            e[k + 1] = c * e[k + 1];
+            x = e[k];
+          }
+          cs[k] = c;
+          sn[k] = s;
+        }
+      }
+    }
+    __syncthreads();
+    if (ctl[2]) break;
+    // The sweep's rotations, a column of Q^T per thread, in order.
+    const int l = ctl[0], h = ctl[1];
+    for (int r = tid; r < m; r += kEigThreads) {
+      double u = Qt(l, r);
+      for (int k = l; k < h; ++k) {
+        const double c = cs[k], s = sn[k], u1 = Qt(k + 1, r);
+        Qt(k, r) = c * u + s * u1;
+        u = -s * u + c * u1;
+      }
+      Qt(h, r) = u;
+    }
+    __syncthreads();
+  }
+  bad = 0;
+  for (int i = tid; i < m; i += kEigThreads) {
+    values[i] = d[i];
+    if (!isfinite(d[i])) bad = 1;
+  }
+  if (bad) ctl[3] = 1;
+  __syncthreads();
+  if (tid == 0) *status = ctl[3];
+}
+
+// The device solve from the host: a (m x m row-major) in, values and
+// vectors (vectors[i * m + l] = component i of eigenvector l) out; ws holds
+// 2 m^2 + m + 1 doubles of device memory.
+static int device_sym_eigen(wg_context* ctx, double* ws, const std::vector<double>& a, int m,
+                            std::vector<double>& values, std::vector<double>& vectors, bool& ok) {
+  hipStream_t st = ctx->stream;
+  const size_t mm = (size_t)m * m;
+  double* da = ws;
+  double* dq = ws + mm;
+  double* dv = ws + 2 * mm;
+  int* dst = (int*)(ws + 2 * mm + m);
+  WG_HIP(ctx, hipMemcpyAsync(da, a.data(), mm * 8, hipMemcpyHostToDevice, st));
+  sym_eigen_dev<<<1, kEigThreads, 0, st>>>(da, dq, m, dv, dst);
+  WG_HIP(ctx, hipGetLastError());
+  std::vector<double> qt(mm);
+  values.resize(m);
+  int status = 1;
+  WG_HIP(ctx, hipMemcpyAsync(qt.data(), dq, mm * 8, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipMemcpyAsync(values.data(), dv, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipMemcpyAsync(&status, dst, 4, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  ok = status == 0;
+  vectors.resize(mm);
   for (int i = 0; i < m; ++i)
-    for (int l = 0; l < m; ++l) vectors[(size_t)i * m + l] = Qt(l, i);
-  return true;
+    for (int l = 0; l < m; ++l) vectors[(size_t)i * m + l] = qt[(size_t)l * m + i];
+  return WG_OK;
 }
 
 // ritz::order (ritz.rs:56-90): indexes of values at the `which` end, the
@@ -1086,8 +1181,13 @@ struct Solver {
   uint32_t size, wanted, len = 0;
   uint64_t dim, max_iter, iterations = 0, streams = 0;
   uint32_t restarts = 0;
+  // What the Rayleigh-Ritz steps so far found, for progress (lanczos.rs:438-446).
+  uint32_t est_converged = 0;
+  bool has_residual = false;
+  double est_residual = 0.0;
   E* Q = nullptr;
   double* C = nullptr;
+  double* W = nullptr;  // the projected eigenproblem's device workspace
   std::vector<double> H;
   bool has_prev = false;
   double prev = 0.0;
@@ -1100,7 +1200,9 @@ struct Solver {
 
   int init() {
     int s;
-    if ((s = op.get(&Q, std::max<uint64_t>(ld, 1) * size)) || (s = op.get(&C, (size_t)size * size))) return s;
+    if ((s = op.get(&Q, std::max<uint64_t>(ld, 1) * size)) || (s = op.get(&C, (size_t)size * size)) ||
+        (s = op.get(&W, eigen_workspace(size))))
+      return s;
     H.assign((size_t)size * size, 0.0);
     return WG_OK;
   }
@@ -1238,8 +1340,10 @@ struct Solver {
     for (uint32_t i = 0; i < len; ++i)
       for (uint32_t j = 0; j < len; ++j) a[(size_t)i * len + j] = h(i, j);
     std::vector<double> vals, vecs;
-    if (!sym_eigen(a, (int)len, vals, vecs))
-      return fail(ctx, WG_NOT_FINITE, "the eigenproblem produced a non-finite value");
+    bool solved = false;
+    int s;
+    if ((s = device_sym_eigen(ctx, W, a, (int)len, vals, vecs, solved))) return s;
+    if (!solved) return fail(ctx, WG_NOT_FINITE, "the eigenproblem produced a non-finite value");
     const double tie = sizeof(E) == 4 ? kF32Tie : kF64Tie;
     const std::vector<int> order = ritz_order(vals, op.o.which, tie);
     r.values.resize(len);
@@ -1256,8 +1360,15 @@ struct Solver {
     double scale = 0.0;
     for (double v : r.values) scale = std::max(scale, std::fabs(v));
     converged = 0;
-    for (uint32_t l = 0; l < wanted; ++l)
-      if (std::fabs(beta * r.vectors[(size_t)(len - 1) * len + l]) <= op.o.tol * scale) ++converged;
+    double largest = 0.0;
+    for (uint32_t l = 0; l < wanted; ++l) {
+      const double est = std::fabs(beta * r.vectors[(size_t)(len - 1) * len + l]);
+      if (est <= op.o.tol * scale) ++converged;
+      largest = std::max(largest, est);
+    }
+    est_converged = converged;
+    est_residual = scale > 0.0 ? largest / scale : largest;
+    has_residual = true;
     const bool done = len == dim || converged == wanted;
     if (done) return 0;
     return iterations >= max_iter ? 1 : 2;
@@ -1301,9 +1412,23 @@ struct Solver {
   }
 
   // lanczos.rs:502-542. converged_out: whether every pair met tol.
+  // The steps taken and the last estimates, to the context's progress port
+  // (lanczos.rs:618-629); the pairs converged count the components' exact
+  // pairs, which come first.
+  void report() {
+    const wg_progress& p = ctx->progress;
+    if (p.done) p.done(p.user, iterations);
+    if (p.detail) {
+      p.detail(p.user, "converged", (double)(op.o.k - wanted + est_converged));
+      p.detail(p.user, "restarts", (double)restarts);
+      if (has_residual) p.detail(p.user, "residual", est_residual);
+    }
+  }
+
   int run(std::vector<Pair>& out, bool& converged_out, bool keep_vectors) {
     int s;
     if ((s = init())) return s;
+    if (ctx->progress.phase) ctx->progress.phase(ctx->progress.user, "lanczos", (int64_t)max_iter);
     bool ok;
     if ((s = extend_randomly(ok))) return s;
     if (!ok)
@@ -1313,6 +1438,7 @@ struct Solver {
       if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
       double beta;
       if ((s = step(beta))) return s;
+      report();
       if (len < size && iterations < max_iter) {
         if ((s = extend(beta, ok))) return s;
         if (ok) continue;
@@ -1325,6 +1451,7 @@ struct Solver {
       if ((s = ritz(r))) return s;
       uint32_t conv;
       const int oc = outcome(r, beta >= 0.0 ? beta : 0.0, conv);
+      report();
       if (oc == 2) {
         ++restarts;
         if ((s = restart(r, conv))) return s;
@@ -1400,10 +1527,11 @@ extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts
     if (size < std::min<uint64_t>(dim, (uint64_t)wanted + 1))
       return fail(ctx, WG_INVALID_ARGUMENT, "basis size must exceed the pairs Lanczos computes");
     // Admission (working_set.rs:114-128): what the operator and the
-    // components already hold plus the basis and its rotation coefficients
-    // (as Solver::init allocates them), before the first step.
+    // components already hold plus the basis, its rotation coefficients and
+    // the projected eigenproblem's workspace (as Solver::init allocates
+    // them), before the first step.
     const uint64_t need = std::max<uint64_t>(std::max<uint64_t>(n, 1) * size * (o->precision == WG_F32 ? 4 : 8), 8) +
-                          std::max<uint64_t>((uint64_t)size * size * 8, 8);
+                          std::max<uint64_t>((uint64_t)size * size * 8, 8) + eigen_workspace(size) * 8;
     const uint64_t held = ctx->in_use.load();
     working_set = held - std::min(held, in_use_at_start) + need;
     if (info) info->working_set = working_set;
@@ -1501,5 +1629,25 @@ extern "C" int wg_eigen(wg_context* ctx, const wg_graph* gc, const wg_eigen_opts
     info->basis_size = size;
     info->working_set = working_set;
   }
+  return ok(ctx);
+}
+
+extern "C" int wg_sym_eigen(wg_context* ctx, uint32_t m, const double* a, double* values, double* vectors) {
+  if (!ctx || (m && (!a || !values || !vectors))) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
+  if (m > kMaxBasis) return fail(ctx, WG_INVALID_ARGUMENT, "the matrix may have at most 256 rows");
+  if (m == 0) return ok(ctx);
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = eigen_workspace(m) * 8;
+  void* ws = nullptr;
+  int s;
+  if ((s = dalloc(ctx, &ws, bytes))) return s;
+  std::vector<double> am(a, a + (size_t)m * m), vals, vecs;
+  bool solved = false;
+  s = device_sym_eigen(ctx, (double*)ws, am, (int)m, vals, vecs, solved);
+  dfree(ctx, ws, bytes);
+  if (s) return s;
+  if (!solved) return fail(ctx, WG_NOT_FINITE, "the eigenproblem produced a non-finite value");
+  std::copy(vals.begin(), vals.end(), values);
+  std::copy(vecs.begin(), vecs.end(), vectors);
   return ok(ctx);
 }

@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "semiring.hpp"
+#include "spmv_tiles.hpp"
 #include "wg_internal.hpp"
 
 namespace wg {
@@ -226,6 +227,116 @@ __global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const doub
   if (e_err) raise_error(a.err, b.row0, e_err);
 }
 
+// ---- the gather through the tiled SpMV (large graphs) ----------------------
+// On graphs of kPrTileMinNnz entries or more an iteration is: the hot and
+// warm message copies (pr_tiers), the tiled fold of each row's messages
+// (spmv_tiles.hpp, the stop flag as its halt word), the wave-boundary rows
+// (tile_fixup), then pr_tile_rows: advance and spread of every row from its
+// folded value, with fixed-order block partials of the L1 change and of the
+// dangling rank, summed by pr_reduce (a block per span of partials) and
+// pr_finish_sum (one workgroup, the stop decision).
+constexpr uint64_t kPrTileMinNnz = 1ull << 22;
+constexpr uint32_t kPrReduceSpan = 256;
+
+__global__ __launch_bounds__(256) void pr_tiers(const double* __restrict__ msg, const uint32_t* __restrict__ inv,
+                                                uint32_t nh, double* __restrict__ xh,
+                                                const uint32_t* __restrict__ warm, uint32_t nw,
+                                                double* __restrict__ xw, const PrState* __restrict__ st) {
+  if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint64_t total = (uint64_t)nh + nw;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+    if (i < nh)
+      xh[i] = msg[inv[i]];
+    else
+      xw[i - nh] = msg[warm[i - nh]];
+  }
+}
+
+// Every row: its folded in-messages from the row's slot (0 for a row with no
+// in-edges, the empty sum), then advance and spread.
+__global__ __launch_bounds__(256) void pr_tile_rows(PrArgs a, const uint64_t* __restrict__ nz_bits,
+                                                    const uint32_t* __restrict__ nz_pre,
+                                                    const double* __restrict__ ycomp, uint64_t n,
+                                                    double* __restrict__ msg_out, double* __restrict__ bpart,
+                                                    uint32_t nbp, int parity) {
+  __shared__ double wsum[8];
+  if (pr_halted(a)) return;
+  const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  double change = 0.0, dang = 0.0;
+  uint32_t e_err = 0;
+  if (r < n) {
+    const uint64_t wd = nz_bits[r >> 6];
+    const uint32_t b = (uint32_t)(r & 63);
+    double y = 0.0;
+    if ((wd >> b) & 1) y = ycomp[nz_pre[r >> 6] + __popcll(wd & ((1ull << b) - 1))];
+    change = pr_row(a, r, y, a.st->dangling[parity], msg_out, dang, e_err);
+  }
+  const double c = block_sum(change, wsum);
+  const double dd = block_sum(dang, wsum + 4);
+  if (threadIdx.x == 0) {
+    bpart[blockIdx.x] = c;
+    bpart[nbp + blockIdx.x] = dd;
+  }
+  if (e_err) raise_error(a.err, r, e_err);
+}
+
+// Block k sums partials [k * span, (k + 1) * span) of both halves, in a
+// fixed order (thread-strided, then a tree).
+__global__ __launch_bounds__(256) void pr_reduce(const PrArgs a, const double* __restrict__ bpart, uint32_t nbp,
+                                                 uint32_t span, double* __restrict__ out, uint32_t nout) {
+  __shared__ double sc[256], sd[256];
+  if (pr_halted(a)) return;
+  const uint32_t tid = threadIdx.x, lo = blockIdx.x * span, hi = min(lo + span, nbp);
+  double c = 0.0, d = 0.0;
+  for (uint32_t i = lo + tid; i < hi; i += 256) {
+    c += bpart[i];
+    d += bpart[nbp + i];
+  }
+  sc[tid] = c;
+  sd[tid] = d;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      sc[tid] += sc[tid + s];
+      sd[tid] += sd[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[blockIdx.x] = sc[0];
+    out[nout + blockIdx.x] = sd[0];
+  }
+}
+
+// The totals of pr_reduce's outputs (fixed order) and the stop decision.
+__global__ __launch_bounds__(256) void pr_finish_sum(PrArgs a, const double* __restrict__ red, uint32_t nred,
+                                                     int parity) {
+  __shared__ double sc[256], sd[256];
+  if (pr_halted(a)) return;
+  const uint32_t tid = threadIdx.x;
+  double c = 0.0, d = 0.0;
+  for (uint32_t i = tid; i < nred; i += 256) {
+    c += red[i];
+    d += red[nred + i];
+  }
+  sc[tid] = c;
+  sd[tid] = d;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      sc[tid] += sc[tid + s];
+      sd[tid] += sd[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.st->delta = sc[0];
+    a.st->dangling[parity ^ 1] = sd[0];
+    a.st->iterations += 1;
+    if (a.has_tol && sql_lt(sc[0], a.bound)) a.st->stop = 1;
+  }
+}
+
 // msg = x / out (or 0, the rank joining the dangling sum) for the first
 // iteration, and the live-row count / seed sums of the set-up.
 __global__ __launch_bounds__(256) void pr_spread0(const double* __restrict__ x,
@@ -435,6 +546,24 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
   // With a tolerance whose bound fails, the reference raises that error
   // after the first iteration (mod.rs:144-147).
   const uint32_t max_iter = bound_err ? 1 : o->max_iter;
+  // The tiled gather on large graphs (its layout built once per graph).
+  bool tiled = false;
+  const double* wt = nullptr;
+  double *ycomp = nullptr, *bpart = nullptr, *red = nullptr;
+  TileRec<double>* recs = nullptr;
+  const uint32_t nbp = (uint32_t)((n + 255) / 256);
+  const uint32_t nred = (nbp + kPrReduceSpan - 1) / kPrReduceSpan;
+  if (rev.nnz >= kPrTileMinNnz) {
+    tiled = prepare_tiles(ctx, rev, sizeof(double), tile_hot_cap<double>(), tile_warm_cap<double>()) == WG_OK;
+    if (!tiled) (void)ok(ctx);
+  }
+  if (tiled) {
+    if (wv && (s = tile_weight_view<double>(ctx, rev, wv, &wt))) return s;
+    if ((s = buf.get_at(&ycomp, std::max<uint64_t>(rev.nz_rows, 1), 160))) return s;
+    if ((s = buf.get_at(&recs, rev.tile_nwaves, 161))) return s;
+    if ((s = buf.get_at(&bpart, 2 * (size_t)nbp, 162))) return s;
+    if ((s = buf.get_at(&red, 2 * (size_t)nred, 163))) return s;
+  }
   uint32_t launched = 0;
   PrState hs{};
   const uint32_t batch = 8;
@@ -446,6 +575,26 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
       const double* mi = parity ? msg1 : msg0;
       double* mo = parity ? msg0 : msg1;
       if (rev.nblocks == 0) break;
+      if (tiled) {
+        const DevCsr::Tiles& t = rev.tiles[(int)sizeof(double)];
+        if (t.nhot || t.nwarm)
+          pr_tiers<<<(unsigned)std::min<uint64_t>(((uint64_t)t.nhot + t.nwarm + 255) / 256, 16384), 256, 0, st>>>(
+              mi, rev.hot_inv, t.nhot, (double*)rev.hot_x, t.warm_ids, t.nwarm, (double*)t.warm_x, state);
+        const unsigned tg = rev.tile_nwaves / kTileWaves;
+        if (wt)
+          spmv_tiles<double, PLUS, TIMES, true><<<tg, 1024, 0, st>>>(
+              t.tagged, wt, mi, (const double*)rev.hot_x, t.nhot, (const double*)t.warm_x, (uint32_t)n,
+              rev.tile_wave, rev.tile_heads, rev.nnz, ycomp, recs, ctx->d_jerr, &state->stop);
+        else
+          spmv_tiles<double, PLUS, SECOND, false><<<tg, 1024, 0, st>>>(
+              t.tagged, nullptr, mi, (const double*)rev.hot_x, t.nhot, (const double*)t.warm_x, (uint32_t)n,
+              rev.tile_wave, rev.tile_heads, rev.nnz, ycomp, recs, ctx->d_jerr, &state->stop);
+        tile_fixup<double, PLUS><<<(rev.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, rev.tile_nwaves, ycomp);
+        pr_tile_rows<<<nbp, 256, 0, st>>>(a, rev.nz_bits, rev.nz_pre, ycomp, n, mo, bpart, nbp, parity);
+        pr_reduce<<<nred, 256, 0, st>>>(a, bpart, nbp, kPrReduceSpan, red, nred);
+        pr_finish_sum<<<1, 256, 0, st>>>(a, red, nred, parity);
+        continue;
+      }
       if (wv)
         pr_iterate<true><<<rev.nblocks, kBlockThreads, 0, st>>>(a, mi, mo, parity);
       else

@@ -98,6 +98,7 @@ struct PprArgs {
   double* piece;             // per tile: pieces x kTw
   uint32_t nparts;           // slots: ppr_block then ppr_long workgroups
   unsigned long long* err;
+  const unsigned long long* cancel;  // the context's host-mapped cancel word
 };
 
 // A lane's columns: position t * kTw + kC * k + q is column c[q] when on[q].
@@ -197,7 +198,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
   const double* tile = a.Min + (uint64_t)t * a.nm * kTw + kC * k;
   double change[kC] = {}, dang[kC] = {};
   uint32_t e_err = 0;
+  __shared__ unsigned long long stop;
   for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    // The cancel word, polled once a work item (its load overlaps the
+    // item's staging); a raised word ends the pass, and the host reports
+    // the call cancelled.
+    if (tid == 0) stop = __hip_atomic_load(a.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const PprItem item = items[it];
     const bool piece = item.nrows == 0;
     const uint32_t nr = piece ? 1 : item.nrows;
@@ -213,6 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
       if (HW) sw[i] = ld_nt(a.w + b0 + i);
     }
     __syncthreads();
+    if (stop) break;
     // Group g's run of entries [c0, c1) (item-relative).
     const uint32_t chunk = (nent + kG - 1) / kG;
     const uint32_t c0 = min(g * chunk, nent), c1 = min(c0 + chunk, nent);
@@ -666,7 +673,8 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
     const uint32_t nact = (uint32_t)active_cols.size();
     const uint32_t tiles = (nact + kTw - 1) / kTw;
     PprArgs a{rev.offsets, g->hot_src, wv, n, g->hot_index, nm, R, nact, cols_d, out, tomb, seed_row_d, Pd,
-              o->damping, teleport, dangling, X, Min, Mout, part, piece, nparts, ctx->d_err};
+              o->damping, teleport, dangling, X, Min, Mout, part, piece, nparts, ctx->d_err,
+              ctx->d_mail + kMailCancel};
     for (uint32_t t = 0; t < tiles; ++t) {
       if (wv) ppr_block<true><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
       else ppr_block<false><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
@@ -679,6 +687,8 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
     std::swap(Min, Mout);
     WG_HIP(ctx, hipMemcpyAsync(act.data(), active_d, R, hipMemcpyDeviceToHost, st));
     if ((s = collect_error(ctx, "pagerank batch"))) return s;
+    // A pass the cancel word cut short is never taken as an iteration.
+    if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
     // Survivors keep their relative order; the messages are repacked.
     std::vector<uint32_t> next_cols, from;
     for (uint32_t p = 0; p < nact; ++p)

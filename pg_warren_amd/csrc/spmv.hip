@@ -557,12 +557,67 @@ __global__ __launch_bounds__(256) void spmm_long(const uint32_t* __restrict__ lo
   if (e_err) raise_error(err, row, e_err);
 }
 
+// ---- the tiled kernel's layout builders (spmv_tiles.hpp) ----------------
+// tagged[slot of p] = p's target, tagged hot or warm, for every entry;
+// padding 0. warm: the warm run's columns, ascending.
+__global__ __launch_bounds__(256) void tile_tag(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ rank,
+                                                uint32_t nhot, const uint32_t* __restrict__ warm, uint32_t nwarm,
+                                                uint32_t ncols, uint64_t nnz, uint64_t padded,
+                                                uint32_t* __restrict__ tagged) {
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
+    uint32_t v = 0;
+    if (p < nnz) {
+      const uint32_t c = tgt[p], k = rank[c];
+      if (k < nhot) {
+        v = kTileHot | k;
+      } else if (k - nhot < nwarm) {
+        // The column's slot in the ascending warm run.
+        uint32_t lo = 0, hi = nwarm;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (warm[mid] < c)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        v = ncols + lo;
+      } else {
+        v = c;
+      }
+    }
+    const uint32_t li = (uint32_t)(p % kTileEntries);
+    tagged[p - li + tile_slot(li >> 3, li & 7)] = v;
+  }
+}
+
+// Marks each non-empty row's first entry.
+__global__ __launch_bounds__(256) void tile_heads(const uint64_t* __restrict__ off, uint64_t n,
+                                                  uint32_t* __restrict__ tagged) {
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
+    const uint64_t p = off[r];
+    if (off[r + 1] > p) {
+      const uint32_t li = (uint32_t)(p % kTileEntries);
+      tagged[p - li + tile_slot(li >> 3, li & 7)] |= kTileHead;
+    }
+  }
+}
+
+// Weights of one element type in the tile order; padding 0.
+template <class E>
+__global__ __launch_bounds__(256) void tile_weights(const E* __restrict__ w, uint64_t nnz, uint64_t padded,
+                                                    E* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
+    const uint32_t li = (uint32_t)(p % kTileEntries);
+    out[p - li + tile_slot(li >> 3, li & 7)] = p < nnz ? w[p] : E();
+  }
+}
+
 __global__ void tile_clear(unsigned long long* p) { *p = ~0ull; }
 
 // The tile layout of c for elements of esize bytes (and the part all sizes
 // share), built once. WG_UNSUPPORTED when the layout does not apply (an id
 // space of 2^30 or more, no entries, or no hot-column ranking).
-static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap, uint32_t warmcap) {
+int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap, uint32_t warmcap) {
   if (c.tile_state < 0) return WG_UNSUPPORTED;
   if (c.n >= (1ull << 30) || c.nnz == 0 || c.nnz >= (1ull << 40)) {
     c.tile_state = -1;
@@ -660,6 +715,25 @@ static int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotc
   return WG_OK;
 }
 
+// The weights of one element type in the tile order (copied once from wv,
+// the CSR-order weights of that type, and kept with the layout).
+template <class E>
+int tile_weight_view(wg_context* ctx, DevCsr& c, const E* wv, const E** out) {
+  DevCsr::Tiles& t = c.tiles[(int)sizeof(E)];
+  const int d = DtypeOf<E>::v;
+  if (!t.w[d]) {
+    void* p = nullptr;
+    if (int s = dalloc(ctx, &p, c.tile_padded * sizeof(E))) return s;
+    tile_weights<E><<<(unsigned)std::min<uint64_t>((c.tile_padded + 255) / 256, 16384), 256, 0, ctx->stream>>>(
+        wv, c.nnz, c.tile_padded, (E*)p);
+    WG_HIP(ctx, hipGetLastError());
+    t.w[d] = p;
+  }
+  *out = (const E*)t.w[d];
+  return WG_OK;
+}
+template int tile_weight_view<double>(wg_context*, DevCsr&, const double*, const double**);
+
 // y = A (+.x) x through the tiled kernel; wv: the weights of this element
 // type in CSR order, or null (every weight 1).
 template <class E, int ADD, int MUL>
@@ -670,17 +744,7 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
   DevCsr::Tiles& t = c.tiles[(int)sizeof(E)];
   int s;
   const E* wt = nullptr;
-  if (wv) {
-    const int d = DtypeOf<E>::v;
-    if (!t.w[d]) {
-      void* p = nullptr;
-      if ((s = dalloc(ctx, &p, c.tile_padded * sizeof(E)))) return s;
-      tile_weights<E><<<(unsigned)std::min<uint64_t>((c.tile_padded + 255) / 256, 16384), 256, 0, st>>>(
-          wv, c.nnz, c.tile_padded, (E*)p);
-      t.w[d] = p;
-    }
-    wt = (const E*)t.w[d];
-  }
+  if (wv && (s = tile_weight_view(ctx, c, wv, &wt))) return s;
   const size_t ybytes = ((c.nz_rows * sizeof(A)) + 255) & ~(size_t)255;
   const size_t need = ybytes + (size_t)c.tile_nwaves * sizeof(TileRec<A>);
   if (need > c.tile_out_bytes) {
@@ -705,11 +769,11 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
   if (wt)
     spmv_tiles<E, ADD, MUL, true><<<grid, 1024, 0, st>>>(t.tagged, wt, x, (const E*)c.hot_x, t.nhot,
                                                          (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
-                                                         c.nnz, ycomp, recs, ctx->d_jerr);
+                                                         c.nnz, ycomp, recs, ctx->d_jerr, nullptr);
   else
     spmv_tiles<E, ADD, MUL, false><<<grid, 1024, 0, st>>>(t.tagged, nullptr, x, (const E*)c.hot_x, t.nhot,
                                                           (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
-                                                          c.nnz, ycomp, recs, ctx->d_jerr);
+                                                          c.nnz, ycomp, recs, ctx->d_jerr, nullptr);
   tile_fixup<E, ADD><<<(c.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, c.tile_nwaves, ycomp);
   tile_write<E, ADD><<<(unsigned)((c.n + 255) / 256), 256, 0, st>>>(c.nz_bits, c.nz_pre, c.n, ycomp, y, wa,
                                                                      ctx->d_jerr, ctx->d_err);

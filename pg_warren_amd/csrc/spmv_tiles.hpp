@@ -135,10 +135,13 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
     const E* __restrict__ xhot, uint32_t nhot, const E* __restrict__ xwarm, uint32_t ncols,
     const uint32_t* __restrict__ wave_tiles,
     const uint32_t* __restrict__ wave_heads, uint64_t nnz, typename Fold<E, ADD>::A* __restrict__ ycomp,
-    TileRec<typename Fold<E, ADD>::A>* __restrict__ recs, unsigned long long* __restrict__ jerr) {
+    TileRec<typename Fold<E, ADD>::A>* __restrict__ recs, unsigned long long* __restrict__ jerr,
+    const uint32_t* __restrict__ halt) {
   using F = Fold<E, ADD>;
   using A = typename F::A;
   constexpr uint32_t HOTN = tile_hot_cap<E>();
+  // A set halt word (an iterative caller's stop flag) makes the launch a no-op.
+  if (halt && __hip_atomic_load(halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   constexpr bool RX = MUL != FIRST && MUL != PAIR;  // the product reads x
   __shared__ E hot[HOTN];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -386,62 +389,6 @@ __global__ __launch_bounds__(256) void tile_write(const uint64_t* __restrict__ n
   }
   write_elem<E, ADD>(wa, r, &y[r], present, res, e_err);
   if (e_err) raise_error(err, r, e_err);
-}
-
-// ---- layout builders ----------------------------------------------------
-
-// tagged[slot of p] = p's target, tagged hot or warm, for every entry;
-// padding 0. warm: the warm run's columns, ascending.
-__global__ __launch_bounds__(256) void tile_tag(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ rank,
-                                                uint32_t nhot, const uint32_t* __restrict__ warm, uint32_t nwarm,
-                                                uint32_t ncols, uint64_t nnz, uint64_t padded,
-                                                uint32_t* __restrict__ tagged) {
-  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
-    uint32_t v = 0;
-    if (p < nnz) {
-      const uint32_t c = tgt[p], k = rank[c];
-      if (k < nhot) {
-        v = kTileHot | k;
-      } else if (k - nhot < nwarm) {
-        // The column's slot in the ascending warm run.
-        uint32_t lo = 0, hi = nwarm;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (warm[mid] < c)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        v = ncols + lo;
-      } else {
-        v = c;
-      }
-    }
-    const uint32_t li = (uint32_t)(p % kTileEntries);
-    tagged[p - li + tile_slot(li >> 3, li & 7)] = v;
-  }
-}
-
-// Marks each non-empty row's first entry.
-__global__ __launch_bounds__(256) void tile_heads(const uint64_t* __restrict__ off, uint64_t n,
-                                                  uint32_t* __restrict__ tagged) {
-  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
-    const uint64_t p = off[r];
-    if (off[r + 1] > p) {
-      const uint32_t li = (uint32_t)(p % kTileEntries);
-      tagged[p - li + tile_slot(li >> 3, li & 7)] |= kTileHead;
-    }
-  }
-}
-
-// Weights of one element type in the tile order; padding 0.
-template <class E>
-__global__ __launch_bounds__(256) void tile_weights(const E* __restrict__ w, uint64_t nnz, uint64_t padded,
-                                                    E* __restrict__ out) {
-  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < padded; p += (uint64_t)gridDim.x * 256) {
-    const uint32_t li = (uint32_t)(p % kTileEntries);
-    out[p - li + tile_slot(li >> 3, li & 7)] = p < nnz ? w[p] : E();
-  }
 }
 
 }  // namespace wg

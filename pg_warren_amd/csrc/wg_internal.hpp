@@ -32,7 +32,10 @@ struct Block {
 };
 
 constexpr uint32_t kBlockThreads = 256;
-constexpr size_t kMailWords = 32;  // wg_context mailbox: seq + 31 values
+constexpr size_t kMailWords = 32;  // wg_context mailbox: seq + 30 values + the cancel word
+// The mailbox's last word: 1 once wg_cancel is called, polled by persistent
+// kernels (a host-mapped word, visible to a running kernel).
+constexpr size_t kMailCancel = kMailWords - 1;
 constexpr uint32_t kBlockNnz = 2048;   // entries staged in LDS per block
 constexpr uint32_t kBlockRows = 512;   // rows a short block may hold
 
@@ -127,6 +130,7 @@ struct wg_context {
   uint64_t limit = 0;
   std::atomic<uint64_t> in_use{0};
   std::atomic<int> cancelled{0};
+  wg_progress progress{};  // where long work reports (all null: nowhere)
   int blocking = 1;
   int last_status = 0;
   std::string last_msg;
@@ -294,6 +298,12 @@ int prepare_spmm(wg_context* ctx, DevCsr& csr);
 // Builds csr's hot-first column numbering; returns WG_OK with hot_state -1
 // when the budget cannot hold it.
 int prepare_hot_columns(wg_context* ctx, DevCsr& csr);
+// The tiled SpMV's layout of csr for elements of esize bytes (spmv.hip);
+// WG_UNSUPPORTED when it does not apply.
+int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap, uint32_t warmcap);
+// The weights in the tiled layout's order, as elements of E (spmv.hip).
+template <class E>
+int tile_weight_view(wg_context* ctx, DevCsr& c, const E* wv, const E** out);
 
 #define WG_HIP(ctx, expr)                                   \
   do {                                                      \

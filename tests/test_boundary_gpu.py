@@ -125,3 +125,35 @@ def test_cancel_from_another_thread_stops_a_running_pagerank():
     assert it == 5
     g.close()
     ctx.close()
+
+
+def test_cancel_word_stops_a_running_batched_pagerank():
+    """Batched personalised PageRank polls the context's host-mapped cancel
+    word once a work item inside its persistent pass (a word a running
+    kernel sees), besides the host's check between iterations: a cancel
+    from a second thread ends a long run with WG_CANCELLED, no cut-short
+    pass is taken as an iteration, and after wg_cancel_reset the same call
+    gives the ranks of an uncancelled run."""
+    ctx = W.Context(0)
+    off, tgt = W.rmat(18, 16, seed=6)
+    n = len(off) - 1
+    g = W.Graph(ctx, n, off, tgt)
+    rng = np.random.default_rng(3)
+    seeds = [rng.choice(n, 1 + (i % 5), replace=False) for i in range(32)]
+    want, _ = W.pagerank_batch(ctx, g, seeds, tol=None, max_iter=3)
+    want = want.numpy()
+    t0 = time.monotonic()
+    timer = threading.Timer(0.2, ctx.cancel)
+    timer.start()
+    with pytest.raises(W.WarrenError) as e:
+        W.pagerank_batch(ctx, g, seeds, tol=None, max_iter=100_000)
+    elapsed = time.monotonic() - t0
+    timer.join()
+    assert e.value.name == "cancelled"
+    assert elapsed < 2.0, elapsed
+    ctx.cancel_reset()
+    got, it = W.pagerank_batch(ctx, g, seeds, tol=None, max_iter=3)
+    assert np.all(np.asarray(it) == 3)
+    assert np.array_equal(got.numpy(), want)
+    g.close()
+    ctx.close()

@@ -57,7 +57,8 @@ def test_eigen_one_byte_short_of_its_working_set_is_refused(oracle, matrix, k, p
 
 def test_working_set_counts_the_basis(oracle):
     """A larger basis needs n * size more elements: the reported working
-    set grows by exactly the basis and rotation bytes."""
+    set grows by exactly the basis, rotation and projected-eigenproblem
+    bytes (the device solve's workspace holds 2 size^2 + size + 1 f64)."""
     n = 300
     csr = path(oracle, n)
     ctx = W.Context(0)
@@ -66,7 +67,8 @@ def test_working_set_counts_the_basis(oracle):
     for size in (30, 40):
         W.eigen(ctx, g, 4, precision="f64", basis_size=size, vectors=False)
         sets[size] = W.eigen(ctx, g, 4, precision="f64", basis_size=size, vectors=False)[3]["working_set"]
-    assert sets[40] - sets[30] == n * 10 * 8 + (40 * 40 - 30 * 30) * 8
+    ws = lambda m: (2 * m * m + m + 1) * 8
+    assert sets[40] - sets[30] == n * 10 * 8 + (40 * 40 - 30 * 30) * 8 + ws(40) - ws(30)
     g.close()
     ctx.close()
 

@@ -203,3 +203,130 @@ def test_config_shape_32_smallest_with_a_basis_of_128(ctx, oracle, precision):
         assert np.all(res <= 1e-8 * np.maximum(1.0, np.abs(vals)))
         check_vectors(vec, vals, csr, "normalized_laplacian", weighted=False)
     g.close()
+
+
+def _sym(rng, m):
+    a = rng.standard_normal((m, m))
+    return (a + a.T) / 2
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 17, 64, 128, 256])
+def test_projected_solver_matches_eigh(ctx, m):
+    """The device solve of each Rayleigh-Ritz step (spectral/ritz.rs:34-52)
+    against LAPACK's eigh: values within 1e-12 ||A||, orthonormal vectors,
+    and A V = V diag(values)."""
+    rng = np.random.default_rng(m)
+    a = _sym(rng, m)
+    vals, vecs = W.sym_eigen(ctx, a)
+    scale = max(np.abs(np.linalg.eigvalsh(a)).max(), 1.0)
+    assert np.allclose(np.sort(vals), np.linalg.eigvalsh(a), rtol=0, atol=1e-12 * scale)
+    assert np.abs(vecs.T @ vecs - np.eye(m)).max() <= 1e-12 * m
+    assert np.abs(a @ vecs - vecs * vals).max() <= 1e-11 * scale
+
+
+def test_projected_solver_special_shapes(ctx):
+    """Shapes a thick restart produces, and closed forms."""
+    # Diagonal: nothing to reduce; the values are the diagonal itself.
+    d = np.array([3.0, -1.0, 2.5, 0.0, 7.0])
+    vals, vecs = W.sym_eigen(ctx, np.diag(d))
+    assert np.array_equal(vals, d) and np.array_equal(np.abs(vecs), np.eye(5))
+    # Arrowhead (kept Ritz values on the diagonal, couplings in the last
+    # row and column) followed by a tridiagonal tail.
+    rng = np.random.default_rng(7)
+    m, kept = 40, 24
+    a = np.zeros((m, m))
+    a[np.arange(kept), np.arange(kept)] = np.sort(rng.random(kept))
+    a[kept, :kept] = a[:kept, kept] = rng.random(kept) * 1e-3
+    for i in range(kept, m):
+        a[i, i] = rng.random()
+        if i + 1 < m:
+            a[i, i + 1] = a[i + 1, i] = rng.random()
+    vals, vecs = W.sym_eigen(ctx, a)
+    assert np.allclose(np.sort(vals), np.linalg.eigvalsh(a), rtol=0, atol=1e-13)
+    assert np.abs(vecs.T @ vecs - np.eye(m)).max() <= 1e-12 * m
+    # The path Laplacian: 2 - 2 cos(pi j / m), j = 0..m-1.
+    m = 50
+    lap = 2 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1)
+    lap[0, 0] = lap[-1, -1] = 1.0
+    vals, _ = W.sym_eigen(ctx, lap)
+    want = 2 - 2 * np.cos(np.pi * np.arange(m) / m)
+    assert np.allclose(np.sort(vals), want, rtol=0, atol=1e-13)
+    # A repeated eigenvalue: I + u u^T has 1 (m - 1 times) and 1 + |u|^2.
+    u = rng.random(12)
+    vals, vecs = W.sym_eigen(ctx, np.eye(12) + np.outer(u, u))
+    assert np.allclose(np.sort(vals), [1.0] * 11 + [1 + u @ u], rtol=0, atol=1e-13)
+    assert np.abs(vecs.T @ vecs - np.eye(12)).max() <= 1e-13 * 12
+
+
+def test_projected_solver_errors(ctx):
+    a = np.eye(4)
+    a[1, 2] = a[2, 1] = np.nan
+    with pytest.raises(W.WarrenError) as e:
+        W.sym_eigen(ctx, a)
+    assert e.value.name == "not_finite"
+    with pytest.raises(W.WarrenError) as e:
+        W.sym_eigen(ctx, np.eye(257))
+    assert e.value.name == "invalid_argument"
+
+
+class _Recorded:
+    """Keeps every report (progress.rs recorded::Recorded)."""
+
+    def __init__(self):
+        self.reports = []
+
+    def phase(self, name, total):
+        self.reports.append(("phase", name, total))
+
+    def done(self, done):
+        self.reports.append(("done", done))
+
+    def detail(self, name, value):
+        self.reports.append(("detail", name, value))
+
+    def phases(self):
+        return [(r[1], r[2]) for r in self.reports if r[0] == "phase"]
+
+    def dones(self):
+        return [r[1] for r in self.reports if r[0] == "done"]
+
+    def last_detail(self, name):
+        vals = [r[2] for r in self.reports if r[0] == "detail" and r[1] == name]
+        return vals[-1] if vals else None
+
+
+@pytest.mark.parametrize("k,max_iter", [(3, None), (4, 30), (2, 5)])
+def test_lanczos_reports_its_steps_and_reporting_changes_nothing(ctx, oracle, k, max_iter):
+    """spectral/lanczos/tests.rs:766-803: one phase "lanczos" of the step
+    limit, steps reported in order up to the call's count, converged pairs
+    at most k (k when converged), restarts counted; results unchanged."""
+    rng = np.random.default_rng(40 + k)
+    n = 90
+    # A ring (connected) plus random chords, symmetrised.
+    src = np.concatenate([np.arange(n), rng.integers(0, n, 200)])
+    dst = np.concatenate([(np.arange(n) + 1) % n, rng.integers(0, n, 200)])
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    w = rng.uniform(0.5, 2.0, len(src))
+    g = upload(ctx, oracle.csr_from_edges(n, np.concatenate([src, dst]), np.concatenate([dst, src]),
+                                          np.concatenate([w, w])))
+    kw = dict(matrix="normalized_laplacian", which="smallest", precision="f64", seed=5,
+              max_iter=max_iter, tol=1e-9)
+    plain = W.eigen(ctx, g, k, **kw)
+    rec = _Recorded()
+    ctx.set_progress(rec)
+    try:
+        got = W.eigen(ctx, g, k, **kw)
+    finally:
+        ctx.set_progress(None)
+    assert np.array_equal(plain[0], got[0]) and np.array_equal(plain[1], got[1])
+    assert np.array_equal(plain[2].numpy(), got[2].numpy())
+    info = got[3]
+    size = max(2 * k + 1, 20)
+    total = max(max_iter if max_iter else 1000 * size, k)
+    assert rec.phases() == [("lanczos", total)]
+    dones = rec.dones()
+    assert dones == sorted(dones) and dones[-1] == info["iterations"] <= total
+    conv = rec.last_detail("converged")
+    assert conv <= k and (not info["converged"] or conv == k)
+    assert rec.last_detail("restarts") == info["restarts"]

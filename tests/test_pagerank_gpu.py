@@ -83,9 +83,13 @@ def test_special_cases_and_errors(ctx, oracle):
 
 
 @pytest.mark.parametrize("weighted", [True, False])
-def test_rmat_matches_the_oracle(ctx, oracle, weighted):
-    off, tgt = W.rmat(17, 16, seed=21)
+@pytest.mark.parametrize("scale", [17, 19])
+def test_rmat_matches_the_oracle(ctx, oracle, weighted, scale):
+    """Scale 17 runs the blocked gather; scale 19 (2^22 entries or more)
+    the tiled one, with its hot and warm message copies."""
+    off, tgt = W.rmat(scale, 16, seed=21)
     n = len(off) - 1
+    assert scale == 17 or len(tgt) >= 1 << 22
     w = W.edge_weights(off, tgt, "lognormal", seed=3, p0=0.0, p1=1.0) if weighted else None
     ref = oracle.Csr(n, off, tgt, w)
     want, wit = oracle.pagerank(ref, tol=1e-10, max_iter=100)
