@@ -479,14 +479,16 @@ typedef struct {
 } wg_gather_opts;
 typedef struct wg_gathered wg_gathered;
 /* gather (mod.rs:96-145) over `count` messages of element type dtype
- * (WG_F32, WG_F64, WG_I64 or WG_BOOL; byte messages are not gathered on
- * the device, WG_UNSUPPORTED), senders[i] sending messages[i] (host arrays,
+ * (WG_F32, WG_F64, WG_I64 or WG_BOOL; byte messages go through
+ * wg_gather_bytes; any other type is WG_UNSUPPORTED), senders[i] sending
+ * messages[i] (host arrays,
  * push order). The result holds the receivers that got a message,
  * ascending, each with its value: count as i64; sum, min, max, first,
  * bool_or, bool_and and mode as one message (i64 sums that overflow are
  * WG_BIGINT_OUT_OF_RANGE, the lowest receiver's); collect as its messages
  * in collect order. A combiner the element type does not take is
- * WG_COMBINER_TYPE (concat always); a sender outside the id space,
+ * WG_COMBINER_TYPE (concat always: it takes byte messages, wg_gather_bytes);
+ * a sender outside the id space,
  * WG_SENDER_OUTSIDE_ID_SPACE. */
 int wg_gather(wg_context* ctx, const wg_graph* g, const wg_gather_opts* opts, int dtype, uint64_t count,
               const uint32_t* senders, const void* messages, wg_gathered** out);
@@ -497,6 +499,22 @@ int wg_gathered_info(const wg_gathered* r, uint64_t* receivers, uint64_t* values
  * Any may be NULL. */
 int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t* receivers, void* values,
                      uint64_t* offsets);
+/* gather over byte-string messages (MessageKind::Bytes, combiner.rs:166-196):
+ * message i is bytes[offsets[i] .. offsets[i + 1]) (host arrays, offsets
+ * count + 1 non-decreasing from 0), sent by senders[i]. Combiners collect,
+ * count, first and concat (the bytes of a receiver's messages back to back
+ * in collect order); any other is WG_COMBINER_TYPE. The result's receivers
+ * as wg_gather's; count gives i64 values (wg_gathered_read); collect gives
+ * one string a collected message with offsets per receiver
+ * (wg_gathered_read's offsets), first and concat one string a receiver.
+ * The strings are copied out with wg_gathered_read_bytes. */
+int wg_gather_bytes(wg_context* ctx, const wg_graph* g, const wg_gather_opts* opts, uint64_t count,
+                    const uint32_t* senders, const uint8_t* bytes, const uint64_t* offsets, wg_gathered** out);
+/* The total length of a byte result's strings. */
+int wg_gathered_bytes_info(const wg_gathered* r, uint64_t* bytes);
+/* Host copies of a byte result's strings: their bytes back to back and
+ * values + 1 offsets into them (values as wg_gathered_info gives). */
+int wg_gathered_read_bytes(wg_context* ctx, const wg_gathered* r, uint8_t* bytes, uint64_t* string_offsets);
 void wg_gathered_destroy(wg_gathered* r);
 
 /* ---- eigenpairs: eigen (spectral/lanczos.rs:69-224) ----------------------

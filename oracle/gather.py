@@ -5,9 +5,10 @@ with the messages and fold each receiver's in collect order. Pure Python
 over small inputs; test infrastructure only.
 
 Edges are (relation, source, target) triples. Values are numpy scalars of
-the messages' type; folds follow SemiringElement::apply: i64 sums checked,
-min and max in SQL order (NaN above every number, the first operand kept
-on ties), bool or / and."""
+the messages' type, or bytes for byte messages (a list of bytes objects);
+folds follow SemiringElement::apply: i64 sums checked, min and max in SQL
+order (NaN above every number, the first operand kept on ties), bool or /
+and; concat joins byte messages in collect order."""
 import math
 
 import numpy as np
@@ -23,6 +24,8 @@ class GatherError(Exception):
 
 
 def _kind(messages):
+    if isinstance(messages, (list, tuple)) and all(isinstance(m, (bytes, bytearray)) for m in messages):
+        return "bytes"
     dt = np.asarray(messages).dtype
     if dt == np.bool_:
         return "bool"
@@ -30,11 +33,12 @@ def _kind(messages):
 
 
 def check(combiner, kind):
-    """Combiner::check (combiner.rs:82-94)."""
+    """Combiner::check (combiner.rs:82-94): concat takes byte messages only,
+    mode element messages only."""
     numeric = kind in ("f32", "f64", "i64")
     ok = {"collect": True, "count": True, "first": True, "sum": numeric, "min": numeric,
           "max": numeric, "bool_or": kind == "bool", "bool_and": kind == "bool",
-          "concat": False, "mode": True}[combiner]
+          "concat": kind == "bytes", "mode": kind != "bytes"}[combiner]
     if not ok:
         raise GatherError("combiner_type")
 
@@ -107,6 +111,8 @@ def gather(n, edges, senders, messages, combiner, *, direction="in", relations=N
             value = len(received)
         elif combiner == "first":
             value = received[0]
+        elif combiner == "concat":
+            value = b"".join(received)  # Bytes::concat (combiner.rs:175-191)
         elif combiner == "mode":
             votes = [sum(1 for r in received if sql_eq(r, m)) for m in received]
             best = 0

@@ -562,23 +562,8 @@ COMBINERS = ["collect", "count", "sum", "min", "max", "first", "concat", "bool_o
 GATHER_DIRECTIONS = {"out": 0, "in": 1, "both": 2}
 
 
-def gather(ctx, graph, senders, messages, combiner, *, direction="in", relations=None,
-           per_relation=False, mask=None, mask_complement=False, include_self=False):
-    """gather (compute/gather/mod.rs:96-145): messages[i] sent by senders[i]
-    (push order), combined at each receiver in collect order. Returns
-    (receivers, values): values one per receiver (count: i64), or for
-    collect a list of arrays, one per receiver. messages: a numpy array of
-    f32, f64, i64 or bool; mask: a Vector or None."""
-    msg = np.asarray(messages)
-    dt = _dtype_of(msg)
-    msg = np.ascontiguousarray(msg, NP_DTYPES[dt])
-    snd = np.asarray(senders)
-    if snd.size and (snd.min() < 0 or snd.max() >= 2**32):
-        bad = int(np.flatnonzero((snd < 0) | (snd >= 2**32))[0])
-        raise WarrenError(26, f"sender {snd[bad]} outside the id space {graph.n}")
-    snd = np.ascontiguousarray(snd, np.uint32)
-    if snd.shape != msg.shape:
-        raise ValueError("senders and messages differ in length")
+def _gather_opts(ctx, combiner, direction, relations, per_relation, mask, mask_complement, include_self):
+    """wg_gather_opts, and what must outlive the call (relation list, mask)."""
     if combiner not in COMBINERS:
         raise WarrenError(11, f"unknown combiner {combiner!r}")
     rels, nrels, keep = None, 0, None
@@ -592,6 +577,80 @@ def gather(ctx, graph, senders, messages, combiner, *, direction="in", relations
     opts = _lib.GatherOpts(COMBINERS.index(combiner), GATHER_DIRECTIONS[direction], rels, nrels,
                            int(per_relation), None if mask is None else mask.h.value,
                            int(mask_complement), int(include_self))
+    return opts, (keep, mask)
+
+
+def _senders(graph, senders):
+    snd = np.asarray(senders)
+    if snd.size and (snd.min() < 0 or snd.max() >= 2**32):
+        bad = int(np.flatnonzero((snd < 0) | (snd >= 2**32))[0])
+        raise WarrenError(26, f"sender {snd[bad]} outside the id space {graph.n}")
+    return np.ascontiguousarray(snd, np.uint32)
+
+
+def _gather_bytes(ctx, graph, senders, messages, combiner, **kw):
+    """gather over byte messages (wg_gather_bytes): collect gives a list of
+    bytes a receiver, first and concat one bytes a receiver, count i64."""
+    snd = _senders(graph, senders)
+    if len(snd) != len(messages):
+        raise ValueError("senders and messages differ in length")
+    lens = np.array([len(m) for m in messages], np.uint64)
+    offsets = np.zeros(len(messages) + 1, np.uint64)
+    offsets[1:] = np.cumsum(lens)
+    data = np.frombuffer(b"".join(bytes(m) for m in messages), np.uint8) if offsets[-1] else np.zeros(1, np.uint8)
+    opts, _keep = _gather_opts(ctx, combiner, kw["direction"], kw["relations"], kw["per_relation"], kw["mask"],
+                               kw["mask_complement"], kw["include_self"])
+    h = C.c_void_p()
+    ctx.check(ctx.lib.wg_gather_bytes(ctx.h, graph.h, C.byref(opts), len(snd), _p(snd), _p(data), _p(offsets),
+                                      C.byref(h)))
+    try:
+        nr, nv, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ctx.lib.wg_gathered_info(h, C.byref(nr), C.byref(nv))
+        recv = np.zeros(max(nr.value, 1), np.uint64)
+        if combiner == "count":
+            vals = np.zeros(max(nv.value, 1), np.int64)
+            ctx.check(ctx.lib.wg_gathered_read(ctx.h, h, _p(recv), _p(vals), None))
+            return recv[:nr.value], vals[:nv.value]
+        offs = np.zeros(nr.value + 1, np.uint64) if combiner == "collect" else None
+        ctx.check(ctx.lib.wg_gathered_read(ctx.h, h, _p(recv), None, _p(offs)))
+        ctx.lib.wg_gathered_bytes_info(h, C.byref(nb))
+        buf = np.zeros(max(nb.value, 1), np.uint8)
+        soffs = np.zeros(nv.value + 1, np.uint64)
+        ctx.check(ctx.lib.wg_gathered_read_bytes(ctx.h, h, _p(buf), _p(soffs)))
+    finally:
+        ctx.lib.wg_gathered_destroy(h)
+    raw = buf.tobytes()
+    strings = [raw[int(soffs[i]):int(soffs[i + 1])] for i in range(nv.value)]
+    recv = recv[:nr.value]
+    if combiner == "collect":
+        return recv, [strings[int(offs[i]):int(offs[i + 1])] for i in range(nr.value)]
+    return recv, strings
+
+
+def gather(ctx, graph, senders, messages, combiner, *, direction="in", relations=None,
+           per_relation=False, mask=None, mask_complement=False, include_self=False):
+    """gather (compute/gather/mod.rs:96-145): messages[i] sent by senders[i]
+    (push order), combined at each receiver in collect order. Returns
+    (receivers, values): values one per receiver (count: i64), or for
+    collect a list of arrays, one per receiver. messages: a numpy array of
+    f32, f64, i64 or bool, or a list of bytes objects (byte messages:
+    collect, count, first and concat, the strings gathered on the device);
+    mask: a Vector or None."""
+    if isinstance(messages, (list, tuple)) and messages and all(
+            isinstance(m, (bytes, bytearray)) for m in messages):
+        return _gather_bytes(ctx, graph, senders, messages, combiner, direction=direction,
+                             relations=relations, per_relation=per_relation, mask=mask,
+                             mask_complement=mask_complement, include_self=include_self)
+    msg = np.asarray(messages)
+    dt = _dtype_of(msg)
+    msg = np.ascontiguousarray(msg, NP_DTYPES[dt])
+    snd = _senders(graph, senders)
+    if snd.shape != msg.shape:
+        raise ValueError("senders and messages differ in length")
+    if combiner not in COMBINERS:
+        raise WarrenError(11, f"unknown combiner {combiner!r}")
+    opts, _keep = _gather_opts(ctx, combiner, direction, relations, per_relation, mask, mask_complement,
+                               include_self)
     h = C.c_void_p()
     ctx.check(ctx.lib.wg_gather(ctx.h, graph.h, C.byref(opts), DTYPES[dt], len(snd), _p(snd), _p(msg),
                                 C.byref(h)))

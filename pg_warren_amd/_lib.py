@@ -109,6 +109,9 @@ SIGNATURES = [
     ("wg_gather", I32, [P, P, P, I32, U64, P, P, P]),
     ("wg_gathered_info", I32, [P, P, P]),
     ("wg_gathered_read", I32, [P, P, P, P, P]),
+    ("wg_gather_bytes", I32, [P, P, P, U64, P, P, P, P]),
+    ("wg_gathered_bytes_info", I32, [P, P]),
+    ("wg_gathered_read_bytes", I32, [P, P, P, P]),
     ("wg_gathered_destroy", None, [P]),
     ("wg_graph_load_segments", I32, [P, U64, P, P, P]),
     ("wg_csr_segment_lengths", I32, [U64, U64, P, P, P]),

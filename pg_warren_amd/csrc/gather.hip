@@ -386,6 +386,44 @@ __global__ __launch_bounds__(kThreads) void mode_pick(const uint64_t* __restrict
   }
 }
 
+// ---- byte messages ------------------------------------------------------
+// Byte-string messages ride through the gather as their indexes (i64
+// messages i = 0..count-1); the strings the result holds are then laid out
+// by these kernels: each slot's length, its place (an exclusive scan), and
+// a thread per output byte that finds its slot by binary search.
+__global__ __launch_bounds__(kThreads) void string_lengths(const int64_t* __restrict__ idx, uint64_t k,
+                                                           const uint64_t* __restrict__ boff,
+                                                           uint64_t* __restrict__ len) {
+  for (uint64_t j = first_index(); j < k; j += stride()) {
+    const uint64_t m = (uint64_t)idx[j];
+    len[j] = boff[m + 1] - boff[m];
+  }
+  if (first_index() == 0) len[k] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void string_bytes(const int64_t* __restrict__ idx, uint64_t k,
+                                                         const uint64_t* __restrict__ pos,
+                                                         const uint64_t* __restrict__ boff,
+                                                         const uint8_t* __restrict__ in, uint64_t total,
+                                                         uint8_t* __restrict__ out) {
+  for (uint64_t b = first_index(); b < total; b += stride()) {
+    uint64_t lo = 0, hi = k;  // the last slot j with pos[j] <= b
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (pos[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    out[b] = in[boff[(uint64_t)idx[lo]] + (b - pos[lo])];
+  }
+}
+
+// concat: each receiver's string spans its messages' strings.
+__global__ __launch_bounds__(kThreads) void receiver_spans(const uint64_t* __restrict__ offs, uint64_t nrecv,
+                                                           const uint64_t* __restrict__ pos,
+                                                           uint64_t* __restrict__ out) {
+  for (uint64_t i = first_index(); i <= nrecv; i += stride()) out[i] = pos[offs[i]];
+}
+
 // A device buffer charged to the context, freed on scope exit.
 template <class T>
 struct Buf {
@@ -408,6 +446,13 @@ struct Buf {
   }
   ~Buf() { reset(); }
 };
+
+// Combiner::check for byte messages (combiner.rs:82-94): collect, count,
+// first and concat.
+int combiner_accepts_bytes(int combiner) {
+  if (combiner < C_COLLECT || combiner > C_MODE) return -1;
+  return combiner == C_COLLECT || combiner == C_COUNT || combiner == C_FIRST || combiner == C_CONCAT;
+}
 
 int combiner_accepts(int combiner, int dtype) {
   const bool numeric = dtype == WG_F32 || dtype == WG_F64 || dtype == WG_I64;
@@ -533,11 +578,20 @@ struct wg_gathered {
   void* vals = nullptr;       // nvals values (count: i64)
   uint64_t* offs = nullptr;   // collect: nrecv + 1 offsets into vals
   size_t recv_bytes = 0, vals_bytes = 0, offs_bytes = 0;
+  // Byte messages: the result's strings (collect: one a collected message;
+  // first and concat: one a receiver), their bytes and nvals + 1 offsets.
+  bool is_bytes = false;
+  uint8_t* str = nullptr;
+  uint64_t* soffs = nullptr;
+  uint64_t nstr_bytes = 0;
+  size_t str_bytes = 0, soffs_bytes = 0;
   ~wg_gathered() {
     if (ctx) {
       dfree(ctx, recv, recv_bytes);
       dfree(ctx, vals, vals_bytes);
       dfree(ctx, offs, offs_bytes);
+      dfree(ctx, str, str_bytes);
+      dfree(ctx, soffs, soffs_bytes);
     }
   }
 };
@@ -681,6 +735,11 @@ static int run_gather(wg_context* ctx, const Receiving& R, const DevCsr& S, uint
 
 }  // namespace wg
 
+// The gather of `count` element messages of dtype under `combiner` (checked
+// by the caller) into a new result.
+static int gather_impl(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, int combiner, int dtype,
+                       uint64_t count, const uint32_t* senders, const void* messages, wg_gathered** out);
+
 extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, int dtype,
                          uint64_t count, const uint32_t* senders, const void* messages, wg_gathered** out) {
   if (!ctx || !gc || !o || !out) return WG_INVALID_ARGUMENT;
@@ -696,6 +755,91 @@ extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_op
     return fail(ctx, WG_UNSUPPORTED, "messages of this element type are not gathered on the device");
   // Combiner::check (combiner.rs:82-94).
   if (!acc) return fail(ctx, WG_COMBINER_TYPE, "the combiner does not accept messages of this type");
+  return gather_impl(ctx, gc, o, o->combiner, dtype, count, senders, messages, out);
+}
+
+extern "C" int wg_gather_bytes(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, uint64_t count,
+                               const uint32_t* senders, const uint8_t* bytes, const uint64_t* offsets,
+                               wg_gathered** out) {
+  if (!ctx || !gc || !o || !out) return WG_INVALID_ARGUMENT;
+  if (int cs_ = check_graph(ctx, gc)) return cs_;
+  *out = nullptr;
+  if (!offsets || (count && !senders)) return fail(ctx, WG_INVALID_ARGUMENT, "null messages");
+  if (o->direction != WG_OUT && o->direction != WG_IN && o->direction != WG_BOTH)
+    return fail(ctx, WG_INVALID_ARGUMENT, "unknown direction");
+  if (o->nrelations && !o->relations) return fail(ctx, WG_INVALID_ARGUMENT, "null relation list");
+  const int acc = combiner_accepts_bytes(o->combiner);
+  if (acc < 0) return fail(ctx, WG_INVALID_ARGUMENT, "unknown combiner");
+  if (!acc) return fail(ctx, WG_COMBINER_TYPE, "the combiner does not accept byte messages");
+  if (offsets[0] != 0) return fail(ctx, WG_INVALID_ARGUMENT, "byte offsets must start at 0");
+  for (uint64_t i = 0; i < count; ++i)
+    if (offsets[i + 1] < offsets[i]) return fail(ctx, WG_INVALID_ARGUMENT, "byte offsets must not decrease");
+  const uint64_t total_in = offsets[count];
+  if (total_in && !bytes) return fail(ctx, WG_INVALID_ARGUMENT, "null bytes");
+  // The messages travel as their indexes; concat gathers them as collect.
+  std::vector<int64_t> ids(count);
+  for (uint64_t i = 0; i < count; ++i) ids[i] = (int64_t)i;
+  const int inner = o->combiner == C_CONCAT ? C_COLLECT : o->combiner;
+  wg_gathered* r = nullptr;
+  int s;
+  if ((s = gather_impl(ctx, gc, o, inner, WG_I64, count, senders, ids.data(), &r))) return s;
+  r->combiner = o->combiner;
+  r->is_bytes = true;
+  if (o->combiner == C_COUNT) {
+    *out = r;
+    return ok(ctx);
+  }
+  hipStream_t st = ctx->stream;
+  auto done = [&](int status) {
+    if (status) wg_gathered_destroy(r);
+    return status;
+  };
+  const uint64_t k = r->nvals;  // collected (or first) message indexes
+  Buf<uint8_t> din;
+  Buf<uint64_t> dboff, len, pos;
+  if ((s = din.alloc(ctx, total_in)) || (s = dboff.alloc(ctx, count + 1)) || (s = len.alloc(ctx, k + 1)) ||
+      (s = pos.alloc(ctx, k + 1)))
+    return done(s);
+  if (total_in) WG_HIP(ctx, hipMemcpyAsync(din.p, bytes, total_in, hipMemcpyHostToDevice, st));
+  WG_HIP(ctx, hipMemcpyAsync(dboff.p, offsets, (count + 1) * 8, hipMemcpyHostToDevice, st));
+  string_lengths<<<grid_of(k + 1), kThreads, 0, st>>>((const int64_t*)r->vals, k, dboff.p, len.p);
+  WG_HIP(ctx, hipGetLastError());
+  if ((s = exclusive_scan_u64(ctx, len.p, pos.p, k + 1))) return done(s);
+  uint64_t total = 0;
+  WG_HIP(ctx, hipMemcpyAsync(&total, pos.p + k, 8, hipMemcpyDeviceToHost, st));
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  r->nstr_bytes = total;
+  r->str_bytes = std::max<uint64_t>(total, 1);
+  if ((s = dalloc(ctx, (void**)&r->str, r->str_bytes))) return done(s);
+  if (total)
+    string_bytes<<<grid_of(total), kThreads, 0, st>>>((const int64_t*)r->vals, k, pos.p, dboff.p, din.p, total,
+                                                       r->str);
+  WG_HIP(ctx, hipGetLastError());
+  if (o->combiner == C_CONCAT) {
+    // One string a receiver: its messages' strings back to back.
+    r->soffs_bytes = (r->nrecv + 1) * 8;
+    if ((s = dalloc(ctx, (void**)&r->soffs, r->soffs_bytes))) return done(s);
+    receiver_spans<<<grid_of(r->nrecv + 1), kThreads, 0, st>>>(r->offs, r->nrecv, pos.p, r->soffs);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipStreamSynchronize(st));
+    dfree(ctx, r->offs, r->offs_bytes);
+    r->offs = nullptr;
+    r->offs_bytes = 0;
+    r->nvals = r->nrecv;
+  } else {
+    r->soffs_bytes = pos.bytes;
+    r->soffs = pos.release();
+  }
+  WG_HIP(ctx, hipStreamSynchronize(st));
+  dfree(ctx, r->vals, r->vals_bytes);
+  r->vals = nullptr;
+  r->vals_bytes = 0;
+  *out = r;
+  return ok(ctx);
+}
+
+static int gather_impl(wg_context* ctx, const wg_graph* gc, const wg_gather_opts* o, int combiner, int dtype,
+                       uint64_t count, const uint32_t* senders, const void* messages, wg_gathered** out) {
   wg_graph* g = const_cast<wg_graph*>(gc);
   const uint64_t n = g->n;
   if (o->mask) {
@@ -754,13 +898,13 @@ extern "C" int wg_gather(wg_context* ctx, const wg_graph* gc, const wg_gather_op
   wg_gathered* res = new (std::nothrow) wg_gathered();
   if (!res) return fail(ctx, WG_OUT_OF_MEMORY, "host allocation");
   res->ctx = ctx;
-  res->combiner = o->combiner;
+  res->combiner = combiner;
   res->dtype = dtype;
   switch (dtype) {
-    case WG_F32: s = run_gather<float>(ctx, R, *S, n, o->combiner, (const float*)mval.p, res); break;
-    case WG_F64: s = run_gather<double>(ctx, R, *S, n, o->combiner, (const double*)mval.p, res); break;
-    case WG_I64: s = run_gather<int64_t>(ctx, R, *S, n, o->combiner, (const int64_t*)mval.p, res); break;
-    default: s = run_gather<uint8_t>(ctx, R, *S, n, o->combiner, (const uint8_t*)mval.p, res); break;
+    case WG_F32: s = run_gather<float>(ctx, R, *S, n, combiner, (const float*)mval.p, res); break;
+    case WG_F64: s = run_gather<double>(ctx, R, *S, n, combiner, (const double*)mval.p, res); break;
+    case WG_I64: s = run_gather<int64_t>(ctx, R, *S, n, combiner, (const int64_t*)mval.p, res); break;
+    default: s = run_gather<uint8_t>(ctx, R, *S, n, combiner, (const uint8_t*)mval.p, res); break;
   }
   if (s) {
     delete res;
@@ -789,8 +933,26 @@ extern "C" int wg_gathered_read(wg_context* ctx, const wg_gathered* r, uint64_t*
     for (uint64_t i = 0; i < r->nrecv; ++i) receivers[i] = tmp[i];
   }
   const size_t es = r->combiner == 1 ? 8 : dtype_size(r->dtype);
-  if (values && r->nvals) WG_HIP(ctx, hipMemcpy(values, r->vals, r->nvals * es, hipMemcpyDeviceToHost));
+  // A byte result's strings are read with wg_gathered_read_bytes.
+  if (values && r->nvals && r->vals) WG_HIP(ctx, hipMemcpy(values, r->vals, r->nvals * es, hipMemcpyDeviceToHost));
   if (offsets && r->offs) WG_HIP(ctx, hipMemcpy(offsets, r->offs, (r->nrecv + 1) * 8, hipMemcpyDeviceToHost));
+  return ok(ctx);
+}
+
+extern "C" int wg_gathered_bytes_info(const wg_gathered* r, uint64_t* bytes) {
+  if (!r) return WG_INVALID_ARGUMENT;
+  if (bytes) *bytes = r->nstr_bytes;
+  return WG_OK;
+}
+
+extern "C" int wg_gathered_read_bytes(wg_context* ctx, const wg_gathered* r, uint8_t* bytes,
+                                      uint64_t* string_offsets) {
+  if (!ctx || !r) return WG_INVALID_ARGUMENT;
+  if (r->ctx != ctx) return fail(ctx, WG_INVALID_ARGUMENT, "the result belongs to another or a destroyed context");
+  if (!r->is_bytes || !r->soffs) return fail(ctx, WG_INVALID_ARGUMENT, "the result holds no strings");
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (bytes && r->nstr_bytes) WG_HIP(ctx, hipMemcpy(bytes, r->str, r->nstr_bytes, hipMemcpyDeviceToHost));
+  if (string_offsets) WG_HIP(ctx, hipMemcpy(string_offsets, r->soffs, (r->nvals + 1) * 8, hipMemcpyDeviceToHost));
   return ok(ctx);
 }
 
@@ -801,9 +963,13 @@ void release_gathered(wg_gathered* r) {
   dfree(r->ctx, r->recv, r->recv_bytes);
   dfree(r->ctx, r->vals, r->vals_bytes);
   dfree(r->ctx, r->offs, r->offs_bytes);
+  dfree(r->ctx, r->str, r->str_bytes);
+  dfree(r->ctx, r->soffs, r->soffs_bytes);
   r->recv = nullptr;
   r->vals = nullptr;
   r->offs = nullptr;
+  r->str = nullptr;
+  r->soffs = nullptr;
   r->ctx = nullptr;
 }
 }  // namespace wg

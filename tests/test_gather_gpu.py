@@ -166,3 +166,60 @@ def test_long_segments_fold_in_order_and_raise(ctx):
     ints = rng.integers(2**58, 2**59, n).astype(np.int64)
     compare(ctx, g, n, edges, senders, ints, "sum")
     compare(ctx, g, n, edges, senders, -ints, "sum")
+
+
+def test_byte_messages_known_answers(ctx):
+    """compute/gather/tests.rs:173-202: byte messages collect, concatenate,
+    take the first and count; a folding combiner is refused."""
+    g = graph_of(ctx, 4, EDGES)
+    sent = [(2, b"c"), (0, b"a"), (2, b"d"), (1, b""), (3, b"z")]
+    s = [a for a, _ in sent]
+    m = [b for _, b in sent]
+    r, v = W.gather(ctx, g, s, m, "collect")
+    assert list(r) == [0, 1, 2] and v == [[b"c", b"d"], [b"a"], [b"a", b""]]
+    r, v = W.gather(ctx, g, s, m, "concat")
+    assert list(r) == [0, 1, 2] and v == [b"cd", b"a", b"a"]
+    assert W.gather(ctx, g, s, m, "first")[1] == [b"c", b"a", b"a"]
+    assert list(W.gather(ctx, g, s, m, "count")[1]) == [2, 1, 2]
+    for comb in ("sum", "min", "max", "bool_or", "bool_and", "mode"):
+        with pytest.raises(W.WarrenError) as e:
+            W.gather(ctx, g, s, m, comb)
+        assert e.value.name == "combiner_type"
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_byte_gathers(ctx, seed):
+    """Random graphs, relations, directions, masks and include_self with
+    byte messages of 0 to 40 bytes (some of 300), against the oracle."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(5, 50))
+    R = int(rng.integers(1, 4))
+    edges = sorted({(int(rng.integers(0, R)), int(rng.integers(0, n)), int(rng.integers(0, n)))
+                    for _ in range(int(rng.integers(1, 4 * n)))})
+    g = graph_of(ctx, n, edges)
+    k = int(rng.integers(1, 3 * n))
+    senders = rng.integers(0, n, k)
+    msgs = [bytes(rng.integers(0, 256, int(rng.choice([0, 1, 3, 7, 40, 300]))).astype(np.uint8))
+            for _ in range(k)]
+    nrel = max(R, 1)
+    print(f"byte gather seed={seed} n={n} edges={len(edges)} messages={k}", flush=True)
+    for trial in range(6):
+        direction = ["in", "out", "both"][trial % 3]
+        rels = None if (trial % 2 or nrel == 1) else sorted(set(rng.integers(0, nrel, 2).tolist()))
+        per = bool(trial % 3 == 1) and nrel > 1
+        mask = (rng.random(n) < 0.7).astype(np.float64) if trial % 4 == 3 else None
+        for comb in ("collect", "count", "first", "concat", "sum"):
+            compare(ctx, g, n, edges, senders, msgs, comb, direction=direction, relations=rels,
+                    per_relation=per, mask=mask, include_self=bool(trial % 2))
+
+
+def test_byte_messages_hub(ctx):
+    """A hub receiving 2,500 senders' strings (about 1 MB concatenated)."""
+    rng = np.random.default_rng(9)
+    n = 3000
+    edges = sorted({(0, int(s), 7) for s in rng.integers(0, n, 2500)})
+    g = graph_of(ctx, n, edges)
+    senders = np.arange(n)
+    msgs = [bytes(rng.integers(0, 256, int(rng.integers(0, 800))).astype(np.uint8)) for _ in range(n)]
+    for comb in ("concat", "collect", "first"):
+        compare(ctx, g, n, edges, senders, msgs, comb)

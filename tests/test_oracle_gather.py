@@ -85,3 +85,22 @@ def test_per_relation_sends_once_per_relation():
     # Both directions of one relation are still one link.
     both = dict(vals(run(SENT, "collect", direction="both", per_relation=True)))
     assert both[0] == [2.0, 2.0, 3.0, 4.0] and both[1] == [1.0, 1.0, 3.0, 4.0]
+
+
+def test_byte_messages():
+    """tests.rs:173-202: byte messages collect, concatenate, take the first
+    and count; sum is refused for them, concat for numbers."""
+    sent = [(2, b"c"), (0, b"a"), (2, b"d"), (1, b""), (3, b"z")]
+    s = [a for a, _ in sent]
+    m = [b for _, b in sent]
+    assert G.gather(4, EDGES, s, m, "collect") == [(0, [b"c", b"d"]), (1, [b"a"]), (2, [b"a", b""])]
+    assert G.gather(4, EDGES, s, m, "concat") == [(0, b"cd"), (1, b"a"), (2, b"a")]
+    assert G.gather(4, EDGES, s, m, "first") == [(0, b"c"), (1, b"a"), (2, b"a")]
+    assert G.gather(4, EDGES, s, m, "count") == [(0, 2), (1, 1), (2, 2)]
+    for comb in ("sum", "mode", "bool_or"):
+        with pytest.raises(G.GatherError) as e:
+            G.gather(4, EDGES, s, m, comb)
+        assert e.value.name == "combiner_type"
+    with pytest.raises(G.GatherError) as e:
+        run(SENT, "concat")
+    assert e.value.name == "combiner_type"
