@@ -224,6 +224,19 @@ int wg_csr_segment_lengths(uint64_t id_space, uint64_t nnz, uint64_t* offsets_le
  * (any may be NULL; weights only for a weighted graph). */
 int wg_graph_encode_segments(wg_context* ctx, const wg_graph* g, int reverse, uint8_t* offsets,
                              uint8_t* targets, uint8_t* weights);
+/* A relation's CSR (forward, or reverse = 1) as its segments in the
+ * reference's layout: with sparse offsets -- the stored rows' ids (kind 11,
+ * u32) and an offset per stored row plus the end -- where they take at most
+ * a quarter of the bytes of dense offsets (graph/csr/sparse.rs:10-40,
+ * graph_csr.rs:769-800), else dense; a graph of one relation holds it as
+ * its all-relations CSR, always dense. The lengths first (rows_len 0 when
+ * dense; weights_len 0 for an unweighted graph); any buffer may be NULL.
+ * WG_UNKNOWN_RELATION for a relation outside the graph's dictionary. */
+int wg_relation_segment_lengths(wg_context* ctx, const wg_graph* g, uint32_t relation, int reverse,
+                                uint64_t* rows_len, uint64_t* offsets_len, uint64_t* targets_len,
+                                uint64_t* weights_len);
+int wg_relation_encode_segments(wg_context* ctx, const wg_graph* g, uint32_t relation, int reverse,
+                                uint8_t* rows, uint8_t* offsets, uint8_t* targets, uint8_t* weights);
 /* The forward CSR (host buffers of n+1, nnz, nnz; targets and weights may
  * be NULL). */
 int wg_graph_read_forward(wg_context* ctx, const wg_graph* g, uint64_t* offsets,

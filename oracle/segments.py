@@ -45,6 +45,25 @@ def encode_csr(offsets, targets, weights=None, rows=None):
     return out
 
 
+def stores_sparse(stored_rows, id_space, target_bytes=4):
+    """stores_sparse (graph/csr/sparse.rs:37-41): sparse offsets -- a row id
+    of the target width and a u64 offset per stored row, and the end -- when
+    they take at most a quarter of the bytes of dense ones."""
+    sparse = stored_rows * (target_bytes + 8) + 8
+    dense = (id_space + 1) * 8
+    return sparse * 4 <= dense
+
+
+def relation_form(id_space, offsets, targets, weights=None, relations=2):
+    """A relation's CSR encoded as the reference lays it out: sparse where
+    stores_sparse says so (never for a graph's only relation)."""
+    off = np.asarray(offsets, np.uint64)
+    stored = int(np.count_nonzero(np.diff(off.astype(np.int64)) > 0))
+    if relations > 1 and stores_sparse(stored, id_space):
+        return sparse_form(off, targets, weights)
+    return encode_csr(off, targets, weights)
+
+
 def sparse_form(offsets, targets, weights=None):
     """The same CSR with sparse offsets: the ids of its non-empty rows and an
     offset per stored row plus the end (Csr::from_sparse_parts)."""

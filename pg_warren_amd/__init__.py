@@ -292,6 +292,22 @@ class Graph:
         return {"offsets": o.tobytes(), "targets": t.tobytes(),
                 "weights": None if w is None else w.tobytes()}
 
+    def encode_relation_segments(self, relation, reverse=False):
+        """A relation's CSR as its segments in the reference's layout (sparse
+        offsets where they pay, graph/csr/sparse.rs): {'rows' (sparse only),
+        'offsets', 'targets', 'weights'}."""
+        ln = [C.c_uint64() for _ in range(4)]
+        self.ctx.check(self.ctx.lib.wg_relation_segment_lengths(self.ctx.h, self.h, int(relation), int(reverse),
+                                                                *[C.byref(x) for x in ln]))
+        bufs = [np.zeros(x.value, np.uint8) if x.value else None for x in ln]
+        self.ctx.check(self.ctx.lib.wg_relation_encode_segments(self.ctx.h, self.h, int(relation), int(reverse),
+                                                                *[_p(b) for b in bufs]))
+        out = dict(zip(("rows", "offsets", "targets", "weights"),
+                       (None if b is None else b.tobytes() for b in bufs)))
+        if out["rows"] is None:
+            del out["rows"]
+        return out
+
     @classmethod
     def _adopt(cls, ctx, h, n, weighted, folded=0):
         g = cls.__new__(cls)

@@ -116,6 +116,8 @@ SIGNATURES = [
     ("wg_graph_load_segments", I32, [P, U64, P, P, P]),
     ("wg_csr_segment_lengths", I32, [U64, U64, P, P, P]),
     ("wg_graph_encode_segments", I32, [P, P, I32, P, P, P]),
+    ("wg_relation_segment_lengths", I32, [P, P, U32, I32, P, P, P, P]),
+    ("wg_relation_encode_segments", I32, [P, P, U32, I32, P, P, P, P]),
     ("wg_host_alloc", I32, [P, U64, P]),
     ("wg_host_free", None, [P]),
     ("wg_bfs_cursor_create", I32, [P, P, I32, P, U32, P, U64, P, C.c_int64, C.c_int64, I32, P]),

@@ -351,3 +351,92 @@ extern "C" int wg_graph_encode_segments(wg_context* ctx, const wg_graph* g, int 
   }
   return ok(ctx);
 }
+
+// ---- a relation's CSR in the reference's layout ---------------------------
+// stores_sparse (graph/csr/sparse.rs:10-40): sparse offsets (a u32 row id
+// and a u64 offset per stored row, and the end offset) when they take at
+// most a quarter of the bytes of dense offsets; a graph's only relation is
+// its all-relations CSR, whose offsets are always dense.
+static bool stores_sparse(uint64_t stored, uint64_t id_space) {
+  const uint64_t sparse = stored * (4 + 8) + 8, dense = (id_space + 1) * 8;
+  return sparse * 4 <= dense;
+}
+
+static int relation_csr(wg_context* ctx, const wg_graph* g, uint32_t relation, int reverse, const DevCsr** out,
+                        bool* sparse, uint64_t* stored) {
+  if (int cs_ = check_graph(ctx, g)) return cs_;
+  if (relation >= g->relation_count)
+    return fail(ctx, WG_UNKNOWN_RELATION, "relation " + std::to_string(relation) + " outside the dictionary");
+  const wg_graph* r = g->relation_count > 1 ? g->relations[relation] : g;
+  const DevCsr& c = reverse ? r->rev : r->fwd;
+  uint64_t k = 0;
+  for (uint64_t v = 0; v < c.n; ++v) k += c.h_offsets[v + 1] > c.h_offsets[v];
+  *out = &c;
+  *stored = k;
+  *sparse = g->relation_count > 1 && stores_sparse(k, g->n);
+  return WG_OK;
+}
+
+extern "C" int wg_relation_segment_lengths(wg_context* ctx, const wg_graph* g, uint32_t relation, int reverse,
+                                           uint64_t* rows_len, uint64_t* offsets_len, uint64_t* targets_len,
+                                           uint64_t* weights_len) {
+  if (!ctx || !g) return WG_INVALID_ARGUMENT;
+  const DevCsr* c = nullptr;
+  bool sparse = false;
+  uint64_t k = 0;
+  if (int s = relation_csr(ctx, g, relation, reverse, &c, &sparse, &k)) return s;
+  if (rows_len) *rows_len = sparse ? kHeader + k * 4 : 0;
+  if (offsets_len) *offsets_len = kHeader + (sparse ? k + 1 : g->n + 1) * 8;
+  if (targets_len) *targets_len = kHeader + c->nnz * 4;
+  if (weights_len) *weights_len = g->weighted ? kHeader + c->nnz * 8 : 0;
+  return ok(ctx);
+}
+
+extern "C" int wg_relation_encode_segments(wg_context* ctx, const wg_graph* g, uint32_t relation, int reverse,
+                                           uint8_t* rows, uint8_t* offsets, uint8_t* targets, uint8_t* weights) {
+  if (!ctx || !g) return WG_INVALID_ARGUMENT;
+  const DevCsr* cp = nullptr;
+  bool sparse = false;
+  uint64_t k = 0;
+  if (int s = relation_csr(ctx, g, relation, reverse, &cp, &sparse, &k)) return s;
+  if (weights && !g->weighted) return fail(ctx, WG_INVALID_ARGUMENT, "the graph has no weights");
+  const DevCsr& c = *cp;
+  WG_HIP(ctx, hipSetDevice(ctx->device));
+  WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const std::vector<uint64_t>& off = c.h_offsets;
+  // encode_csr (csr.rs:123-130): rows (sparse only), offsets, targets, weights.
+  if (sparse) {
+    if (rows) {
+      put_header(rows, kRows, 4, k);
+      uint8_t* p = rows + kHeader;
+      for (uint64_t v = 0; v < c.n; ++v)
+        if (off[v + 1] > off[v]) {
+          const uint32_t id = (uint32_t)v;
+          memcpy(p, &id, 4);
+          p += 4;
+        }
+    }
+    if (offsets) {
+      put_header(offsets, kOffsets, 8, k + 1);
+      uint8_t* p = offsets + kHeader;
+      for (uint64_t v = 0; v < c.n; ++v)
+        if (off[v + 1] > off[v]) {
+          memcpy(p, &off[v], 8);
+          p += 8;
+        }
+      memcpy(p, &off[c.n], 8);
+    }
+  } else if (offsets) {
+    put_header(offsets, kOffsets, 8, g->n + 1);
+    memcpy(offsets + kHeader, off.data(), (g->n + 1) * 8);
+  }
+  if (targets) {
+    put_header(targets, kTargets, 4, c.nnz);
+    if (c.nnz) WG_HIP(ctx, hipMemcpy(targets + kHeader, c.targets, c.nnz * 4, hipMemcpyDeviceToHost));
+  }
+  if (weights) {
+    put_header(weights, kWeights, 8, c.nnz);
+    if (c.nnz) WG_HIP(ctx, hipMemcpy(weights + kHeader, c.weights, c.nnz * 8, hipMemcpyDeviceToHost));
+  }
+  return ok(ctx);
+}

@@ -124,3 +124,13 @@ def test_random_round_trips_dense_and_sparse():
         for seg in (S.encode_csr(off, tgt, w), S.sparse_form(off, tgt, w)):
             o, t, ww = S.decode_csr(n, seg.get("rows"), seg["offsets"], seg["targets"], seg["weights"])
             assert np.array_equal(o, off) and np.array_equal(t, tgt) and np.array_equal(ww, w)
+
+
+def test_stores_sparse_thresholds():
+    """graph/csr/sparse.rs:64-79: 1,000 nodes; 12 bytes a stored row at u32
+    targets, 16 at u64, plus 8."""
+    assert S.stores_sparse(0, 1000)
+    assert S.stores_sparse(166, 1000)
+    assert not S.stores_sparse(167, 1000)
+    assert S.stores_sparse(124, 1000, target_bytes=8)
+    assert not S.stores_sparse(125, 1000, target_bytes=8)

@@ -147,3 +147,65 @@ def test_reverse_segments_are_used(ctx, oracle):
     xv, yv = W.Vector(ctx, x), W.Vector(ctx, np.zeros(n))
     W.spmv(ctx, g, W.Product("min_plus", transpose=True), xv, yv)
     assert np.array_equal(yv.numpy(), oracle.spmv(ref, x, "min_plus", rev=rref, transpose=True))
+
+
+def test_relation_csrs_store_sparse_offsets_where_they_pay(ctx):
+    """graph/csr/graph_csr.rs:769-800: over 1,000 nodes relation a has rows 3
+    and 500 and is written with sparse offsets both ways; relation b has a
+    row from every node and is written dense; a one-relation graph is
+    always dense. The segments decode back to the relation's CSR."""
+    a_edges = [(500, 9, 1.0), (3, 7, 2.0), (500, 7, 3.0), (500, 9, 4.0)]
+    src = [e[0] for e in a_edges] + list(range(1000))
+    dst = [e[1] for e in a_edges] + [(v + 1) % 1000 for v in range(1000)]
+    w = [e[2] for e in a_edges] + [1.0] * 1000
+    rel = [0] * 4 + [1] * 1000
+    g = W.Graph.build(ctx, 1000, np.array(src), np.array(dst), np.array(w), relations=np.array(rel),
+                      relation_count=2)
+    fa = g.encode_relation_segments(0)
+    assert np.frombuffer(fa["rows"][16:], "<u4").tolist() == [3, 500]
+    assert np.frombuffer(fa["offsets"][16:], "<u8").tolist() == [0, 1, 3]
+    assert np.frombuffer(fa["targets"][16:], "<u4").tolist() == [7, 7, 9]
+    assert np.frombuffer(fa["weights"][16:], "<f8").tolist() == [2.0, 3.0, 5.0]
+    ra = g.encode_relation_segments(0, reverse=True)
+    assert np.frombuffer(ra["rows"][16:], "<u4").tolist() == [7, 9]
+    assert np.frombuffer(ra["offsets"][16:], "<u8").tolist() == [0, 2, 3]
+    assert np.frombuffer(ra["targets"][16:], "<u4").tolist() == [3, 500, 500]
+    for rev in (False, True):
+        b = g.encode_relation_segments(1, reverse=rev)
+        assert "rows" not in b and len(b["offsets"]) == 16 + 1001 * 8
+    # Bytes equal to the oracle's encoding of the same CSR, and a round trip.
+    ref = S.sparse_form([0] * 4 + [1] * 497 + [3] * 500, [7, 7, 9], [2.0, 3.0, 5.0])
+    assert fa == ref
+    back = W.Graph.load_segments(ctx, 1000, fa)
+    off, tgt, wt = back.forward()
+    assert off[501] == 3 and off[3] == 0 and off[4] == 1 and list(tgt) == [7, 7, 9]
+    one = W.Graph.build(ctx, 1000, np.array([3]), np.array([7]), np.array([1.0]))
+    assert "rows" not in one.encode_relation_segments(0)
+    with pytest.raises(W.WarrenError) as e:
+        g.encode_relation_segments(2)
+    assert e.value.name == "unknown_relation"
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_relation_segments_match_the_oracle(ctx, oracle, seed):
+    """Random relations over random id spaces: each relation's segments,
+    both ways, equal the oracle's encoding in the reference's layout."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(50, 3000))
+    R = 3
+    m = int(rng.integers(1, 4 * n))
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    # Relation 0 sparse-sized, the others spread.
+    rel = np.where(rng.random(m) < 0.05, 0, rng.integers(1, R, m))
+    src = np.where(rel == 0, rng.integers(0, max(n // 20, 1), m), src)
+    w = rng.uniform(-2, 2, m)
+    g = W.Graph.build(ctx, n, src, dst, w, relations=rel, relation_count=R)
+    csrs = oracle.relation_csrs(n, src, dst, w, rel, R)
+    for r in range(R):
+        c = csrs[r]
+        want_f = S.relation_form(n, c.offsets, c.targets, c.weights, relations=R)
+        t = c.transpose()
+        want_r = S.relation_form(n, t.offsets, t.targets, t.weights, relations=R)
+        assert g.encode_relation_segments(r) == want_f, r
+        assert g.encode_relation_segments(r, reverse=True) == want_r, r
