@@ -232,7 +232,8 @@ def bench_batched_ppr(W, ctx, g, n, deg, R=256):
             "speedup_vs_plain": round(t_plain / t_batch, 3),
             "iterations_min_max": [int(its.min()), int(its.max())],
             "iterations_equal_to_plain": it_equal,
-            "max_l1_batched_vs_plain": worst}
+            "max_l1_batched_vs_plain": worst,
+            "launch": "each iteration a captured HIP graph (tile passes, split rows, column finish)"}
 
 
 def bench_msbfs(W, ctx, g, n, deg, ns=512):
@@ -666,8 +667,11 @@ def main():
     mm_ms = timed(timer, run_mm, max(3, args.steps // 10))
     ctx.synchronize()
     mm_bytes = (n + 1) * 8 + m * 8 + n * width * 4 * 2
+    shape = W.spmm_shape(gf, "plus_times", "f32", width)
     legs["spmm_plus_times_f32_w16"] = {"ms": round(mm_ms, 3), "gteps": round(m * width / mm_ms / 1e6, 1),
-                                       "gbs": round(mm_bytes / mm_ms / 1e6, 1), "bytes": mm_bytes}
+                                       "gbs": round(mm_bytes / mm_ms / 1e6, 1), "bytes": mm_bytes,
+                                       "tuned_shape": None if shape is None else
+                                       {"lanes_per_row": shape[0], "column_groups_16_bytes": shape[1]}}
     log(f"spmm w16: {mm_ms:.2f} ms  {m * width / mm_ms / 1e6:.0f} G entry-columns/s  "
         f"{mm_bytes / mm_ms / 1e6:.0f} GB/s")
     xm.close()

@@ -131,6 +131,12 @@ int wg_synchronize(wg_context* ctx);
  * element type cannot fail (float products) return without waiting, for
  * back-to-back launches; errors surface at wg_synchronize. */
 void wg_set_blocking(wg_context* ctx, int blocking);
+/* Whether iterative calls replay each iteration's launch sequence as a
+ * captured HIP graph (1, the default: batched personalised PageRank captures
+ * one per message parity and active-column set) or enqueue the launches one
+ * by one (0). The results are the same either way. No counterpart in the
+ * reference. */
+void wg_set_graph_launch(wg_context* ctx, int on);
 /* Page-locked host memory for the buffers results are copied into (visit
  * rows, gathered values, vectors read back): a copy into it is one DMA, where
  * a copy into ordinary (pageable) memory goes through a staging buffer at a
@@ -323,6 +329,19 @@ int wg_spmv_noalloc(wg_context* ctx, const wg_graph* g, const wg_product* p, con
  * wg_spmv_variant reports the choice made (-1: not tuned yet). */
 void wg_set_spmv_variant(wg_context* ctx, int variant);
 int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, int weighted);
+/* SpMM launch shapes (no counterpart in the reference; the autotuner the
+ * batched workloads ask for): lanes per row (a power of two up to 64, each
+ * lane taking every lanes-th column group) | 256 when a group is 16 bytes
+ * of adjacent columns (4 f32 / u32, 2 f64 / i64) rather than one column.
+ * Every shape folds each column over the same entries in the same order,
+ * so the choice never changes a result. The first wg_spmm over a graph of
+ * 2^22 entries or more times up to six shapes at its width and keeps the
+ * fastest; wg_set_spmm_shape forces one (-1: autotune; a 16-byte grouping
+ * the width or alignment cannot take is dropped), wg_spmm_shape reports
+ * the choice kept for (orientation, dtype, semiring, weighted, width), -1
+ * if none. */
+void wg_set_spmm_shape(wg_context* ctx, int shape);
+int wg_spmm_shape(const wg_graph* g, int transpose, int dtype, int semiring, int weighted, uint64_t width);
 
 /* ---- PageRank: PageRank / pagerank (compute/pagerank/mod.rs:27-151) ----- */
 typedef struct {

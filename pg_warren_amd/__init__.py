@@ -123,6 +123,11 @@ class Context:
     def synchronize(self):
         self.check(self.lib.wg_synchronize(self.h))
 
+    def set_graph_launch(self, on):
+        """Replay iterative launch sequences as captured HIP graphs (default)
+        or enqueue them one by one."""
+        self.lib.wg_set_graph_launch(self.h, int(bool(on)))
+
     def set_blocking(self, blocking):
         self.lib.wg_set_blocking(self.h, int(blocking))
 
@@ -837,6 +842,20 @@ def set_spmv_variant(ctx, variant):
     hot-first gathers, LDS hot-column cache; 32: the tiled kernel), or -1 to
     autotune."""
     ctx.lib.wg_set_spmv_variant(ctx.h, variant)
+
+
+def set_spmm_shape(ctx, shape):
+    """Forces an SpMM launch shape (lanes per row | 256 for 16-byte column
+    groups), or -1 to autotune."""
+    ctx.lib.wg_set_spmm_shape(ctx.h, shape)
+
+
+def spmm_shape(graph, semiring, dtype, width, *, transpose=False, weighted=True):
+    """The SpMM autotuner's shape for this graph orientation and width, or -1:
+    (lanes per row, 16-byte column groups)."""
+    s = graph.ctx.lib.wg_spmm_shape(graph.h, int(transpose), DTYPES[dtype], semiring_index(semiring),
+                                    int(weighted), int(width))
+    return None if s < 0 else (s & 255, bool(s & 256))
 
 
 def spmv_variant(graph, semiring, dtype, *, transpose=False, weighted=True):

@@ -99,6 +99,7 @@ SIGNATURES = [
     ("wg_set_memory_limit", I32, [P, U64]),
     ("wg_synchronize", I32, [P]),
     ("wg_set_blocking", None, [P, I32]),
+    ("wg_set_graph_launch", None, [P, I32]),
     ("wg_stream", P, [P]),
     ("wg_graph_upload", I32, [P, U64, P, P, P, P]),
     ("wg_graph_destroy", None, [P]),
@@ -138,6 +139,8 @@ SIGNATURES = [
     ("wg_spmm", I32, [P, P, P, P, P]),
     ("wg_set_spmv_variant", None, [P, I32]),
     ("wg_spmv_variant", I32, [P, I32, I32, I32, I32]),
+    ("wg_set_spmm_shape", None, [P, I32]),
+    ("wg_spmm_shape", I32, [P, I32, I32, I32, I32, U64]),
     ("wg_pagerank", I32, [P, P, P, P, P, P, P]),
     ("wg_pagerank_batch", I32, [P, P, P, P, P, P, P, U32, P, P]),
     ("wg_pagerank_backward", I32, [P, P, P, P, P, P, P, P, P, P]),

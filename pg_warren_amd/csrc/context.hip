@@ -728,6 +728,10 @@ int wg_synchronize(wg_context* ctx) {
   return collect_error(ctx, "deferred kernel");
 }
 
+void wg_set_graph_launch(wg_context* ctx, int on) {
+  if (ctx) ctx->graph_launch = on != 0;
+}
+
 void wg_set_blocking(wg_context* ctx, int blocking) {
   if (ctx) ctx->blocking = blocking;
 }

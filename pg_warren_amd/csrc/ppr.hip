@@ -26,6 +26,7 @@
 // changes and dangling ranks go to per-workgroup partial slots that
 // ppr_finish sums in a fixed order. No atomics and no cross-workgroup fences.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <vector>
 
@@ -99,6 +100,7 @@ struct PprArgs {
   uint32_t nparts;           // slots: ppr_block then ppr_long workgroups
   unsigned long long* err;
   const unsigned long long* cancel;  // the context's host-mapped cancel word
+  unsigned* stop;                    // device word: the cancel word, forwarded
 };
 
 // A lane's columns: position t * kTw + kC * k + q is column c[q] when on[q].
@@ -198,12 +200,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
   const double* tile = a.Min + (uint64_t)t * a.nm * kTw + kC * k;
   double change[kC] = {}, dang[kC] = {};
   uint32_t e_err = 0;
-  __shared__ unsigned long long stop;
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    // The cancel word, polled once a work item (its load overlaps the
-    // item's staging); a raised word ends the pass, and the host reports
-    // the call cancelled.
-    if (tid == 0) stop = __hip_atomic_load(a.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __shared__ unsigned stop;
+  uint32_t polls = 0;
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x, ++polls) {
+    // Cancellation, checked once a work item (the load overlaps the item's
+    // staging): one workgroup in 64 reads the host-mapped cancel word every
+    // eighth item and forwards a raised word to a device word, which every
+    // workgroup reads (host reads from every workgroup at every item held
+    // the pass up on the bus). A raised word ends the pass, and the host
+    // reports the call cancelled.
+    if (tid == 0) {
+      if ((blockIdx.x & 63) == 0 && (polls & 7) == 0 &&
+          __hip_atomic_load(a.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+        __hip_atomic_store(a.stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      stop = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     const PprItem item = items[it];
     const bool piece = item.nrows == 0;
     const uint32_t nr = piece ? 1 : item.nrows;
@@ -638,6 +649,7 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   const size_t tiles_bytes = (size_t)ntiles * std::max<uint64_t>(nm, 1) * kTw * 8;
   double *X = (double*)ranks->data, *M0, *M1, *out, *part, *piece, *Pd, *dangling;
   uint32_t *seed_row_d, *cols_d, *from_d, *iters_d;
+  unsigned* stop_d;
   uint8_t* active_d;
   if ((s = get((void**)&M0, tiles_bytes)) || (s = get((void**)&M1, tiles_bytes)) ||
       (s = get((void**)&out, std::max<uint64_t>(n, 1) * 8)) ||
@@ -646,8 +658,10 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
       (s = get((void**)&Pd, P.size() * 8)) || (s = get((void**)&dangling, R * 8)) ||
       (s = get((void**)&seed_row_d, std::max<uint64_t>(n, 1) * 4)) ||
       (s = get((void**)&cols_d, R * 4)) || (s = get((void**)&from_d, R * 4)) ||
-      (s = get((void**)&iters_d, R * 4)) || (s = get((void**)&active_d, R)))
+      (s = get((void**)&iters_d, R * 4)) || (s = get((void**)&active_d, R)) ||
+      (s = get((void**)&stop_d, 4)))
     return s;
+  WG_HIP(ctx, hipMemsetAsync(stop_d, 0, 4, st));
   WG_HIP(ctx, hipMemcpyAsync(Pd, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
   if (n) WG_HIP(ctx, hipMemcpyAsync(seed_row_d, seed_row.data(), n * 4, hipMemcpyHostToDevice, st));
   WG_HIP(ctx, hipMemsetAsync(iters_d, 0, R * 4, st));
@@ -668,21 +682,63 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   const uint32_t max_iter = bound_err ? 1 : o->max_iter;
   std::vector<uint8_t> act(R, 1);
   double *Min = M0, *Mout = M1;
+  // An iteration's launches (every tile's persistent pass and split rows,
+  // then the columns' finish) are captured once as a HIP graph per message
+  // parity and replayed while the active columns stay the same; a repack
+  // captures them again. Kernel launches only: nothing in them is a memset.
+  struct Captured {
+    hipGraphExec_t exec = nullptr;
+    const double* min = nullptr;
+    uint32_t nact = 0;
+  } cap[2];
+  auto drop = [&]() {
+    for (auto& c : cap)
+      if (c.exec) {
+        (void)hipGraphExecDestroy(c.exec);
+        c = Captured{};
+      }
+  };
+  struct Dropper {
+    std::function<void()> f;
+    ~Dropper() { f(); }
+  } dropper{drop};
   for (uint32_t it = 0; it < max_iter && !active_cols.empty(); ++it) {
     if (is_cancelled(ctx)) return fail(ctx, WG_CANCELLED, "cancelled");
     const uint32_t nact = (uint32_t)active_cols.size();
     const uint32_t tiles = (nact + kTw - 1) / kTw;
     PprArgs a{rev.offsets, g->hot_src, wv, n, g->hot_index, nm, R, nact, cols_d, out, tomb, seed_row_d, Pd,
               o->damping, teleport, dangling, X, Min, Mout, part, piece, nparts, ctx->d_err,
-              ctx->d_mail + kMailCancel};
-    for (uint32_t t = 0; t < tiles; ++t) {
-      if (wv) ppr_block<true><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
-      else ppr_block<false><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
-      if (long_blocks)
-        ppr_long<<<long_blocks, 256, 0, st>>>(a, t, g->ppr_lrows, g->ppr_lbase, nlong, npieces, grid);
+              ctx->d_mail + kMailCancel, stop_d};
+    auto enqueue = [&]() {
+      for (uint32_t t = 0; t < tiles; ++t) {
+        if (wv) ppr_block<true><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
+        else ppr_block<false><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
+        if (long_blocks)
+          ppr_long<<<long_blocks, 256, 0, st>>>(a, t, g->ppr_lrows, g->ppr_lbase, nlong, npieces, grid);
+      }
+      ppr_finish<<<nact, 256, 0, st>>>(part, nparts, cols_d, dangling, iters_d, active_d, bound,
+                                       o->has_tol && !bound_err, 1);
+    };
+    if (ctx->graph_launch) {
+      Captured& c = cap[Min == M0 ? 0 : 1];
+      if (!c.exec || c.min != Min || c.nact != nact) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        c = Captured{};
+        hipGraph_t graph = nullptr;
+        WG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        const hipError_t ce = hipStreamEndCapture(st, &graph);
+        if (ce != hipSuccess) return hip_fail(ctx, ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return hip_fail(ctx, ie, "hipGraphInstantiate");
+        c.min = Min;
+        c.nact = nact;
+      }
+      WG_HIP(ctx, hipGraphLaunch(c.exec, st));
+    } else {
+      enqueue();
     }
-    ppr_finish<<<nact, 256, 0, st>>>(part, nparts, cols_d, dangling, iters_d, active_d, bound,
-                                     o->has_tol && !bound_err, 1);
     WG_HIP(ctx, hipGetLastError());
     std::swap(Min, Mout);
     WG_HIP(ctx, hipMemcpyAsync(act.data(), active_d, R, hipMemcpyDeviceToHost, st));

@@ -952,30 +952,81 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       // A lane takes 16 bytes of columns an entry when the rows of x split
       // into aligned 16-byte groups; else one column.
       constexpr int kC = (sizeof(E) == 4 || sizeof(E) == 8) ? (int)(16 / sizeof(E)) : 1;
-      const bool grouped = kC > 1 && width % kC == 0 && (uintptr_t)x % 16 == 0;
-      const uint64_t groups = grouped ? width / kC : width;
-      uint32_t lpr = 1;
-      while (lpr < groups && lpr < 64) lpr <<= 1;
-      const uint64_t grid = (csr.spmm_ntasks + (256 / lpr) - 1) / (256 / lpr);
-      if (grid == 0) return WG_OK;
+      const bool groupable = kC > 1 && width % kC == 0 && (uintptr_t)x % 16 == 0;
+      if (csr.spmm_ntasks == 0) return WG_OK;
       A* part = (A*)ctx->scratch;
-      auto run = [&](auto hw, auto cols) {
-        spmm_rows<E, ADD, MUL, decltype(hw)::value, decltype(cols)::value>
-            <<<(unsigned)grid, 256, 0, ctx->stream>>>(csr.targets, wv, x, y, width, csr.spmm_tasks,
-                                                      csr.spmm_ntasks, lpr, part, wa, ctx->d_err);
+      // A launch shape: lanes per row (a power of two, each lane taking
+      // every lpr-th column group) and whether a group is 16 bytes of
+      // columns or one. Every shape folds each column over the same entries
+      // in the same order, so which one runs never changes a result.
+      auto lanes_for = [&](bool grouped) {
+        const uint64_t groups = grouped ? width / kC : width;
+        uint32_t l = 1;
+        while (l < groups && l < 64) l <<= 1;
+        return l;
       };
-      if (wv) {
-        if (grouped) run(std::true_type{}, std::integral_constant<int, kC>{});
-        else run(std::true_type{}, std::integral_constant<int, 1>{});
-      } else {
-        if (grouped) run(std::false_type{}, std::integral_constant<int, kC>{});
-        else run(std::false_type{}, std::integral_constant<int, 1>{});
+      auto launch_mm = [&](int shape, E* out) {
+        const uint32_t lpr = (uint32_t)(shape & 255);
+        const bool grouped = (shape & 256) != 0;
+        const uint64_t grid = (csr.spmm_ntasks + (256 / lpr) - 1) / (256 / lpr);
+        auto run = [&](auto hw, auto cols) {
+          spmm_rows<E, ADD, MUL, decltype(hw)::value, decltype(cols)::value>
+              <<<(unsigned)grid, 256, 0, ctx->stream>>>(csr.targets, wv, x, out, width, csr.spmm_tasks,
+                                                        csr.spmm_ntasks, lpr, part, wa, ctx->d_err);
+        };
+        if (wv) {
+          if (grouped) run(std::true_type{}, std::integral_constant<int, kC>{});
+          else run(std::true_type{}, std::integral_constant<int, 1>{});
+        } else {
+          if (grouped) run(std::false_type{}, std::integral_constant<int, kC>{});
+          else run(std::false_type{}, std::integral_constant<int, 1>{});
+        }
+        if (csr.spmm_nlong) {
+          const uint64_t items = (uint64_t)csr.spmm_nlong * width;
+          spmm_long<E, ADD><<<(unsigned)((items + 255) / 256), 256, 0, ctx->stream>>>(
+              csr.spmm_long_rows, csr.spmm_pbase, csr.spmm_nlong, width, part, out, wa, ctx->d_err);
+        }
+      };
+      int shape = (int)lanes_for(groupable) | (groupable ? 256 : 0);
+      const uint64_t mkey = ((uint64_t)(((DtypeOf<E>::v * 16 + ADD) * 16 + MUL) * 2 + (wv ? 1 : 0)) << 32) | width;
+      if (ctx->spmm_shape > 0) {
+        shape = ctx->spmm_shape;
+        if ((shape & 256) && !groupable) shape &= 255;
+      } else if (csr.spmm_tuned.count(mkey)) {
+        shape = csr.spmm_tuned[mkey];
+      } else if (csr.nnz >= kTuneMinNnz) {
+        // Autotune the shape on this graph and width: each candidate timed
+        // into a scratch y; the fastest is kept with the CSR.
+        std::vector<int> cands;
+        for (int grouped = groupable ? 1 : 0; grouped >= 0; --grouped)
+          for (uint32_t l = lanes_for(grouped != 0), k = 0; k < 3 && l >= 1; l >>= 1, ++k)
+            cands.push_back((int)l | (grouped ? 256 : 0));
+        void* ys = nullptr;
+        if ((s = dalloc(ctx, &ys, csr.n * width * sizeof(E)))) return s;
+        hipEvent_t e0, e1;
+        WG_HIP(ctx, hipEventCreate(&e0));
+        WG_HIP(ctx, hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int c : cands) {
+          launch_mm(c, (E*)ys);
+          WG_HIP(ctx, hipEventRecord(e0, ctx->stream));
+          for (int r = 0; r < 3; ++r) launch_mm(c, (E*)ys);
+          WG_HIP(ctx, hipEventRecord(e1, ctx->stream));
+          WG_HIP(ctx, hipEventSynchronize(e1));
+          float ms = 0;
+          WG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            shape = c;
+          }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        dfree(ctx, ys, csr.n * width * sizeof(E));
+        csr.spmm_tuned[mkey] = shape;
+        if ((s = clear_error(ctx))) return s;
       }
-      if (csr.spmm_nlong) {
-        const uint64_t items = (uint64_t)csr.spmm_nlong * width;
-        spmm_long<E, ADD><<<(unsigned)((items + 255) / 256), 256, 0, ctx->stream>>>(
-            csr.spmm_long_rows, csr.spmm_pbase, csr.spmm_nlong, width, part, y, wa, ctx->d_err);
-      }
+      launch_mm(shape, y);
     }
     WG_HIP(ctx, hipGetLastError());
     return WG_OK;
@@ -1237,6 +1288,25 @@ int wg_spmv_variant(const wg_graph* g, int transpose, int dtype, int semiring, i
   const int key = ((dtype * 16 + add_of[semiring]) * 16 + mul) * 2 + (hw ? 1 : 0);
   auto it = c.tuned.find(key);
   return it == c.tuned.end() ? -1 : it->second;
+}
+
+int wg_spmm_shape(const wg_graph* g, int transpose, int dtype, int semiring, int weighted, uint64_t width) {
+  static const int add_of[13] = {wg::PLUS, wg::MIN, wg::MAX, wg::MAX, wg::MIN, wg::LOR, wg::ANY,
+                                 wg::PLUS, wg::PLUS, wg::PLUS, wg::MIN, wg::MIN, wg::MAX};
+  static const int mul_of[13] = {wg::TIMES, wg::PLUS, wg::PLUS, wg::TIMES, wg::TIMES, wg::LAND,
+                                 wg::PAIR, wg::PAIR, wg::FIRST, wg::SECOND, wg::FIRST, wg::SECOND,
+                                 wg::SECOND};
+  if (!g || semiring < 0 || semiring > 12) return -1;
+  const wg::DevCsr& c = transpose ? g->rev : g->fwd;
+  const int mul = mul_of[semiring];
+  const bool hw = weighted && c.weights && mul != wg::SECOND && mul != wg::PAIR;
+  const int key = ((dtype * 16 + add_of[semiring]) * 16 + mul) * 2 + (hw ? 1 : 0);
+  auto it = c.spmm_tuned.find(((uint64_t)key << 32) | width);
+  return it == c.spmm_tuned.end() ? -1 : it->second;
+}
+
+void wg_set_spmm_shape(wg_context* ctx, int shape) {
+  if (ctx) ctx->spmm_shape = shape;
 }
 
 int wg_spmv(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,

@@ -79,6 +79,9 @@ struct DevCsr {
   bool spmm_ready = false;
   // Autotuned SpMV launch variant per (dtype, monoid, multiply, weighted).
   std::map<int, int> tuned;
+  // Autotuned SpMM launch shape per (dtype, monoid, multiply, weighted,
+  // width): lanes per row | 256 when a lane takes 16-byte column groups.
+  std::map<uint64_t, int> spmm_tuned;
   // Hot-first column numbering for the gathers (built on first use): the
   // gathered columns ranked by how many entries read them (descending, ties
   // by id), the targets in that numbering, the original id of each rank,
@@ -150,6 +153,8 @@ struct wg_context {
   void* scratch = nullptr;              // partial results of split rows
   size_t scratch_bytes = 0;
   int spmv_variant = -1;  // -1: autotune; else the launch variant to force
+  int spmm_shape = -1;    // -1: autotune; else the SpMM launch shape to force
+  int graph_launch = 1;   // replay iterative launch sequences as captured HIP graphs
   void* static_scratch = nullptr;  // this device's copy of wg_spmv_noalloc's partials
   // The live objects handed out through this context. wg_context_destroy
   // releases their device memory and orphans them (their ctx becomes null),

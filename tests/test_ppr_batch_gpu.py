@@ -101,3 +101,31 @@ def test_rmat_scale16_256_seed_sets(ctx, oracle):
         want, wit = oracle.pagerank(ref, tol=1e-10, max_iter=100, seed=dense(ref.n, sets[c]))
         assert its[c] == wit
         assert np.abs(got[:, c] - want).sum() < L1_TOL
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_graph_replays_equal_eager_launches(ctx, weighted):
+    """Each iteration replays a captured HIP graph (one per message parity
+    and active-column set, re-captured after a repack), with the host's
+    own copies, repacks and synchronisations between replays; every
+    column equals the one-by-one launches' bit for bit, iteration counts
+    too, over runs whose columns converge at different iterations."""
+    off, tgt = W.rmat(14, 16, seed=3)
+    n = len(off) - 1
+    w = W.edge_weights(off, tgt, "lognormal", seed=2, p0=0.0, p1=1.0) if weighted else None
+    g = W.Graph(ctx, n, off, tgt, w)
+    rng = np.random.default_rng(8)
+    sets = seed_sets(rng, n, 40)
+    try:
+        ctx.set_graph_launch(False)
+        want, wit = W.pagerank_batch(ctx, g, sets, tol=1e-10, max_iter=80)
+        want = want.numpy()
+        ctx.set_graph_launch(True)
+        for _ in range(2):
+            got, it = W.pagerank_batch(ctx, g, sets, tol=1e-10, max_iter=80)
+            assert list(it) == list(wit)
+            assert got.numpy().tobytes() == want.tobytes()
+        assert len(set(wit)) > 1  # columns left at different iterations: repacks ran
+    finally:
+        ctx.set_graph_launch(True)
+    g.close()

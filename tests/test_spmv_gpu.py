@@ -263,6 +263,49 @@ def test_spmm_matches_the_oracle(ctx, oracle, width):
             assert got.tobytes() == want.tobytes(), (sr, dtype, width, transpose)
 
 
+@pytest.mark.parametrize("grouped", [0, 256])
+def test_every_spmm_shape_matches_the_oracle(ctx, oracle, grouped):
+    """Each launch shape the SpMM autotuner may pick (lanes per row 1..64,
+    one column or 16-byte column groups a lane) gives the oracle's bits."""
+    g = hub_graph(oracle, 11, n=2000, m=15000)
+    rng = np.random.default_rng(grouped)
+    try:
+        for lanes in (1, 2, 4, 8, 16, 32, 64):
+            W.set_spmm_shape(ctx, lanes | grouped)
+            for sr, dtype, width in [("plus_times", "f64", 8), ("min_plus", "u32", 12), ("plus_times", "f32", 16),
+                                     ("min_plus", "i64", 6), ("lor_land", "bool", 5)]:
+                x = values(dtype, g.n * width, rng, ADD_OF[sr]).reshape(g.n, width)
+                want = run_oracle(oracle, g, x, sr, dtype, width=width)
+                got = run_gpu(ctx, g, x, sr, dtype, spmm=True)
+                assert got.tobytes() == want.tobytes(), (lanes, grouped, sr, dtype, width)
+    finally:
+        W.set_spmm_shape(ctx, -1)
+
+
+def test_spmm_autotuner_keeps_a_shape_and_the_bits(ctx):
+    """On a graph of 2^22 entries or more the first SpMM tunes its shape; the
+    tuned product equals the product under the default shape bit for bit."""
+    off, tgt = W.rmat(19, 16, seed=12)
+    n = len(off) - 1
+    assert len(tgt) >= 1 << 22
+    wf = W.edge_weights(off, tgt, "uniform01", seed=4)
+    g = W.Graph(ctx, n, off, tgt, wf)
+    x = np.random.default_rng(3).random((n, 16)).astype(np.float32)
+    assert W.spmm_shape(g, "plus_times", "f32", 16) is None
+    got = W.spmm(ctx, g, W.Product("plus_times"), W.Vector(ctx, x),
+                 W.Vector(ctx, dtype="f32", id_space=n, width=16)).numpy()
+    shape = W.spmm_shape(g, "plus_times", "f32", 16)
+    assert shape is not None and shape[0] in (4, 2, 1, 16, 8) and isinstance(shape[1], bool)
+    W.set_spmm_shape(ctx, 4 | 256)  # the default: 4 lanes of 4 columns
+    try:
+        want = W.spmm(ctx, g, W.Product("plus_times"), W.Vector(ctx, x),
+                      W.Vector(ctx, dtype="f32", id_space=n, width=16)).numpy()
+    finally:
+        W.set_spmm_shape(ctx, -1)
+    assert got.tobytes() == want.tobytes()
+    g.close()
+
+
 def test_rmat_scale18_three_semirings(ctx, oracle):
     """The benchmark's three products on R-MAT scale 18: min_plus over u32
     and or_and over bool bit-exact, plus_times f32 within its bound."""
