@@ -156,6 +156,20 @@ void* wg_stream(wg_context* ctx);
  * NULL (unweighted: every weight reads as 1). Host pointers; copied. */
 int wg_graph_upload(wg_context* ctx, uint64_t n, const uint64_t* offsets,
                     const uint32_t* targets, const double* weights, wg_graph** out);
+/* The same with u64 targets (TargetWidth::U64, graph/csr/target.rs:16-31:
+ * the width warren-core uses for id spaces of 2^32 nodes or more; taken
+ * here for any id space). Csr::from_parts' checks run on the device; the
+ * reverse CSR is built by a stable radix sort by target. Such a graph is
+ * read by wg_spmv and wg_spmm (plain, transposed, masked, accumulated; a
+ * thread folds each output in the reference's order, so float plus too is
+ * the reference's left fold) and by wg_graph_read_u64; every other entry
+ * point, and a relation selection, returns WG_UNSUPPORTED for it. */
+int wg_graph_upload_u64(wg_context* ctx, uint64_t n, const uint64_t* offsets,
+                        const uint64_t* targets, const double* weights, wg_graph** out);
+/* A u64-target graph's forward (reverse = 0) or reverse CSR: host buffers
+ * of n+1, nnz and nnz (targets and weights may be NULL). */
+int wg_graph_read_u64(wg_context* ctx, const wg_graph* g, int reverse, uint64_t* offsets,
+                      uint64_t* targets, double* weights);
 /* Also releases any BFS cursor still reading the graph (the cursor's next
  * step then returns WG_INVALID_ARGUMENT; wg_bfs_cursor_destroy still frees
  * it). */

@@ -215,21 +215,35 @@ class Vector:
 class Graph:
     """A device GraphCsr: the merged all-relations CSR and its reverse."""
 
-    def __init__(self, ctx, n, offsets, targets, weights=None):
+    def __init__(self, ctx, n, offsets, targets, weights=None, target_width=None):
+        """target_width: 4 or 8 bytes; by default u64 targets exactly when the
+        id space is 2^32 or more (TargetWidth::for_id_space), else u32. A
+        graph with u64 targets serves products only."""
         self.ctx = ctx
         self.n = int(n)
         offsets = np.ascontiguousarray(offsets, np.uint64)
-        targets = np.ascontiguousarray(targets, np.uint32)
+        if target_width is None:
+            target_width = 8 if self.n >= 1 << 32 else 4
+        self.wide = target_width == 8
+        targets = np.ascontiguousarray(targets, np.uint64 if self.wide else np.uint32)
         if weights is not None:
             weights = np.ascontiguousarray(weights, np.float64)
         h = C.c_void_p()
-        ctx.check(ctx.lib.wg_graph_upload(ctx.h, self.n, _p(offsets), _p(targets), _p(weights),
-                                          C.byref(h)))
+        upload = ctx.lib.wg_graph_upload_u64 if self.wide else ctx.lib.wg_graph_upload
+        ctx.check(upload(ctx.h, self.n, _p(offsets), _p(targets), _p(weights), C.byref(h)))
         self.h = h
         ctx._children.add(self)
         self.nnz = int(offsets[-1])
         self.weighted = weights is not None
         self.deduplicated = 0
+
+    def read_u64(self, reverse=False):
+        """A u64-target graph's forward or reverse CSR: (offsets, targets, weights)."""
+        off = np.zeros(self.n + 1, np.uint64)
+        tgt = np.zeros(max(self.nnz, 1), np.uint64)
+        w = np.zeros(max(self.nnz, 1), np.float64) if self.weighted else None
+        self.ctx.check(self.ctx.lib.wg_graph_read_u64(self.ctx.h, self.h, int(reverse), _p(off), _p(tgt), _p(w)))
+        return off, tgt[:self.nnz], None if w is None else w[:self.nnz]
 
     @classmethod
     def build(cls, ctx, n, sources, targets, weights=None, relations=None, relation_count=1):
@@ -325,6 +339,7 @@ class Graph:
         g.nnz = int(nnz.value)
         g.weighted = weighted
         g.deduplicated = folded
+        g.wide = False
         return g
 
     def select(self, relations=None, per_relation=False):

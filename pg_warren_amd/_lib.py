@@ -102,6 +102,8 @@ SIGNATURES = [
     ("wg_set_graph_launch", None, [P, I32]),
     ("wg_stream", P, [P]),
     ("wg_graph_upload", I32, [P, U64, P, P, P, P]),
+    ("wg_graph_upload_u64", I32, [P, U64, P, P, P, P]),
+    ("wg_graph_read_u64", I32, [P, P, I32, P, P, P]),
     ("wg_graph_destroy", None, [P]),
     ("wg_graph_info", I32, [P, P, P, P]),
     ("wg_graph_read_reverse", I32, [P, P, P, P, P]),

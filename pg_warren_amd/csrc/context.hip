@@ -351,11 +351,12 @@ void destroy_graph(wg_graph* g) {
   delete g;
 }
 
-int check_graph(wg_context* ctx, const wg_graph* g) {
+int check_graph(wg_context* ctx, const wg_graph* g, bool wide_ok) {
   if (!g) return fail(ctx, WG_INVALID_ARGUMENT, "null graph");
   if (g->ctx != ctx)
     return fail(ctx, WG_INVALID_ARGUMENT,
                 g->ctx ? "the graph belongs to another context" : "the graph's context was destroyed");
+  if (!wide_ok) return check_narrow(ctx, g);
   return WG_OK;
 }
 
@@ -370,7 +371,8 @@ int check_vector(wg_context* ctx, const wg_vector* v) {
 void free_csr(wg_context* ctx, DevCsr& c) {
   dfree(ctx, c.offsets, (c.n + 1) * 8);
   dfree(ctx, c.targets, c.nnz * 4);
-  if (c.weights) dfree(ctx, c.weights, c.nnz * 8);
+  dfree(ctx, c.targets64, std::max<uint64_t>(c.nnz, 1) * 8);
+  if (c.weights) dfree(ctx, c.weights, c.targets64 ? std::max<uint64_t>(c.nnz, 1) * 8 : c.nnz * 8);
   dfree(ctx, c.blocks, c.nblocks * sizeof(Block));
   dfree(ctx, c.long_rows, std::max<uint32_t>(c.nlong, 1) * 4);
   dfree(ctx, c.long_pbase, (c.nlong + 1) * 4);

@@ -19,6 +19,7 @@
 
 #include "semiring.hpp"
 #include "spmv_tiles.hpp"
+#include "wide.hpp"
 #include "wg_internal.hpp"
 
 namespace wg {
@@ -794,6 +795,7 @@ int launch_product(wg_context* ctx, DevCsr& csr, bool weighted, const E* x, E* y
       int s = weight_view(ctx, csr, DtypeOf<E>::v, (const void**)&wv);
       if (s) return s;
     }
+    if (csr.targets64) return launch_wide<E, ADD, MUL>(ctx, csr, wv, x, y, width, wa);
     int s;
     if (width == 1) {
       if ((s = ensure_scratch(ctx, (size_t)std::max<uint32_t>(csr.npartials, 1) * sizeof(A)))) return s;
@@ -1074,7 +1076,7 @@ static int product_checks(wg_context* ctx, const wg_graph* g, const wg_product* 
                           wg_vector* y, bool spmv) {
   if (!ctx || !g || !p || !x || !y) return fail(ctx, WG_INVALID_ARGUMENT, "null argument");
   int s;
-  if ((s = check_graph(ctx, g)) || (s = check_vector(ctx, x)) || (s = check_vector(ctx, y)) ||
+  if ((s = check_graph(ctx, g, true)) || (s = check_vector(ctx, x)) || (s = check_vector(ctx, y)) ||
       (p->mask && (s = check_vector(ctx, p->mask))))
     return s;
   if (spmv) {
@@ -1106,6 +1108,7 @@ static int product(wg_context* ctx, const wg_graph* g, const wg_product* p, cons
   if (s) return s;
   WG_HIP(ctx, hipSetDevice(ctx->device));
   wg_graph* gm = const_cast<wg_graph*>(g);
+  if ((p->relations || p->per_relation) && (s = check_narrow(ctx, g))) return s;
   if ((p->relations || p->per_relation) &&
       (s = select_graph(ctx, gm, p->relations, p->nrelations, p->per_relation, &gm)))
     return s;
@@ -1234,6 +1237,7 @@ static int dispatch_noalloc(wg_context* ctx, DevCsr& csr, int semiring, bool wei
 
 static int product_noalloc(wg_context* ctx, const wg_graph* g, const wg_product* p, const wg_vector* x,
                            wg_vector* y) {
+  if (int cs_ = check_narrow(ctx, g)) return cs_;
   int s = product_checks(ctx, g, p, x, y, true);
   if (s) return s;
   if (p->relations || p->nrelations || p->per_relation)

@@ -55,6 +55,7 @@ struct DevCsr {
   uint64_t n = 0, nnz = 0;
   uint64_t* offsets = nullptr;   // n + 1
   uint32_t* targets = nullptr;   // nnz
+  uint64_t* targets64 = nullptr; // nnz, instead of targets: u64 targets (wg_graph_upload_u64)
   double* weights = nullptr;     // nnz or null
   std::vector<uint64_t> h_offsets;  // host copy of the offsets (partitioning)
   Block* blocks = nullptr;
@@ -246,7 +247,10 @@ void release_gathered(wg_gathered* r);
 bool cursor_reads(const wg_bfs_cursor* c, const wg_graph* g);
 // WG_INVALID_ARGUMENT unless g is a live graph of ctx (a graph of another
 // context, or one orphaned by wg_context_destroy, is refused).
-int check_graph(wg_context* ctx, const wg_graph* g);
+// wide_ok: a graph with u64 targets is accepted (products only).
+int check_graph(wg_context* ctx, const wg_graph* g, bool wide_ok = false);
+// WG_UNSUPPORTED for a graph with u64 targets.
+int check_narrow(wg_context* ctx, const wg_graph* g);
 int check_vector(wg_context* ctx, const wg_vector* v);
 // Whether two CSRs hold equal arrays (on the device).
 int same_csr(wg_context* ctx, const DevCsr& a, const DevCsr& b, bool* same);
