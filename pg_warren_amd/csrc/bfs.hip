@@ -95,6 +95,42 @@ __global__ __launch_bounds__(256) void bfs_seed(const uint32_t* __restrict__ sta
   }
 }
 
+// A search's set-up in one launch for at most kInitStarts starts: depth and
+// via unvisited, the three bitmaps (visited, frontier, claims) clear, and
+// the starts at depth 0 in the visited and frontier bitmaps. Each element
+// is written by one thread, its start bits included.
+constexpr uint32_t kInitStarts = 8;
+struct InitStarts {
+  uint32_t v[kInitStarts];
+  uint32_t count;
+};
+
+__global__ __launch_bounds__(256) void bfs_init(uint64_t n, uint64_t words, uint32_t* __restrict__ depth,
+                                                uint32_t* __restrict__ via, uint32_t* __restrict__ bitmaps,
+                                                InitStarts st) {
+  const uint64_t total = std::max<uint64_t>(n, 3 * words);
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+    if (i < n) {
+      uint32_t dv = kUnvisited;
+      for (uint32_t k = 0; k < st.count; ++k)
+        if (st.v[k] == i) dv = 0;
+      depth[i] = dv;
+      via[i] = kUnvisited;
+    }
+    if (i < 3 * words) {
+      // Words [0, words) visited, [words, 2 words) the frontier: a start's
+      // bit in both; [2 words, 3 words) the claims, clear.
+      uint32_t bits = 0;
+      if (i < 2 * words) {
+        const uint64_t wi = i < words ? i : i - words;
+        for (uint32_t k = 0; k < st.count; ++k)
+          if ((st.v[k] >> 5) == wi) bits |= 1u << (st.v[k] & 31);
+      }
+      bitmaps[i] = bits;
+    }
+  }
+}
+
 // Entries i, i + step, ... below b of frontier node u's row, pushed four
 // at a time: the targets' loads, then their visited words' loads, go out
 // together (a lane past the end reads node 0's word and drops it). An
@@ -681,15 +717,27 @@ struct BfsSearch {
     depth = depth_d;
     via = via_d;
     hipStream_t st = ctx->stream;
-    WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
-    WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
-    WG_HIP(ctx, hipMemsetAsync(w.visited, 0, 3 * words * 4, st));  // visited, bm_a, bm_b
     mf = visited_rev = 0;
-    if (!starts.empty()) {
+    if (starts.size() <= kInitStarts) {
+      // One launch: fills and seeds (w.visited, bm_a, bm_b are contiguous).
+      InitStarts is{};
+      is.count = (uint32_t)starts.size();
+      for (size_t k = 0; k < starts.size(); ++k) is.v[k] = starts[k];
+      const uint64_t total = std::max<uint64_t>(n, 3 * words);
+      bfs_init<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 8192)), 256, 0, st>>>(
+          n, words, depth, via, w.visited, is);
+      if (!starts.empty())
+        WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
+    } else {
+      WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
+      WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
+      WG_HIP(ctx, hipMemsetAsync(w.visited, 0, 3 * words * 4, st));  // visited, bm_a, bm_b
       WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
       bfs_seed<<<(unsigned)std::min<size_t>((starts.size() + 255) / 256, 1024), 256, 0, st>>>(
           w.q_a, (uint32_t)starts.size(), depth, w.visited, w.bm_a);
-      WG_HIP(ctx, hipGetLastError());
+    }
+    WG_HIP(ctx, hipGetLastError());
+    if (!starts.empty()) {
       for (uint32_t v : starts) {
         for (const DevCsr* cs : t.rows) mf += cs->h_offsets[v + 1] - cs->h_offsets[v];
         for (const DevCsr* cs : t.rrows) visited_rev += cs->h_offsets[v + 1] - cs->h_offsets[v];

@@ -251,6 +251,10 @@ bool cursor_reads(const wg_bfs_cursor* c, const wg_graph* g);
 int check_graph(wg_context* ctx, const wg_graph* g, bool wide_ok = false);
 // WG_UNSUPPORTED for a graph with u64 targets.
 int check_narrow(wg_context* ctx, const wg_graph* g);
+// Csr::from_parts' per-row checks on the device (targets in the id space,
+// rows strictly ascending) for offsets known to be monotone.
+template <class T>
+int check_rows_device(wg_context* ctx, uint64_t n, const uint64_t* off, const T* tgt, uint64_t m);
 int check_vector(wg_context* ctx, const wg_vector* v);
 // Whether two CSRs hold equal arrays (on the device).
 int same_csr(wg_context* ctx, const DevCsr& a, const DevCsr& b, bool* same);

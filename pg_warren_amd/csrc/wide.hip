@@ -44,11 +44,12 @@ __global__ __launch_bounds__(kT) void offsets_descent(const uint64_t* __restrict
 
 // The first rows (ascending) holding a target outside the id space and a
 // pair of targets not strictly ascending: bad[0], bad[1].
-__global__ __launch_bounds__(kT) void wide_entries(const uint64_t* __restrict__ off, const uint64_t* __restrict__ tgt,
-                                                   uint64_t m, uint64_t n, unsigned long long* __restrict__ bad) {
+template <class T>
+__global__ __launch_bounds__(kT) void csr_entries(const uint64_t* __restrict__ off, const T* __restrict__ tgt,
+                                                  uint64_t m, uint64_t n, unsigned long long* __restrict__ bad) {
   for (uint64_t e = gid(); e < m; e += gstride()) {
     const uint64_t t = tgt[e];
-    const bool out = t >= n, desc = e > 0 && tgt[e - 1] >= t;
+    const bool out = t >= n, desc = e > 0 && (uint64_t)tgt[e - 1] >= t;
     if (!out && !desc) continue;
     const uint64_t r = row_of_entry(off, n, e);
     if (out) atomicMin(&bad[0], (unsigned long long)r);
@@ -93,6 +94,34 @@ unsigned key_bits(uint64_t n) {
 }
 
 }  // namespace
+
+// Csr::from_parts' per-row checks (graph/csr/mod.rs:117-126) over a CSR
+// already on the device whose offsets are known to be monotone: the first
+// row holding a target outside the id space or targets not strictly
+// ascending, the former named first within a row.
+template <class T>
+int check_rows_device(wg_context* ctx, uint64_t n, const uint64_t* off, const T* tgt, uint64_t m) {
+  if (!m) return WG_OK;
+  hipStream_t st = ctx->stream;
+  unsigned long long* bad = nullptr;
+  int s;
+  if ((s = dalloc(ctx, (void**)&bad, 16))) return s;
+  unsigned long long hb[2] = {~0ull, ~0ull};
+  hipError_t e = hipMemsetAsync(bad, 0xff, 16, st);
+  if (e == hipSuccess) {
+    csr_entries<T><<<grid_of(m), kT, 0, st>>>(off, tgt, m, n, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hb, bad, 16, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dfree(ctx, bad, 16);
+  if (e != hipSuccess) return hip_fail(ctx, e, "row checks");
+  if (hb[0] != ~0ull && hb[0] <= hb[1])
+    return fail(ctx, WG_INVALID_CSR, "target outside id space in row " + std::to_string(hb[0]));
+  if (hb[1] != ~0ull) return fail(ctx, WG_INVALID_CSR, "row " + std::to_string(hb[1]) + " not sorted");
+  return WG_OK;
+}
+template int check_rows_device<uint32_t>(wg_context*, uint64_t, const uint64_t*, const uint32_t*, uint64_t);
 
 int check_narrow(wg_context* ctx, const wg_graph* g) {
   if (g && g->fwd.targets64)
@@ -153,17 +182,7 @@ extern "C" int wg_graph_upload_u64(wg_context* ctx, uint64_t n, const uint64_t* 
   WG_HIP(ctx, hipMemcpyAsync(hb, bad, 24, hipMemcpyDeviceToHost, st));
   WG_HIP(ctx, hipStreamSynchronize(st));
   if (hb[0] != ~0ull) return cleanup(fail(ctx, WG_INVALID_CSR, "offsets not monotone at row " + std::to_string(hb[0])));
-  if (m) {
-    wide_entries<<<grid_of(m), kT, 0, st>>>(f.offsets, f.targets64, m, n, bad + 1);
-    WG_HIP(ctx, hipGetLastError());
-    WG_HIP(ctx, hipMemcpyAsync(hb, bad, 24, hipMemcpyDeviceToHost, st));
-    WG_HIP(ctx, hipStreamSynchronize(st));
-    // The first row with either fault; a target outside the id space is
-    // named before the row's order (graph/csr/mod.rs:117-126).
-    if (hb[1] != ~0ull && hb[1] <= hb[2])
-      return cleanup(fail(ctx, WG_INVALID_CSR, "target outside id space in row " + std::to_string(hb[1])));
-    if (hb[2] != ~0ull) return cleanup(fail(ctx, WG_INVALID_CSR, "row " + std::to_string(hb[2]) + " not sorted"));
-  }
+  if ((s = check_rows_device<uint64_t>(ctx, n, f.offsets, f.targets64, m))) return cleanup(s);
   // The reverse CSR: entries stably sorted by target, each the source row
   // of its forward entry (ascending within a target, as transpose.rs
   // places them).

@@ -91,18 +91,28 @@ def test_u64_targets_give_the_oracles_products(ctx, oracle, dtype):
                         assert got.tobytes() == want.tobytes(), (sr, dtype, transpose, width, kw)
 
 
-def test_u64_upload_checks_and_refusals(ctx, oracle):
-    """Csr::from_parts' checks in its order, and the entry points that do
-    not read u64 targets refuse the graph."""
-    off = np.array([0, 2, 3], np.uint64)
-    for tgt, name in [([1, 0, 1], "row 0 not sorted"), ([0, 5, 1], "target outside id space in row 0"),
-                      ([0, 1, 7], "target outside id space in row 1")]:
+@pytest.mark.parametrize("width", [4, 8])
+def test_upload_checks_in_the_references_order(ctx, width):
+    """Csr::from_parts' checks (graph/csr/mod.rs:97-126), u32 and u64
+    targets alike, the per-row ones on the device: the first row with a
+    fault, a target outside the id space named before the row's order."""
+    off = np.array([0, 2, 3, 5], np.uint64)
+    for tgt, name in [([1, 0, 1, 0, 2], "row 0 not sorted"), ([0, 5, 1, 0, 1], "target outside id space in row 0"),
+                      ([0, 1, 7, 2, 1], "target outside id space in row 1"), ([0, 1, 2, 2, 1], "row 2 not sorted"),
+                      ([2, 1, 9, 0, 0], "row 0 not sorted"), ([0, 9, 1, 1, 0], "target outside id space in row 0")]:
         with pytest.raises(W.WarrenError) as e:
-            W.Graph(ctx, 2, off, np.array(tgt, np.uint64), target_width=8)
+            W.Graph(ctx, 3, off, np.array(tgt), target_width=width)
         assert e.value.name == "invalid_csr" and name in str(e.value), (tgt, str(e.value))
     with pytest.raises(W.WarrenError) as e:
-        W.Graph(ctx, 2, np.array([0, 2, 1], np.uint64), np.array([0, 1], np.uint64), target_width=8)
-    assert e.value.name == "invalid_csr" and "monotone" in str(e.value)
+        W.Graph(ctx, 3, np.array([0, 2, 1, 3], np.uint64), np.array([0, 1, 2]), target_width=width)
+    assert e.value.name == "invalid_csr" and "monotone at row 1" in str(e.value)
+    g = W.Graph(ctx, 3, off, np.array([0, 2, 1, 0, 2]), target_width=width)
+    g.close()
+
+
+def test_u64_refusals(ctx):
+    """The entry points that do not read u64 targets refuse such a graph."""
+    off = np.array([0, 2, 3], np.uint64)
     g = W.Graph(ctx, 2, off, np.array([0, 1, 0], np.uint64), target_width=8)
     for call in (lambda: W.pagerank(ctx, g), lambda: W.bfs(ctx, g, [0]),
                  lambda: W.gather(ctx, g, [0], np.array([1.0]), "sum"),
