@@ -40,7 +40,9 @@ namespace wg {
 
 constexpr uint32_t kTileEntries = 512;
 constexpr uint32_t kTileWaves = 16;
-constexpr uint32_t kTileHotBytes = 128 * 1024;
+// LDS for the hot columns: 152 KB of the CU's 160 KB for 4- and 8-byte
+// elements; 1-byte elements keep a power-of-two 128 K slots, read with a mask.
+constexpr uint32_t kTileHotBytes = 152 * 1024, kTileHotBytesByte = 128 * 1024;
 #ifndef WG_TILE_WARM_BYTES
 #define WG_TILE_WARM_BYTES (2u << 20)
 #endif
@@ -49,7 +51,7 @@ constexpr uint32_t kTileHot = 0x80000000u, kTileHead = 0x40000000u, kTileCol = 0
 
 template <class E>
 constexpr uint32_t tile_hot_cap() {
-  return kTileHotBytes / sizeof(E);
+  return sizeof(E) == 1 ? kTileHotBytesByte : kTileHotBytes / sizeof(E);
 }
 
 template <class E>
@@ -239,7 +241,10 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
       for (int q = 0; q < 8; ++q) {
         E xv = gq[q];
         if constexpr (RX) {
-          const E hv = hot[tq[q] & (HOTN - 1)];
+          // In bounds for every tag: cold entries read some slot and drop it.
+          const uint32_t slot =
+              (HOTN & (HOTN - 1)) == 0 ? (tq[q] & (HOTN - 1)) : min(tq[q] & kTileCol, HOTN - 1);
+          const E hv = hot[slot];
           xv = (tq[q] & kTileHot) ? hv : xv;
         }
         uint32_t ee = 0;
