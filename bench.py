@@ -834,7 +834,7 @@ def main():
         # stream (hot x staged, the tiled fold, the wave-boundary rows, the
         # row writes); the step's time, all four, is the figure.
         kernel_name = "spmv_tiles<float, 3, 4, true>"
-        step_kernels = ["gather_tiers<float>", kernel_name, "tile_fixup<float, 3>", "tile_write<float, 3>"]
+        step_kernels = ["gather_tiers<float>", kernel_name, "tile_write<float, 3>"]
     else:
         kernel_name = ("spmv_direct" if tuned >= 0 and tuned & 1 else "spmv_blocks") + \
             "<float, 3, 4, true>"

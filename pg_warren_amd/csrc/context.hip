@@ -398,6 +398,7 @@ void free_csr(wg_context* ctx, DevCsr& c) {
     dfree(ctx, c.tile_heads, std::max<uint32_t>(c.tile_nwaves, 1) * 4);
     dfree(ctx, c.nz_bits, ((c.n + 63) / 64 + 1) * 8);
     dfree(ctx, c.nz_pre, ((c.n + 63) / 64 + 1) * 4);
+    dfree(ctx, c.fix_bits, ((c.n + 63) / 64 + 1) * 8);
   }
   if (c.tile_out) dfree(ctx, c.tile_out, c.tile_out_bytes);
   if (c.spmm_ready) {

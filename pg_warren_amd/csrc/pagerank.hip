@@ -230,9 +230,9 @@ __global__ __launch_bounds__(kBlockThreads) void pr_iterate(PrArgs a, const doub
 // ---- the gather through the tiled SpMV (large graphs) ----------------------
 // On graphs of kPrTileMinNnz entries or more an iteration is: the hot and
 // warm message copies (pr_tiers), the tiled fold of each row's messages
-// (spmv_tiles.hpp, the stop flag as its halt word), the wave-boundary rows
-// (tile_fixup), then pr_tile_rows: advance and spread of every row from its
-// folded value, with fixed-order block partials of the L1 change and of the
+// (spmv_tiles.hpp, the stop flag as its halt word), then pr_tile_rows:
+// advance and spread of every row from its folded value (the rows a wave
+// leaves open joined from the wave records), with fixed-order block partials of the L1 change and of the
 // dangling rank, summed by pr_reduce (a block per span of partials) and
 // pr_finish_sum (one workgroup, the stop decision).
 constexpr uint64_t kPrTileMinNnz = 1ull << 22;
@@ -254,9 +254,8 @@ __global__ __launch_bounds__(256) void pr_tiers(const double* __restrict__ msg, 
 
 // Every row: its folded in-messages from the row's slot (0 for a row with no
 // in-edges, the empty sum), then advance and spread.
-__global__ __launch_bounds__(256) void pr_tile_rows(PrArgs a, const uint64_t* __restrict__ nz_bits,
-                                                    const uint32_t* __restrict__ nz_pre,
-                                                    const double* __restrict__ ycomp, uint64_t n,
+__global__ __launch_bounds__(256) void pr_tile_rows(PrArgs a, const TileGeom g, const double* __restrict__ ycomp,
+                                                    const TileRec<double>* __restrict__ recs, uint64_t n,
                                                     double* __restrict__ msg_out, double* __restrict__ bpart,
                                                     uint32_t nbp, int parity) {
   __shared__ double wsum[8];
@@ -265,10 +264,12 @@ __global__ __launch_bounds__(256) void pr_tile_rows(PrArgs a, const uint64_t* __
   double change = 0.0, dang = 0.0;
   uint32_t e_err = 0;
   if (r < n) {
-    const uint64_t wd = nz_bits[r >> 6];
+    const uint64_t wd = g.nz_bits[r >> 6];
     const uint32_t b = (uint32_t)(r & 63);
     double y = 0.0;
-    if ((wd >> b) & 1) y = ycomp[nz_pre[r >> 6] + __popcll(wd & ((1ull << b) - 1))];
+    if ((wd >> b) & 1)
+      y = tile_row_value<double, PLUS>(g, r, g.nz_pre[r >> 6] + __popcll(wd & ((1ull << b) - 1)),
+                                       (g.fix_bits[r >> 6] >> b) & 1, ycomp, recs);
     change = pr_row(a, r, y, a.st->dangling[parity], msg_out, dang, e_err);
   }
   const double c = block_sum(change, wsum);
@@ -589,8 +590,9 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
           spmv_tiles<double, PLUS, SECOND, false><<<tg, 1024, 0, st>>>(
               t.tagged, nullptr, mi, (const double*)rev.hot_x, t.nhot, (const double*)t.warm_x, (uint32_t)n,
               rev.tile_wave, rev.tile_heads, rev.nnz, ycomp, recs, ctx->d_jerr, &state->stop);
-        tile_fixup<double, PLUS><<<(rev.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, rev.tile_nwaves, ycomp);
-        pr_tile_rows<<<nbp, 256, 0, st>>>(a, rev.nz_bits, rev.nz_pre, ycomp, n, mo, bpart, nbp, parity);
+        const TileGeom geo{rev.nz_bits, rev.nz_pre, rev.fix_bits, rev.offsets, rev.n, rev.tile_padded / kTileEntries,
+                           rev.tile_nwaves};
+        pr_tile_rows<<<nbp, 256, 0, st>>>(a, geo, ycomp, recs, n, mo, bpart, nbp, parity);
         pr_reduce<<<nred, 256, 0, st>>>(a, bpart, nbp, kPrReduceSpan, red, nred);
         pr_finish_sum<<<1, 256, 0, st>>>(a, red, nred, parity);
         continue;

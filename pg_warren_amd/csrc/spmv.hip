@@ -655,25 +655,49 @@ int prepare_tiles(wg_context* ctx, DevCsr& c, size_t esize, uint32_t hotcap, uin
         if (off[r + 1] > off[r]) ++starts;
       heads[g] = (uint32_t)starts;
     }
+    // The rows a wave leaves open at its end (its last row start, when it
+    // holds one): their values are joined from the wave records.
+    std::vector<uint64_t> fix(words + 1, 0);
+    {
+      std::vector<uint64_t> last;  // ranks among non-empty rows, ascending
+      for (uint32_t g = 0; g < W; ++g) {
+        const uint64_t h1 = g + 1 < W ? heads[g + 1] : acc;
+        if (h1 > heads[g]) last.push_back(h1 - 1);
+      }
+      uint64_t j = 0;
+      size_t k = 0;
+      for (uint64_t r = 0; r < c.n && k < last.size(); ++r) {
+        if (off[r + 1] == off[r]) continue;
+        if (j == last[k]) {
+          fix[r >> 6] |= 1ull << (r & 63);
+          ++k;
+        }
+        ++j;
+      }
+    }
     uint32_t *dw = nullptr, *dh = nullptr, *dp = nullptr;
-    uint64_t* db = nullptr;
+    uint64_t *db = nullptr, *df = nullptr;
     if ((s = dalloc(ctx, (void**)&dw, (W + 1) * 4)) || (s = dalloc(ctx, (void**)&dh, W * 4)) ||
-        (s = dalloc(ctx, (void**)&db, (words + 1) * 8)) || (s = dalloc(ctx, (void**)&dp, (words + 1) * 4))) {
+        (s = dalloc(ctx, (void**)&db, (words + 1) * 8)) || (s = dalloc(ctx, (void**)&dp, (words + 1) * 4)) ||
+        (s = dalloc(ctx, (void**)&df, (words + 1) * 8))) {
       dfree(ctx, dw, (W + 1) * 4);
       dfree(ctx, dh, W * 4);
       dfree(ctx, db, (words + 1) * 8);
+      dfree(ctx, dp, (words + 1) * 4);
       return s;
     }
     WG_HIP(ctx, hipMemcpyAsync(dw, wave.data(), (W + 1) * 4, hipMemcpyHostToDevice, st));
     WG_HIP(ctx, hipMemcpyAsync(dh, heads.data(), W * 4, hipMemcpyHostToDevice, st));
     WG_HIP(ctx, hipMemcpyAsync(db, bits.data(), (words + 1) * 8, hipMemcpyHostToDevice, st));
     WG_HIP(ctx, hipMemcpyAsync(dp, pre.data(), (words + 1) * 4, hipMemcpyHostToDevice, st));
+    WG_HIP(ctx, hipMemcpyAsync(df, fix.data(), (words + 1) * 8, hipMemcpyHostToDevice, st));
     // The copies read host vectors that go out of scope here.
     WG_HIP(ctx, hipStreamSynchronize(st));
     c.tile_wave = dw;
     c.tile_heads = dh;
     c.nz_bits = db;
     c.nz_pre = dp;
+    c.fix_bits = df;
     c.nz_rows = acc;
     c.tile_nwaves = W;
     c.tile_padded = ntiles * kTileEntries;
@@ -775,9 +799,9 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
     spmv_tiles<E, ADD, MUL, false><<<grid, 1024, 0, st>>>(t.tagged, nullptr, x, (const E*)c.hot_x, t.nhot,
                                                           (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
                                                           c.nnz, ycomp, recs, ctx->d_jerr, nullptr);
-  tile_fixup<E, ADD><<<(c.tile_nwaves + 255) / 256, 256, 0, st>>>(recs, c.tile_nwaves, ycomp);
-  tile_write<E, ADD><<<(unsigned)((c.n + 255) / 256), 256, 0, st>>>(c.nz_bits, c.nz_pre, c.n, ycomp, y, wa,
-                                                                     ctx->d_jerr, ctx->d_err);
+  const TileGeom geo{c.nz_bits, c.nz_pre, c.fix_bits, c.offsets, c.n, c.tile_padded / kTileEntries, c.tile_nwaves};
+  tile_write<E, ADD><<<(unsigned)((c.n + 4 * 256 - 1) / (4 * 256)), 256, 0, st>>>(geo, ycomp, recs, y, wa,
+                                                                                ctx->d_jerr, ctx->d_err);
   WG_HIP(ctx, hipGetLastError());
   return WG_OK;
 }

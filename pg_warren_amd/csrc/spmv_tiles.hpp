@@ -26,10 +26,11 @@
 // the end of a tile is carried into the next tile. A finished row's value
 // goes to slot j (the row's rank among non-empty rows) of a compact array;
 // the run a wave begins inside a row and the run it leaves open at its end
-// go to the wave's record, which tile_fixup joins in wave order. tile_write
-// then writes every row (empty ones too) through the product's mask and
-// accumulate rules. Every join combines a run with the run that follows it,
-// so every monoid but float plus gives the reference's left fold exactly
+// go to the wave's record. tile_write then writes every row (empty ones
+// too) through the product's mask and accumulate rules, joining the rows
+// that waves leave open from the records in wave order. Every join combines
+// a run with the run that follows it, so every monoid but float plus gives
+// the reference's left fold exactly
 // (compute/product/kernel.rs:100-109); float plus is grouped by runs of up
 // to eight entries and their lane-order scan.
 #pragma once
@@ -352,48 +353,91 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
 
 // The rows a wave leaves open at its end, joined with the following waves'
 // leading runs in wave order.
+// Where the tiled layout puts a row's value: bit r of nz_bits marks a
+// non-empty row, whose rank among those is nz_pre[r / 64] plus the marked
+// rows before it in its word; bit r of fix_bits marks the last row start of
+// a wave, whose value is joined from the wave records.
+struct TileGeom {
+  const uint64_t* nz_bits;
+  const uint32_t* nz_pre;
+  const uint64_t* fix_bits;
+  const uint64_t* offsets;
+  uint64_t n;
+  uint64_t ntiles;
+  uint32_t nwaves;
+};
+
+// The value of non-empty row r of rank j. A row a wave leaves open is the
+// wave's open run joined, in wave order, with the runs that the following
+// waves hold before their first row start.
 template <class E, int ADD>
-__global__ __launch_bounds__(256) void tile_fixup(const TileRec<typename Fold<E, ADD>::A>* __restrict__ recs,
-                                                  uint32_t nwaves, typename Fold<E, ADD>::A* __restrict__ ycomp) {
+__device__ __forceinline__ typename Fold<E, ADD>::A tile_row_value(
+    const TileGeom& g, uint64_t r, uint32_t j, bool open, const typename Fold<E, ADD>::A* __restrict__ ycomp,
+    const TileRec<typename Fold<E, ADD>::A>* __restrict__ recs) {
   using F = Fold<E, ADD>;
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nwaves) return;
-  const TileRec<typename F::A> r = recs[i];
-  if (!(r.flags & 2)) return;
-  typename F::A v = r.S;
+  if (!open) return ycomp[j];
+  // Wave g holds tiles [ntiles * g / nwaves, ntiles * (g + 1) / nwaves); the
+  // row's start lies in the last wave whose first tile is at or before it.
+  const uint64_t t = g.offsets[r] / kTileEntries;
+  const uint32_t i = (uint32_t)min<uint64_t>(((t + 1) * g.nwaves + g.ntiles - 1) / g.ntiles - 1, g.nwaves - 1);
+  typename F::A v = recs[i].S;
   uint32_t ce = 0;
-  for (uint32_t k = i + 1; k < nwaves; ++k) {
+  for (uint32_t k = i + 1; k < g.nwaves; ++k) {
     const uint32_t fk = recs[k].flags;
     if (fk & 1) v = F::combine(v, recs[k].P, ce);
     if (fk & 2) break;
   }
-  ycomp[r.Sj] = v;
+  return v;
 }
 
-// Every row of y: a non-empty row's value from its slot, an empty row as
-// "no entry", through the mask / replace / accumulate rules.
+// Rows r0 .. r0 + 3 for a thread: each row's value (the identity for an
+// empty row, the empty fold) written through the product's mask and
+// accumulate rules; a plain write of four rows is one vector store.
 template <class E, int ADD>
-__global__ __launch_bounds__(256) void tile_write(const uint64_t* __restrict__ nz_bits,
-                                                  const uint32_t* __restrict__ nz_pre, uint64_t n,
-                                                  const typename Fold<E, ADD>::A* __restrict__ ycomp, E* __restrict__ y,
-                                                  WriteArgs wa, const unsigned long long* __restrict__ jerr,
+__global__ __launch_bounds__(256) void tile_write(const TileGeom g,
+                                                  const typename Fold<E, ADD>::A* __restrict__ ycomp,
+                                                  const TileRec<typename Fold<E, ADD>::A>* __restrict__ recs,
+                                                  E* __restrict__ y, WriteArgs wa,
+                                                  const unsigned long long* __restrict__ jerr,
                                                   unsigned long long* __restrict__ err) {
   using F = Fold<E, ADD>;
-  const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  const uint64_t wd = nz_bits[r >> 6];
-  const uint32_t b = r & 63;
-  const bool present = (wd >> b) & 1;
-  uint32_t e_err = 0;
-  E res = E();
-  if (present) {
-    const uint32_t j = nz_pre[r >> 6] + __popcll(wd & ((1ull << b) - 1));
-    res = F::finish(ycomp[j], e_err);
-    const unsigned long long je = *jerr;
-    if (je != ~0ull && (je >> 8) == j) e_err = (uint32_t)(je & 0xff);
+  const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (r0 >= g.n) return;
+  const uint64_t wd = g.nz_bits[r0 >> 6], fw = g.fix_bits[r0 >> 6];
+  const uint32_t b0 = (uint32_t)(r0 & 63);
+  const uint32_t cnt = (uint32_t)min<uint64_t>(4, g.n - r0);
+  uint32_t j = g.nz_pre[r0 >> 6] + __popcll(wd & ((1ull << b0) - 1));
+  const unsigned long long je = *jerr;
+  E res[4];
+  uint32_t e_err[4];
+  bool present[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    present[k] = k < cnt && ((wd >> (b0 + k)) & 1);
+    e_err[k] = 0;
+    res[k] = E();
+    if (present[k]) {
+      res[k] = F::finish(tile_row_value<E, ADD>(g, r0 + k, j, (fw >> (b0 + k)) & 1, ycomp, recs), e_err[k]);
+      if (je != ~0ull && (je >> 8) == j) e_err[k] = (uint32_t)(je & 0xff);
+      ++j;
+    }
   }
-  write_elem<E, ADD>(wa, r, &y[r], present, res, e_err);
-  if (e_err) raise_error(err, r, e_err);
+  struct alignas(4 * sizeof(E)) Row4 {
+    E v[4];
+  };
+  if (!wa.mask && wa.accumulate < 0 && cnt == 4 && ((uintptr_t)y & (sizeof(Row4) - 1)) == 0) {
+    Row4 o;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) o.v[k] = present[k] ? res[k] : identity<E, ADD>();
+    *(Row4*)(y + r0) = o;
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      if (k < cnt) write_elem<E, ADD>(wa, r0 + k, &y[r0 + k], present[k], res[k], e_err[k]);
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    if (e_err[k]) raise_error(err, r0 + k, e_err[k]);
 }
 
 }  // namespace wg

@@ -118,6 +118,7 @@ struct DevCsr {
   uint32_t tile_nwaves = 0;
   uint64_t* nz_bits = nullptr;     // (n + 63) / 64 words
   uint32_t* nz_pre = nullptr;      // (n + 63) / 64
+  uint64_t* fix_bits = nullptr;    // (n + 63) / 64 words: rows joined from wave records
   uint64_t nz_rows = 0;
   void* tile_out = nullptr;        // finished rows and wave records
   size_t tile_out_bytes = 0;
