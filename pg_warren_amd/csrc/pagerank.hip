@@ -243,12 +243,25 @@ __global__ __launch_bounds__(256) void pr_tiers(const double* __restrict__ msg, 
                                                 const uint32_t* __restrict__ warm, uint32_t nw,
                                                 double* __restrict__ xw, const PrState* __restrict__ st) {
   if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  // Four messages a thread, a grid apart, their gathers in flight together.
+  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x, T = (uint64_t)gridDim.x * 256;
   const uint64_t total = (uint64_t)nh + nw;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+  uint32_t id[4];
+  double v[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint64_t i = q + k * T;
+    id[k] = i < nh ? inv[i] : i < total ? warm[i - nh] : 0u;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) v[k] = msg[id[k]];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint64_t i = q + k * T;
     if (i < nh)
-      xh[i] = msg[inv[i]];
-    else
-      xw[i - nh] = msg[warm[i - nh]];
+      xh[i] = v[k];
+    else if (i < total)
+      xw[i - nh] = v[k];
   }
 }
 
@@ -579,7 +592,7 @@ extern "C" int wg_pagerank(wg_context* ctx, const wg_graph* gc, const wg_vector*
       if (tiled) {
         const DevCsr::Tiles& t = rev.tiles[(int)sizeof(double)];
         if (t.nhot || t.nwarm)
-          pr_tiers<<<(unsigned)std::min<uint64_t>(((uint64_t)t.nhot + t.nwarm + 255) / 256, 16384), 256, 0, st>>>(
+          pr_tiers<<<(unsigned)(((uint64_t)t.nhot + t.nwarm + 4 * 256 - 1) / (4 * 256)), 256, 0, st>>>(
               mi, rev.hot_inv, t.nhot, (double*)rev.hot_x, t.warm_ids, t.nwarm, (double*)t.warm_x, state);
         const unsigned tg = rev.tile_nwaves / kTileWaves;
         if (wt)

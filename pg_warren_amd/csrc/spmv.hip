@@ -84,19 +84,38 @@ __global__ __launch_bounds__(256) void gather_hot(const E* __restrict__ x, const
 
 // The tiled kernel's two x copies in one launch: the hot run in rank order
 // (xh[r] = x[inv[r]]) and the warm run in ascending column order
-// (xw[j] = x[warm[j]]).
+// (xw[j] = x[warm[j]]). A thread copies four elements a grid apart, their
+// gathers in flight together. Thread 0 also clears the error word jclear
+// when one is given.
 template <class E>
 __global__ __launch_bounds__(256) void gather_tiers(const E* __restrict__ x, const uint32_t* __restrict__ inv,
                                                     uint32_t nh, E* __restrict__ xh,
                                                     const uint32_t* __restrict__ warm, uint32_t nw,
-                                                    E* __restrict__ xw) {
+                                                    E* __restrict__ xw, unsigned long long* __restrict__ jclear) {
+  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x, T = (uint64_t)gridDim.x * 256;
   const uint64_t total = (uint64_t)nh + nw;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
-    if (i < nh)
-      xh[i] = x[inv[i]];
-    else
-      xw[i - nh] = x[warm[i - nh]];
+  if (jclear && q == 0) *jclear = ~0ull;
+  uint32_t id[4];
+  E v[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint64_t i = q + k * T;
+    id[k] = i < nh ? inv[i] : i < total ? warm[i - nh] : 0u;
   }
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) v[k] = x[id[k]];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint64_t i = q + k * T;
+    if (i < nh)
+      xh[i] = v[k];
+    else if (i < total)
+      xw[i - nh] = v[k];
+  }
+}
+
+inline unsigned gather_tiers_grid(uint32_t nh, uint32_t nw) {
+  return (unsigned)(((uint64_t)nh + nw + 4 * 256 - 1) / (4 * 256));
 }
 
 // A streamed load, non-temporal when nt (the row streams are read once and
@@ -784,11 +803,12 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
   }
   A* ycomp = (A*)c.tile_out;
   TileRec<A>* recs = (TileRec<A>*)((char*)c.tile_out + ybytes);
-  if (can_fail) tile_clear<<<1, 1, 0, st>>>(ctx->d_jerr);
   constexpr bool rx = reads_x<MUL>();
   if (rx && (t.nhot || t.nwarm))
-    gather_tiers<E><<<(unsigned)std::min<uint64_t>(((uint64_t)t.nhot + t.nwarm + 255) / 256, 16384), 256, 0, st>>>(
-        x, c.hot_inv, t.nhot, (E*)c.hot_x, t.warm_ids, t.nwarm, (E*)t.warm_x);
+    gather_tiers<E><<<gather_tiers_grid(t.nhot, t.nwarm), 256, 0, st>>>(
+        x, c.hot_inv, t.nhot, (E*)c.hot_x, t.warm_ids, t.nwarm, (E*)t.warm_x, can_fail ? ctx->d_jerr : nullptr);
+  else if (can_fail)
+    tile_clear<<<1, 1, 0, st>>>(ctx->d_jerr);
   const unsigned grid = c.tile_nwaves / kTileWaves;
   const uint32_t ncols = (uint32_t)c.n;
   if (wt)
@@ -800,8 +820,9 @@ static int launch_tiles(wg_context* ctx, DevCsr& c, const E* wv, const E* x, E* 
                                                           (const E*)t.warm_x, ncols, c.tile_wave, c.tile_heads,
                                                           c.nnz, ycomp, recs, ctx->d_jerr, nullptr);
   const TileGeom geo{c.nz_bits, c.nz_pre, c.fix_bits, c.offsets, c.n, c.tile_padded / kTileEntries, c.tile_nwaves};
-  tile_write<E, ADD><<<(unsigned)((c.n + 4 * 256 - 1) / (4 * 256)), 256, 0, st>>>(geo, ycomp, recs, y, wa,
-                                                                                ctx->d_jerr, ctx->d_err);
+  constexpr uint64_t wrows = 256 * kTileWriteRows;
+  tile_write<E, ADD><<<(unsigned)((c.n + wrows - 1) / wrows), 256, 0, st>>>(geo, ycomp, recs, y, wa, ctx->d_jerr,
+                                                                         ctx->d_err);
   WG_HIP(ctx, hipGetLastError());
   return WG_OK;
 }
