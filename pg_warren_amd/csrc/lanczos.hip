@@ -472,8 +472,12 @@ __global__ __launch_bounds__(256) void store_column(const double* __restrict__ x
 // Givens rotations accumulated into Q. a is overwritten; qt receives Q^T
 // (row l = the eigenvector of values[l]); *status 0 on success, 1 when a
 // value is not finite or the QR steps do not settle. Reductions are in a
-// fixed order, so the result does not vary from run to run.
+// fixed order, so the result does not vary from run to run. Up to
+// kEigLdsM rows, Q^T lives in LDS while the rotations run (each sweep's
+// rotations are a dependent chain down a column) and is copied out at the
+// end.
 constexpr int kEigThreads = 256;
+constexpr int kEigLdsM = 128;
 
 // Doubles of device workspace device_sym_eigen needs for an m x m matrix.
 constexpr uint64_t eigen_workspace(uint64_t m) { return 2 * m * m + m + 1; }
@@ -489,16 +493,19 @@ __device__ double eig_sum(double v, double* sh) {
   return t;
 }
 
+template <bool LDSQ>
 __global__ __launch_bounds__(kEigThreads) void sym_eigen_dev(double* __restrict__ a, double* __restrict__ qt,
                                                              int m, double* __restrict__ values,
                                                              int* __restrict__ status) {
   __shared__ double v[kMaxBasis], p[kMaxBasis], w[kMaxBasis], t[kMaxBasis];
   __shared__ double d[kMaxBasis], e[kMaxBasis], cs[kMaxBasis], sn[kMaxBasis];
   __shared__ double sh[kEigThreads / 64];
+  __shared__ double qs[LDSQ ? kEigLdsM * kEigLdsM : 1];
   __shared__ int ctl[4];  // l, h, done, failed
   const int tid = threadIdx.x;
+  double* const q_ = LDSQ ? qs : qt;
   auto A = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
-  auto Qt = [&](int k, int r) -> double& { return qt[(size_t)k * m + r]; };
+  auto Qt = [&](int k, int r) -> double& { return q_[(size_t)k * m + r]; };
   int bad = 0;
   for (int idx = tid; idx < m * m; idx += kEigThreads) {
     Qt(idx / m, idx % m) = (idx / m == idx % m) ? 1.0 : 0.0;
@@ -622,16 +629,21 @@ __global__ __launch_bounds__(kEigThreads) void sym_eigen_dev(double* __restrict_
     // The sweep's rotations, a column of Q^T per thread, in order.
     const int l = ctl[0], h = ctl[1];
     for (int r = tid; r < m; r += kEigThreads) {
-      double u = Qt(l, r);
+      // Element k + 2 is read before element k is written, a step ahead.
+      double u = Qt(l, r), u1 = Qt(l + 1, r);
       for (int k = l; k < h; ++k) {
-        const double c = cs[k], s = sn[k], u1 = Qt(k + 1, r);
+        const double nx = k + 2 <= h ? Qt(k + 2, r) : 0.0;
+        const double c = cs[k], s = sn[k];
         Qt(k, r) = c * u + s * u1;
         u = -s * u + c * u1;
+        u1 = nx;
       }
       Qt(h, r) = u;
     }
     __syncthreads();
   }
+  if (LDSQ)
+    for (int idx = tid; idx < m * m; idx += kEigThreads) qt[idx] = qs[idx];
   bad = 0;
   for (int i = tid; i < m; i += kEigThreads) {
     values[i] = d[i];
@@ -654,7 +666,10 @@ static int device_sym_eigen(wg_context* ctx, double* ws, const std::vector<doubl
   double* dv = ws + 2 * mm;
   int* dst = (int*)(ws + 2 * mm + m);
   WG_HIP(ctx, hipMemcpyAsync(da, a.data(), mm * 8, hipMemcpyHostToDevice, st));
-  sym_eigen_dev<<<1, kEigThreads, 0, st>>>(da, dq, m, dv, dst);
+  if (m <= kEigLdsM)
+    sym_eigen_dev<true><<<1, kEigThreads, 0, st>>>(da, dq, m, dv, dst);
+  else
+    sym_eigen_dev<false><<<1, kEigThreads, 0, st>>>(da, dq, m, dv, dst);
   WG_HIP(ctx, hipGetLastError());
   std::vector<double> qt(mm);
   values.resize(m);
