@@ -25,8 +25,9 @@
 // count / scan / write compaction with per-block partials; the scan posts
 // the counters to the context's mailbox (coherent host memory), which the
 // host polls, so a level costs no copy or stream synchronisation. The hot
-// path has no device-scope atomics (on gfx950 they resolve past the per-XCD
-// L2 and serialise on a shared address).
+// path has no device-scope atomics on a shared address (on gfx950 they
+// resolve past the per-XCD L2 and serialise: an arrival counter bumped by
+// each of the count pass's 2048 blocks cost more than the scan's launch).
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -38,7 +39,7 @@
 namespace wg {
 
 constexpr uint32_t kUnvisited = 0xffffffffu;
-constexpr uint32_t kWordsPerCountBlock = 512;  // bfs_count / bfs_write: 2 words a thread
+constexpr uint32_t kWordsPerCountBlock = 64;  // bfs_count / bfs_write: 8 nodes a thread
 // A push level whose frontier rows hold fewer than n / kSmallPushRatio
 // entries sets vias by atomicMin while marking (bfs_push<true>).
 constexpr uint64_t kSmallPushRatio = 4;
@@ -196,25 +197,19 @@ __global__ __launch_bounds__(256) void bfs_push(const uint32_t* __restrict__ que
   }
 }
 
-// The claim bits of 32-node word w from the candidate bytes, which are
-// cleared.
-__device__ __forceinline__ uint32_t claim_word(uint64_t n, uint64_t w, uint8_t* __restrict__ cand) {
-  const uint64_t v0 = w * 32;
+// The claim bits of the eight nodes from v0 (a multiple of 8) from the
+// candidate bytes, which are cleared.
+__device__ __forceinline__ uint32_t claim_byte(uint64_t n, uint64_t v0, uint8_t* __restrict__ cand) {
   uint32_t bits = 0;
-  if (v0 + 32 <= n) {
-    uint4 c[2];
-    c[0] = ((const uint4*)(cand + v0))[0];
-    c[1] = ((const uint4*)(cand + v0))[1];
-    const uint32_t* cw = (const uint32_t*)c;
-    for (int k = 0; k < 8; ++k)
-      for (int j = 0; j < 4; ++j)
-        if ((cw[k] >> (8 * j)) & 0xffu) bits |= 1u << (k * 4 + j);
-    if (bits) {
-      ((uint4*)(cand + v0))[0] = make_uint4(0, 0, 0, 0);
-      ((uint4*)(cand + v0))[1] = make_uint4(0, 0, 0, 0);
+  if (v0 + 8 <= n) {
+    const uint2 c = *(const uint2*)(cand + v0);
+    for (int j = 0; j < 4; ++j) {
+      if ((c.x >> (8 * j)) & 0xffu) bits |= 1u << j;
+      if ((c.y >> (8 * j)) & 0xffu) bits |= 1u << (4 + j);
     }
+    if (bits) *(uint2*)(cand + v0) = make_uint2(0, 0);
   } else {
-    for (uint64_t v = v0; v < n && v < v0 + 32; ++v)
+    for (uint64_t v = v0; v < n && v < v0 + 8; ++v)
       if (cand[v]) {
         bits |= 1u << (v - v0);
         cand[v] = 0;
@@ -391,63 +386,19 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
   }
 }
 
-// Compaction, first pass: per block of words, the claimed nodes and their
-// row entries in the rows read and in the reversed rows (summed over the
-// CSRs of each); also folds the claims into visited and
-// clears the old frontier bitmap (it becomes the next level's claim map).
-// kClaim (after a small push, whose vias are set): the claims come from the
-// candidate bytes here, and the claimed nodes take depth d1.
-template <bool kClaim>
-__global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uint32_t* __restrict__ claim,
-                                                 uint32_t* __restrict__ visited,
-                                                 uint32_t* __restrict__ old_frontier, const RowSets adj,
-                                                 const RowSets rev, unsigned long long* __restrict__ part,
-                                                 uint8_t* __restrict__ cand, uint32_t* __restrict__ depth,
-                                                 uint32_t d1) {
+// Compaction, second pass (one workgroup of 256): an exclusive scan of the
+// blocks' counts into offsets, the entry sums and the pulled level's
+// scanned entries, posted to the context's mailbox (values first, then the
+// sequence number with release).
+__global__ __launch_bounds__(256) void bfs_scan(uint32_t nb, const unsigned long long* __restrict__ part,
+                                                unsigned long long* __restrict__ offsets,
+                                                const unsigned long long* __restrict__ part_edges,
+                                                uint32_t nedges, unsigned long long* __restrict__ mail,
+                                                unsigned long long seq) {
+  __shared__ unsigned long long sh[256];
   __shared__ unsigned long long red[4];
-  unsigned long long cnt = 0, ent = 0, vr = 0;
-  const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerCountBlock + threadIdx.x * 2;
-  for (uint64_t w = w0; w < w0 + 2 && w < words; ++w) {
-    uint32_t bits;
-    if (kClaim) {
-      bits = claim_word(n, w, cand);
-      claim[w] = bits;
-    } else {
-      bits = claim[w];
-    }
-    visited[w] |= bits;
-    old_frontier[w] = 0;
-    cnt += __popc(bits);
-    while (bits) {
-      const uint32_t v = (uint32_t)(w * 32) + (__ffs(bits) - 1);
-      bits &= bits - 1;
-      if (kClaim) depth[v] = d1;
-      ent += degree_in(adj, v);
-      vr += degree_in(rev, v);
-    }
-  }
-  cnt = block_reduce_sum(cnt, red);
-  ent = block_reduce_sum(ent, red);
-  vr = block_reduce_sum(vr, red);
-  if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = cnt;
-    part[3 * blockIdx.x + 1] = ent;
-    part[3 * blockIdx.x + 2] = vr;
-  }
-}
-
-// Compaction, second pass (one workgroup): the blocks' queue offsets by an
-// exclusive scan of their counts, and the level's totals, posted to the
-// context's mailbox.
-__global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long* __restrict__ part,
-                                                 unsigned long long* __restrict__ offsets,
-                                                 const unsigned long long* __restrict__ part_edges,
-                                                 uint32_t nedges, unsigned long long* __restrict__ mail,
-                                                 unsigned long long seq) {
-  __shared__ unsigned long long sh[1024];
-  __shared__ unsigned long long red[16];
   const uint32_t tid = threadIdx.x;
-  const uint32_t per = (nb + 1023) / 1024, b0 = tid * per;
+  const uint32_t per = (nb + 255) / 256, b0 = tid * per;
   unsigned long long local = 0, ent = 0, vr = 0, ed = 0;
   for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
     local += part[3 * b];
@@ -458,14 +409,14 @@ __global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long
     // Eight loads in flight a thread (the partials are many after a pull).
     unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t k = tid;
-    for (; k + 7 * 1024 < nedges; k += 8 * 1024)
-      for (int j = 0; j < 8; ++j) acc[j] += part_edges[k + j * 1024];
-    for (; k < nedges; k += 1024) acc[0] += part_edges[k];
+    for (; k + 7 * 256 < nedges; k += 8 * 256)
+      for (int j = 0; j < 8; ++j) acc[j] += part_edges[k + j * 256];
+    for (; k < nedges; k += 256) acc[0] += part_edges[k];
     for (int j = 0; j < 8; ++j) ed += acc[j];
   }
   sh[tid] = local;
   __syncthreads();
-  for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+  for (uint32_t d = 1; d < 256; d <<= 1) {  // inclusive Hillis-Steele scan
     unsigned long long t = tid >= d ? sh[tid - d] : 0;
     __syncthreads();
     sh[tid] += t;
@@ -480,11 +431,92 @@ __global__ __launch_bounds__(1024) void bfs_scan(uint32_t nb, unsigned long long
   vr = block_reduce_sum(vr, red);
   ed = block_reduce_sum(ed, red);
   if (tid == 0) {
-    // The level's counters (BfsCounters' order) posted to the host's
-    // mailbox: values first, then the sequence number with release.
-    const unsigned long long v[4] = {sh[1023], ent, vr, ed};
+    // BfsCounters' order.
+    const unsigned long long v[4] = {sh[255], ent, vr, ed};
     for (int i = 0; i < 4; ++i) __hip_atomic_store(&mail[1 + i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&mail[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Compaction, first pass: per block of words, the claimed nodes and their
+// row entries in the rows read and in the reversed rows (summed over the
+// CSRs of each); also folds the claims into visited and clears the old
+// frontier bitmap (it becomes the next level's claim map). A thread takes
+// eight nodes, their offsets read together (2048 blocks at 2^22 nodes, so
+// that enough loads are in flight).
+// kClaim (after a small push, whose vias are set): the claims come from the
+// candidate bytes here, and the claimed nodes take depth d1.
+template <bool kClaim>
+__global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uint32_t* __restrict__ claim,
+                                                 uint32_t* __restrict__ visited,
+                                                 uint32_t* __restrict__ old_frontier, const RowSets adj,
+                                                 const RowSets rev, unsigned long long* __restrict__ part,
+                                                 uint8_t* __restrict__ cand, uint32_t* __restrict__ depth,
+                                                 uint32_t d1) {
+  __shared__ unsigned long long red[4];
+  unsigned long long cnt = 0, ent = 0, vr = 0;
+  // Four threads a word, eight nodes each (the lanes of a word are adjacent
+  // in the wave).
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x, w = t >> 2;
+  const uint32_t sub = (uint32_t)(t & 3);
+  uint32_t word = 0;
+  if (kClaim) {
+    if (w < words) word = claim_byte(n, w * 32 + 8 * sub, cand) << (8 * sub);
+    word |= __shfl_xor(word, 1, 64);
+    word |= __shfl_xor(word, 2, 64);
+  } else if (w < words) {
+    word = claim[w];
+  }
+  if (w < words && sub == 0) {
+    if (kClaim) claim[w] = word;
+    visited[w] |= word;
+    old_frontier[w] = 0;
+  }
+  uint32_t bits = (word >> (8 * sub)) & 0xffu;
+  cnt = __popc(bits);
+  if (bits) {
+    constexpr int B = 8;
+    uint32_t v[B];
+    uint32_t live = 0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      v[k] = (uint32_t)(w * 32 + 8 * sub) + (bits ? __ffs(bits) - 1 : 0);
+      live |= (bits ? 1u : 0u) << k;
+      bits &= bits - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+      if (kClaim && ((live >> k) & 1)) depth[v[k]] = d1;
+    for (uint32_t j = 0; j < adj.k; ++j) {
+      uint64_t lo[B], hi[B];
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        lo[k] = adj.off[j][v[k]];
+        hi[k] = adj.off[j][v[k] + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < B; ++k)
+        if ((live >> k) & 1) ent += hi[k] - lo[k];
+    }
+    for (uint32_t j = 0; j < rev.k; ++j) {
+      uint64_t lo[B], hi[B];
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        lo[k] = rev.off[j][v[k]];
+        hi[k] = rev.off[j][v[k] + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < B; ++k)
+        if ((live >> k) & 1) vr += hi[k] - lo[k];
+    }
+  }
+  cnt = block_reduce_sum(cnt, red);
+  ent = block_reduce_sum(ent, red);
+  vr = block_reduce_sum(vr, red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = cnt;
+    part[3 * blockIdx.x + 1] = ent;
+    part[3 * blockIdx.x + 2] = vr;
   }
 }
 
@@ -494,26 +526,23 @@ __global__ __launch_bounds__(256) void bfs_write(uint64_t words, const uint32_t*
                                                  const unsigned long long* __restrict__ offsets,
                                                  uint32_t* __restrict__ queue) {
   __shared__ uint32_t sh[256];
-  const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerCountBlock + threadIdx.x * 2;
-  uint32_t bits[2] = {0, 0};
-  for (int k = 0; k < 2; ++k)
-    if (w0 + k < words) bits[k] = claim[w0 + k];
-  const uint32_t mine = __popc(bits[0]) + __popc(bits[1]);
+  // Four threads a word, eight nodes each, in node order.
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x, w = t >> 2;
+  const uint32_t sub = (uint32_t)(t & 3);
+  uint32_t b = w < words ? (claim[w] >> (8 * sub)) & 0xffu : 0u;
+  const uint32_t mine = __popc(b);
   sh[threadIdx.x] = mine;
   __syncthreads();
   for (uint32_t d = 1; d < 256; d <<= 1) {
-    uint32_t t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+    uint32_t x = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
     __syncthreads();
-    sh[threadIdx.x] += t;
+    sh[threadIdx.x] += x;
     __syncthreads();
   }
   unsigned long long pos = offsets[blockIdx.x] + sh[threadIdx.x] - mine;
-  for (int k = 0; k < 2; ++k) {
-    uint32_t b = bits[k];
-    while (b) {
-      queue[pos++] = (uint32_t)((w0 + k) * 32) + (__ffs(b) - 1);
-      b &= b - 1;
-    }
+  while (b) {
+    queue[pos++] = (uint32_t)(w * 32 + 8 * sub) + (__ffs(b) - 1);
+    b &= b - 1;
   }
 }
 
@@ -785,6 +814,7 @@ struct BfsSearch {
       }
     }
     uint32_t nedges = 0;
+    const unsigned long long seq = ++ctx->mail_seq;
     if (small_push) {
       bfs_count<true><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, w.cand, depth,
                                            d + 1);
@@ -808,8 +838,7 @@ struct BfsSearch {
       bfs_count<false><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, nullptr,
                                             nullptr, 0);
     }
-    const unsigned long long seq = ++ctx->mail_seq;
-    bfs_scan<<<1, 1024, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? nedges : 0, ctx->d_mail, seq);
+    bfs_scan<<<1, 256, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? nedges : 0, ctx->d_mail, seq);
     // After a push the next level is most often pushed too: its queue is
     // listed now, while the host waits for the counters.
     if (!pull) bfs_write<<<nbc, 256, 0, st>>>(words, claim, w.offsets, w.q_a);
