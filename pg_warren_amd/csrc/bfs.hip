@@ -282,7 +282,59 @@ __global__ __launch_bounds__(256) void bfs_pull_blocks(
     const uint32_t groups = kBlockThreads / T, g = tid / T, l = tid % T;
     const uint32_t gbase = lane & ~(T - 1);
     const unsigned long long gmask = T == 64 ? ~0ull : ((1ull << T) - 1);
-    for (uint32_t r = g; r < nr; r += groups) {
+    if (T == 1) {
+      // A thread per row (rows tid and tid + 256): both rows' offsets, the
+      // search test and then their first entries are read together, so a
+      // row whose first entry meets the frontier costs three dependent
+      // reads; a row that goes on reads four entries at a time.
+      constexpr int R = kBlockRows / kBlockThreads;
+      uint32_t v[R], u[R];
+      uint64_t a[R], e[R], at[R];
+      bool act[R], hit[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t r = tid + kBlockThreads * k;
+        v[k] = b.row0 + (r < nr ? r : 0);
+        a[k] = roff[v[k]];
+        e[k] = roff[v[k] + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        act[k] = tid + kBlockThreads * k < nr && pull_searches(v[k], candidates, pass, cand, visited, filter,
+                                                                depth, d1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) u[k] = act[k] && a[k] < e[k] ? rtgt[a[k]] : 0u;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        hit[k] = act[k] && a[k] < e[k] && bit_of(frontier, u[k]);
+        at[k] = a[k];
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        if (!act[k]) continue;
+        for (uint64_t i = a[k] + 1; !hit[k] && i < e[k]; i += 4) {
+          uint32_t q[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) q[j] = i + j < e[k] ? rtgt[i + j] : 0u;
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (!hit[k] && i + j < e[k] && bit_of(frontier, q[j])) {
+              hit[k] = true;
+              u[k] = q[j];
+              at[k] = i + j;
+            }
+        }
+        if (hit[k]) {
+          pull_claims(v[k], u[k], pass, depth, via, d1);
+          atomicOr(&lbm[(v[k] >> 5) - w0], 1u << (v[k] & 31));
+          edges += at[k] - a[k] + 1;
+        } else {
+          edges += e[k] - a[k];
+        }
+        if (candidates) cand[v[k]] = 0;
+      }
+    }
+    for (uint32_t r = g; T > 1 && r < nr; r += groups) {
       const uint32_t v = b.row0 + r;
       if (pull_searches(v, candidates, pass, cand, visited, filter, depth, d1)) {
         const uint64_t a = roff[v], e = roff[v + 1];
