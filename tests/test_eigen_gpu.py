@@ -50,8 +50,12 @@ def check_vectors(vec, vals, csr, matrix, weighted=True, symmetrize=True, tol=1e
         assert r <= tol * max(1.0, abs(lam)), (l, lam, r)
         top = np.argmax(np.abs(x[:, l]))
         assert x[top, l] > 0
+    # Orthonormality: f32 eigenvectors carry their output rounding (each
+    # component within 2^-24 relative, so a Gram entry within 2^-23 of the
+    # exact vectors'), hence 1e-6 for them; the reference's own bound is
+    # sqrt(accuracy) (spectral/lanczos/tests.rs:200-203), far looser.
     g = xa.T @ xa
-    np.testing.assert_allclose(g, np.eye(len(vals)), atol=1e-7)
+    np.testing.assert_allclose(g, np.eye(len(vals)), atol=1e-7 if vec.dtype == "f64" else 1e-6)
 
 
 def test_path_laplacian_closed_form(ctx, oracle):
