@@ -101,12 +101,18 @@ struct PprArgs {
   unsigned long long* err;
   const unsigned long long* cancel;  // the context's host-mapped cancel word
   unsigned* stop;                    // device word: the cancel word, forwarded
+  double start;   // every live row's rank before the first iteration
+  uint32_t mode;  // 0: the first iteration (the old ranks are `start`, not read);
+                  // 1: the second (zero rows' ranks left alone); 2: later
+                  // (zero rows' messages left alone too)
+  double* y0;     // first iteration, tile 0: each row's gathered sum, or null
 };
 
 // A lane's columns: position t * kTw + kC * k + q is column c[q] when on[q].
 struct LaneCols {
   uint32_t c[kC];
   bool on[kC];
+  bool pair;  // both on, adjacent and 16-byte aligned in every row: one access
 };
 __device__ __forceinline__ LaneCols lane_cols(const PprArgs& a, uint32_t t, uint32_t k) {
   LaneCols lc;
@@ -116,35 +122,77 @@ __device__ __forceinline__ LaneCols lane_cols(const PprArgs& a, uint32_t t, uint
     lc.on[q] = pos < a.nact;
     lc.c[q] = lc.on[q] ? a.cols[pos] : 0;
   }
+  lc.pair = lc.on[0] && lc.on[1] && lc.c[1] == lc.c[0] + 1 && ((lc.c[0] | a.R) & 1) == 0;
   return lc;
 }
 
 // What a row's update reads besides its gathered sums.
+//
+// A zero row (dead, or with no in-edges and no seed share) ranks 0 in every
+// column once the first iteration has run: its sum is empty and its seed
+// share 0, so x' = d * (0 + dangling * 0) + (1 - d) * 0 = +0. From the second
+// iteration on its ranks are neither read nor written (they hold that 0),
+// its change and dangling terms are +0 and left out of the sums, and its
+// message is 0; from the third on, when both message buffers hold that 0 in
+// every position, the message is not written either.
 struct PprRowIn {
   double xo[kC];
   double o;
   uint32_t h, sr;
-  bool dead;
+  bool dead, zero;
 };
 
-__device__ __forceinline__ PprRowIn ppr_load(const PprArgs& a, uint64_t v, const LaneCols& lc) {
+__device__ __forceinline__ PprRowIn ppr_load(const PprArgs& a, uint64_t v, const LaneCols& lc, uint32_t sr,
+                                             bool dead, bool empty) {
   PprRowIn in;
+  in.sr = sr;
+  in.dead = dead;
+  in.zero = a.mode != 0 && (dead || (empty && sr == ~0u));
+  in.o = 0.0;
+  in.h = ~0u;
 #pragma unroll
-  for (uint32_t q = 0; q < kC; ++q) in.xo[q] = lc.on[q] ? ld_nt(a.X + v * a.R + lc.c[q]) : 0.0;
+  for (uint32_t q = 0; q < kC; ++q) in.xo[q] = 0.0;
+  if (in.zero) {
+    if (a.mode == 1) in.h = ld_nt(a.hot + v);
+    return in;
+  }
+  if (a.mode == 0) {
+#pragma unroll
+    for (uint32_t q = 0; q < kC; ++q)
+      if (lc.on[q]) in.xo[q] = dead ? 0.0 : a.start;
+  } else if (lc.pair) {
+    const dvec2 x2 = ld_nt(reinterpret_cast<const dvec2*>(a.X + v * a.R + lc.c[0]));
+    in.xo[0] = x2.x;
+    in.xo[1] = x2.y;
+  } else {
+#pragma unroll
+    for (uint32_t q = 0; q < kC; ++q)
+      if (lc.on[q]) in.xo[q] = ld_nt(a.X + v * a.R + lc.c[q]);
+  }
   in.o = ld_nt(a.out + v);
   in.h = ld_nt(a.hot + v);
-  in.sr = ld_nt(a.seed_row + v);
-  in.dead = a.tomb && ld_nt(a.tomb + v);
   return in;
+}
+
+// The same, the row's seed index and tombstone read from memory.
+__device__ __forceinline__ PprRowIn ppr_load_g(const PprArgs& a, uint64_t v, const LaneCols& lc, bool empty) {
+  return ppr_load(a, v, lc, ld_nt(a.seed_row + v), a.tomb && ld_nt(a.tomb + v), empty);
 }
 
 __device__ __forceinline__ void ppr_advance(const PprArgs& a, uint64_t v, const LaneCols& lc, uint32_t t,
                                             uint32_t k, const double* y, const PprRowIn& in, double* change,
                                             double* dang, uint32_t& err) {
-  double m[kC];
+  if (in.zero) {
+    if (in.h != ~0u) {
+      const dvec2 mm = {0.0, 0.0};
+      st_nt(reinterpret_cast<dvec2*>(a.Mout + ((uint64_t)t * a.nm + in.h) * kTw + kC * k), mm);
+    }
+    return;
+  }
+  double m[kC], nx[kC];
 #pragma unroll
   for (uint32_t q = 0; q < kC; ++q) {
-    m[q] = 0.0;
+    m[q] = nx[q] = 0.0;
     if (!lc.on[q]) continue;
     const uint32_t c = lc.c[q];
     double next = 0.0;
@@ -153,10 +201,18 @@ __device__ __forceinline__ void ppr_advance(const PprArgs& a, uint64_t v, const 
       const double flowed = sqlf<ADD_>(y[q], sqlf<MUL_>(a.dangling[c], p, err), err);
       next = sqlf<ADD_>(sqlf<MUL_>(a.damping, flowed, err), sqlf<MUL_>(a.teleport, p, err), err);
     }
-    st_nt(a.X + v * a.R + c, next);
+    nx[q] = next;
     if (sql_lt(0.0, in.o)) m[q] = sqlf<DIV_>(next, in.o, err);
     else dang[q] += next;
     change[q] += fabs(next - in.xo[q]);
+  }
+  if (lc.pair) {
+    const dvec2 x2 = {nx[0], nx[1]};
+    st_nt(reinterpret_cast<dvec2*>(a.X + v * a.R + lc.c[0]), x2);
+  } else {
+#pragma unroll
+    for (uint32_t q = 0; q < kC; ++q)
+      if (lc.on[q]) st_nt(a.X + v * a.R + lc.c[q], nx[q]);
   }
   if (in.h != ~0u) {
     const dvec2 mm = {m[0], m[1]};
@@ -192,12 +248,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
   __shared__ uint32_t ss[kItemNnz];
   __shared__ double sw[HW ? kItemNnz : 1];
   __shared__ uint64_t so[kItemRows + 1];
+  __shared__ uint32_t ssr[kItemRows];  // the rows' seed indices
+  __shared__ uint8_t sdead[kItemRows]; // and tombstones
   __shared__ double Y[kItemRows * kTw];          // rows summed within one run
   __shared__ double segf[kG * kTw], segl[kG * kTw];  // a run's first / last row, when cut
   const uint32_t tid = threadIdx.x, k = tid % kL, g = tid / kL;
   const LaneCols lc = lane_cols(a, t, k);
   const bool any_on = lc.on[0];
   const double* tile = a.Min + (uint64_t)t * a.nm * kTw + kC * k;
+  auto gather = [&](uint32_t s) -> Cols { return load_cols(tile + (uint64_t)s * kTw); };
   double change[kC] = {}, dang[kC] = {};
   uint32_t e_err = 0;
   __shared__ unsigned stop;
@@ -223,7 +282,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
     if (piece) {
       if (tid < 2) so[tid] = b0 + (tid ? nent : 0);
     } else {
-      for (uint32_t i = tid; i <= nr; i += 256) so[i] = ld_nt(a.off + (uint64_t)item.row0 + i);
+      for (uint32_t i = tid; i <= nr; i += 256) {
+        so[i] = ld_nt(a.off + (uint64_t)item.row0 + i);
+        if (i < nr) {
+          ssr[i] = ld_nt(a.seed_row + (uint64_t)item.row0 + i);
+          sdead[i] = a.tomb ? ld_nt(a.tomb + (uint64_t)item.row0 + i) : 0;
+        }
+      }
     }
     for (uint32_t i = tid; i < nent; i += 256) {
       ss[i] = ld_nt(a.src + b0 + i);
@@ -255,7 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
         if (nb == kGatherBatch) {
 #pragma unroll
           for (uint32_t j = 0; j < kGatherBatch; ++j) {
-            m[j] = load_cols(tile + (uint64_t)ss[e + j] * kTw);
+            m[j] = gather(ss[e + j]);
             ws[j] = HW ? sw[e + j] : 1.0;
           }
         } else {
@@ -264,11 +329,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
             m[j] = Cols{};
             ws[j] = 1.0;
             if (j < nb) {
-              m[j] = load_cols(tile + (uint64_t)ss[e + j] * kTw);
+              m[j] = gather(ss[e + j]);
               if (HW) ws[j] = sw[e + j];
             }
           }
         }
+
         if (e + nb <= rend) {
           // No row ends before the batch does.
 #pragma unroll
@@ -330,8 +396,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
         for (uint32_t b = 0; b < kBatch; ++b) {
           const uint32_t r = r0 + b * kG;
           if (r < nr) {
-            in[b] = ppr_load(a, (uint64_t)item.row0 + r, lc);
+            in[b] = ppr_load(a, (uint64_t)item.row0 + r, lc, ssr[r], sdead[r] != 0, so[r] == so[r + 1]);
             row_sum(r, y[b]);
+            if (a.y0 && k == 0) a.y0[item.row0 + r] = y[b][0];
           }
         }
 #pragma unroll
@@ -363,7 +430,8 @@ __global__ __launch_bounds__(256) void ppr_long(PprArgs a, uint32_t t, const uin
     for (uint32_t p = pbase[i]; p < pbase[i + 1]; ++p)
 #pragma unroll
       for (uint32_t q = 0; q < kC; ++q) y[q] += a.piece[((uint64_t)t * npieces + p) * kTw + kC * k + q];
-    ppr_advance(a, v, lc, t, k, y, ppr_load(a, v, lc), change, dang, e_err);
+    if (a.y0 && k == 0) a.y0[v] = y[0];
+    ppr_advance(a, v, lc, t, k, y, ppr_load_g(a, v, lc, false), change, dang, e_err);
   }
   ppr_block_partial(a, t, slot0 + blockIdx.x, change, dang, red);
   if (e_err) raise_error(a.err, blockIdx.x, e_err);
@@ -405,43 +473,79 @@ __global__ __launch_bounds__(256) void ppr_finish(const double* __restrict__ par
   }
 }
 
-// x = start at live rows, msg = x / out in the first layout (every column
-// active, position = column), and per-workgroup dangling partials.
-__global__ __launch_bounds__(256) void ppr_start(uint64_t n, uint32_t R, double start,
+// Before the first iteration every column holds x = start at live rows (0
+// at dead ones), so its messages are the same in every column: tile 0's
+// messages x / out (every position) and each workgroup's dangling sum,
+// which every tile's partial slot takes.
+__global__ __launch_bounds__(256) void ppr_start(uint64_t n, uint32_t ntiles, double start,
                                                  const uint8_t* __restrict__ tomb,
-                                                 const uint32_t* __restrict__ hot, uint64_t nm,
-                                                 const double* __restrict__ out, double* __restrict__ X,
-                                                 double* __restrict__ M, double* __restrict__ part,
-                                                 uint32_t nparts, unsigned long long* err) {
-  __shared__ double red[kRowStep][kTw];
-  const uint32_t tid = threadIdx.x, k = tid % kTw, r = tid / kTw;
+                                                 const uint32_t* __restrict__ hot, const double* __restrict__ out,
+                                                 double* __restrict__ M, double* __restrict__ part, uint32_t nparts,
+                                                 unsigned long long* err) {
+  __shared__ double red[256];
+  const uint32_t tid = threadIdx.x;
   uint32_t e_err = 0;
-  const uint32_t ntiles = (R + kTw - 1) / kTw;
-  for (uint32_t t = 0; t < ntiles; ++t) {
-    const uint32_t c = t * kTw + k;
-    double dang = 0.0;
-    for (uint64_t v = (uint64_t)blockIdx.x * kRowStep + r; v < n; v += (uint64_t)gridDim.x * kRowStep) {
-      if (c >= R) continue;
-      const double x = (tomb && tomb[v]) ? 0.0 : start;
-      X[v * R + c] = x;
-      const double o = out[v];
-      double m = 0.0;
-      if (sql_lt(0.0, o)) m = sqlf<DIV_>(x, o, e_err);
-      else dang += x;
-      if (hot[v] != ~0u) M[((uint64_t)t * nm + hot[v]) * kTw + k] = m;
+  double dang = 0.0;
+  for (uint64_t v = (uint64_t)blockIdx.x * 256 + tid; v < n; v += (uint64_t)gridDim.x * 256) {
+    const double x = (tomb && tomb[v]) ? 0.0 : start;
+    const double o = out[v];
+    double m = 0.0;
+    if (sql_lt(0.0, o)) m = sqlf<DIV_>(x, o, e_err);
+    else dang += x;
+    const uint32_t h = hot[v];
+    if (h != ~0u) {
+      const dvec2 mm = {m, m};
+      dvec2* dst = reinterpret_cast<dvec2*>(M + (uint64_t)h * kTw);
+#pragma unroll
+      for (uint32_t q = 0; q < kTw / 2; ++q) dst[q] = mm;
     }
-    red[r][k] = dang;
-    __syncthreads();
-    if (tid < kTw) {
-      double ds = 0.0;
-      for (uint32_t q = 0; q < kRowStep; ++q) ds += red[q][tid];
-      double* p = part + ((uint64_t)t * nparts + blockIdx.x) * 2 * kTw;
-      p[tid] = 0.0;
-      p[kTw + tid] = ds;
-    }
+  }
+  red[tid] = dang;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
     __syncthreads();
   }
+  if (tid < ntiles) {
+    double* p = part + ((uint64_t)tid * nparts + blockIdx.x) * 2 * kTw;
+#pragma unroll
+    for (uint32_t k = 0; k < kTw; ++k) {
+      p[k] = 0.0;
+      p[kTw + k] = red[0];
+    }
+  }
   if (e_err) raise_error(err, blockIdx.x, e_err);
+}
+
+// The first iteration of tiles 1 .. ntiles - 1 (tile blockIdx.y + 1), whose
+// gathered sums equal tile 0's column 0 (y0) bit for bit: every position's
+// messages are the same, and each is folded in the same order. Workgroup
+// blockIdx.x takes every gridDim.x-th run of kG rows and fills partial slot
+// blockIdx.x (gridDim.x is the partial slot count).
+__global__ __launch_bounds__(256) void ppr_first_rest(PprArgs a) {
+  __shared__ double red[2 * kG * kTw];
+  const uint32_t t = blockIdx.y + 1, tid = threadIdx.x, k = tid % kL, g = tid / kL;
+  const LaneCols lc = lane_cols(a, t, k);
+  uint32_t e_err = 0;
+  double change[kC] = {}, dang[kC] = {};
+  if (lc.on[0]) {
+    for (uint64_t v = (uint64_t)blockIdx.x * kG + g; v < a.n; v += (uint64_t)gridDim.x * kG) {
+      const double yv = ld_nt(a.y0 + v);
+      double y[kC];
+#pragma unroll
+      for (uint32_t q = 0; q < kC; ++q) y[q] = yv;
+      ppr_advance(a, v, lc, t, k, y, ppr_load_g(a, v, lc, false), change, dang, e_err);
+    }
+  }
+  ppr_block_partial(a, t, blockIdx.x, change, dang, red);
+  if (e_err) raise_error(a.err, blockIdx.x, e_err);
+}
+
+// The ranks of a call that runs no iteration: start at live rows.
+__global__ __launch_bounds__(256) void ppr_fill_start(uint64_t n, uint32_t R, double start,
+                                                      const uint8_t* __restrict__ tomb, double* __restrict__ X) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n * R; i += (uint64_t)gridDim.x * 256)
+    X[i] = (tomb && tomb[i / R]) ? 0.0 : start;
 }
 
 // Moves the surviving positions' messages into a packed layout: position
@@ -651,6 +755,7 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   uint32_t *seed_row_d, *cols_d, *from_d, *iters_d;
   unsigned* stop_d;
   uint8_t* active_d;
+  double* y0 = nullptr;
   if ((s = get((void**)&M0, tiles_bytes)) || (s = get((void**)&M1, tiles_bytes)) ||
       (s = get((void**)&out, std::max<uint64_t>(n, 1) * 8)) ||
       (s = get((void**)&part, (size_t)ntiles * nparts * 2 * kTw * 8)) ||
@@ -661,6 +766,8 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
       (s = get((void**)&iters_d, R * 4)) || (s = get((void**)&active_d, R)) ||
       (s = get((void**)&stop_d, 4)))
     return s;
+  // Each row's first-iteration sum (tile 0's), which the other tiles take.
+  if (ntiles > 1 && (s = get((void**)&y0, std::max<uint64_t>(n, 1) * 8))) return s;
   WG_HIP(ctx, hipMemsetAsync(stop_d, 0, 4, st));
   WG_HIP(ctx, hipMemcpyAsync(Pd, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
   if (n) WG_HIP(ctx, hipMemcpyAsync(seed_row_d, seed_row.data(), n * 4, hipMemcpyHostToDevice, st));
@@ -671,7 +778,7 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   if ((s = launch_out_weights(ctx, g, weighted, out, out))) return s;
   // Slots other than ppr_start's hold zeros for the set-up sums.
   WG_HIP(ctx, hipMemsetAsync(part, 0, (size_t)ntiles * nparts * 2 * kTw * 8, st));
-  ppr_start<<<grid, 256, 0, st>>>(n, R, start, tomb, g->hot_index, nm, out, X, M0, part, nparts, ctx->d_err);
+  ppr_start<<<grid, 256, 0, st>>>(n, ntiles, start, tomb, g->hot_index, out, M0, part, nparts, ctx->d_err);
   std::vector<uint32_t> active_cols(R);
   for (uint32_t c = 0; c < R; ++c) active_cols[c] = c;
   WG_HIP(ctx, hipMemcpyAsync(cols_d, active_cols.data(), R * 4, hipMemcpyHostToDevice, st));
@@ -680,12 +787,16 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
   if ((s = collect_error(ctx, "pagerank batch"))) return s;
   const double* wv = weighted && rev.weights ? rev.weights : nullptr;
   const uint32_t max_iter = bound_err ? 1 : o->max_iter;
+  if (max_iter == 0 && n) ppr_fill_start<<<2048, 256, 0, st>>>(n, R, start, tomb, X);
   std::vector<uint8_t> act(R, 1);
   double *Min = M0, *Mout = M1;
   // An iteration's launches (every tile's persistent pass and split rows,
   // then the columns' finish) are captured once as a HIP graph per message
   // parity and replayed while the active columns stay the same; a repack
   // captures them again. Kernel launches only: nothing in them is a memset.
+  // The first iteration (tile 0's pass and the other tiles from its sums)
+  // and the second run eagerly; from the third on the launches do not
+  // change while the active columns stay the same.
   struct Captured {
     hipGraphExec_t exec = nullptr;
     const double* min = nullptr;
@@ -708,18 +819,19 @@ extern "C" int wg_pagerank_batch(wg_context* ctx, const wg_graph* gc, const wg_v
     const uint32_t tiles = (nact + kTw - 1) / kTw;
     PprArgs a{rev.offsets, g->hot_src, wv, n, g->hot_index, nm, R, nact, cols_d, out, tomb, seed_row_d, Pd,
               o->damping, teleport, dangling, X, Min, Mout, part, piece, nparts, ctx->d_err,
-              ctx->d_mail + kMailCancel, stop_d};
+              ctx->d_mail + kMailCancel, stop_d, start, std::min<uint32_t>(it, 2), it == 0 ? y0 : nullptr};
     auto enqueue = [&]() {
-      for (uint32_t t = 0; t < tiles; ++t) {
+      for (uint32_t t = 0; t < (it == 0 ? 1 : tiles); ++t) {
         if (wv) ppr_block<true><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
         else ppr_block<false><<<grid, 256, 0, st>>>(a, t, items, nitems, npieces);
         if (long_blocks)
           ppr_long<<<long_blocks, 256, 0, st>>>(a, t, g->ppr_lrows, g->ppr_lbase, nlong, npieces, grid);
       }
+      if (it == 0 && tiles > 1) ppr_first_rest<<<dim3(nparts, tiles - 1), 256, 0, st>>>(a);
       ppr_finish<<<nact, 256, 0, st>>>(part, nparts, cols_d, dangling, iters_d, active_d, bound,
                                        o->has_tol && !bound_err, 1);
     };
-    if (ctx->graph_launch) {
+    if (ctx->graph_launch && it >= 2) {
       Captured& c = cap[Min == M0 ? 0 : 1];
       if (!c.exec || c.min != Min || c.nact != nact) {
         if (c.exec) (void)hipGraphExecDestroy(c.exec);
