@@ -439,32 +439,32 @@ __global__ __launch_bounds__(256) void bfs_pull_long(
 }
 
 // Compaction, second pass (one workgroup of 256): an exclusive scan of the
-// blocks' counts into offsets, the entry sums and the pulled level's
-// scanned entries, posted to the context's mailbox (values first, then the
-// sequence number with release).
+// blocks' counts into offsets, and the sums of their entries, reverse
+// entries and (a pulled level's) scanned entries, posted to the context's
+// mailbox (values first, then the sequence number with release).
 __global__ __launch_bounds__(256) void bfs_scan(uint32_t nb, const unsigned long long* __restrict__ part,
                                                 unsigned long long* __restrict__ offsets,
-                                                const unsigned long long* __restrict__ part_edges,
-                                                uint32_t nedges, unsigned long long* __restrict__ mail,
-                                                unsigned long long seq) {
+                                                unsigned long long* __restrict__ mail, unsigned long long seq) {
   __shared__ unsigned long long sh[256];
   __shared__ unsigned long long red[4];
   const uint32_t tid = threadIdx.x;
-  const uint32_t per = (nb + 255) / 256, b0 = tid * per;
+  const uint32_t per = (nb + 255) / 256, b0 = tid * per, b1 = min(b0 + per, nb);
   unsigned long long local = 0, ent = 0, vr = 0, ed = 0;
-  for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
-    local += part[3 * b];
-    ent += part[3 * b + 1];
-    vr += part[3 * b + 2];
-  }
-  {
-    // Eight loads in flight a thread (the partials are many after a pull).
-    unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t k = tid;
-    for (; k + 7 * 256 < nedges; k += 8 * 256)
-      for (int j = 0; j < 8; ++j) acc[j] += part_edges[k + j * 256];
-    for (; k < nedges; k += 256) acc[0] += part_edges[k];
-    for (int j = 0; j < 8; ++j) ed += acc[j];
+  // A thread's blocks eight at a time, their loads in flight together.
+  constexpr uint32_t U = 8;
+  for (uint32_t b = b0; b < b1; b += U) {
+    unsigned long long p[U][4];
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j)
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) p[j][i] = b + j < b1 ? part[4 * (b + j) + i] : 0ull;
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) {
+      local += p[j][0];
+      ent += p[j][1];
+      vr += p[j][2];
+      ed += p[j][3];
+    }
   }
   sh[tid] = local;
   __syncthreads();
@@ -475,9 +475,15 @@ __global__ __launch_bounds__(256) void bfs_scan(uint32_t nb, const unsigned long
     __syncthreads();
   }
   unsigned long long run = sh[tid] - local;
-  for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
-    offsets[b] = run;
-    run += part[3 * b];
+  for (uint32_t b = b0; b < b1; b += U) {
+    unsigned long long c[U];
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) c[j] = b + j < b1 ? part[4 * (b + j)] : 0ull;
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) {
+      if (b + j < b1) offsets[b + j] = run;
+      run += c[j];
+    }
   }
   ent = block_reduce_sum(ent, red);
   vr = block_reduce_sum(vr, red);
@@ -497,16 +503,23 @@ __global__ __launch_bounds__(256) void bfs_scan(uint32_t nb, const unsigned long
 // eight nodes, their offsets read together (2048 blocks at 2^22 nodes, so
 // that enough loads are in flight).
 // kClaim (after a small push, whose vias are set): the claims come from the
-// candidate bytes here, and the claimed nodes take depth d1.
+// candidate bytes here, and the claimed nodes take depth d1. A pulled
+// level's per-block scanned entries (nedges of them) are summed too, an
+// equal slice a block, into each block's fourth partial.
 template <bool kClaim>
 __global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uint32_t* __restrict__ claim,
                                                  uint32_t* __restrict__ visited,
                                                  uint32_t* __restrict__ old_frontier, const RowSets adj,
                                                  const RowSets rev, unsigned long long* __restrict__ part,
                                                  uint8_t* __restrict__ cand, uint32_t* __restrict__ depth,
-                                                 uint32_t d1) {
+                                                 uint32_t d1, const unsigned long long* __restrict__ part_edges,
+                                                 uint32_t nedges) {
   __shared__ unsigned long long red[4];
-  unsigned long long cnt = 0, ent = 0, vr = 0;
+  unsigned long long cnt = 0, ent = 0, vr = 0, ed = 0;
+  {
+    const uint32_t per = (nedges + gridDim.x - 1) / gridDim.x, e0 = blockIdx.x * per;
+    for (uint32_t e = e0 + threadIdx.x; e < e0 + per && e < nedges; e += 256) ed += part_edges[e];
+  }
   // Four threads a word, eight nodes each (the lanes of a word are adjacent
   // in the wave).
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x, w = t >> 2;
@@ -565,10 +578,12 @@ __global__ __launch_bounds__(256) void bfs_count(uint64_t n, uint64_t words, uin
   cnt = block_reduce_sum(cnt, red);
   ent = block_reduce_sum(ent, red);
   vr = block_reduce_sum(vr, red);
+  ed = block_reduce_sum(ed, red);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = cnt;
-    part[3 * blockIdx.x + 1] = ent;
-    part[3 * blockIdx.x + 2] = vr;
+    part[4 * blockIdx.x] = cnt;
+    part[4 * blockIdx.x + 1] = ent;
+    part[4 * blockIdx.x + 2] = vr;
+    part[4 * blockIdx.x + 3] = ed;
   }
 }
 
@@ -595,6 +610,208 @@ __global__ __launch_bounds__(256) void bfs_write(uint64_t words, const uint32_t*
   while (b) {
     queue[pos++] = (uint32_t)(w * 32 + 8 * sub) + (__ffs(b) - 1);
     b &= b - 1;
+  }
+}
+
+// Small push levels in one workgroup, level after level without a return
+// to the host: while a level's frontier rows hold at most kTinyEntries
+// entries and the direction switch keeps pushing (the same test as
+// BfsSearch::step), the 1024 threads push it as bfs_push<true> and
+// bfs_count<true> do. An unvisited target admitted by the filter takes the
+// smallest frontier node with an edge to it as its via (atomicMin) and is
+// claimed once (the claim bitmap's atomicOr returns its old bit), into the
+// next queue; the claimed nodes take depth d + 1 and join the visited
+// bitmap, their row and reverse-row entries are summed, and the old
+// frontier's bits are cleared (the bitmap becomes the next claim map). The
+// queue a level leaves is in claim order: a push reads its frontier in any
+// order. Each level's counters are posted to the mailbox (levels first).
+// visited is read with device-scope loads: the claims' atomics update it in
+// L2, past this CU's L1.
+constexpr uint64_t kTinyEntries = 16384, kTinyNodes = 4096;
+constexpr uint32_t kTinyLevels = 9;  // 1 + 3 per level in the mailbox's 30 values
+
+// Exclusive scan of x over a 1024-thread workgroup; *total the sum.
+__device__ __forceinline__ uint32_t scan_1024(uint32_t x, uint32_t* wsum, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = x;
+#pragma unroll
+  for (uint32_t k = 1; k < 64; k <<= 1) {
+    const uint32_t y = __shfl_up(inc, k, 64);
+    if (lane >= k) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (uint32_t w = 0; w < 16; ++w) {
+    const uint32_t t = wsum[w];
+    if (w < wave) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - x;
+}
+struct TinyState {
+  unsigned long long frontier_size, mf, visited_rev, visited;
+  uint32_t d;
+};
+
+__global__ __launch_bounds__(1024) void bfs_tiny_levels(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ tgt, const uint64_t* __restrict__ roff,
+    uint64_t n, uint64_t rr_nnz, int pulling, int64_t max_depth, uint32_t* visited, uint32_t* fa, uint32_t* fb,
+    uint32_t* qa, uint32_t* qb, const uint8_t* __restrict__ filter, uint32_t* __restrict__ depth,
+    uint32_t* __restrict__ via, TinyState st, unsigned long long* __restrict__ mail, unsigned long long seq) {
+  __shared__ uint32_t qcount;
+  __shared__ unsigned long long red[16], tot[2];
+  __shared__ unsigned long long recs[3 * kTinyLevels];
+  // The frontier's rows: node, first entry, and entry offset (exclusive
+  // prefix of the row lengths, pre[F] the level's entries).
+  __shared__ uint32_t nod[kTinyNodes], pre[kTinyNodes + 1], wsum[16];
+  __shared__ uint64_t rowa[kTinyNodes];
+  const uint32_t tid = threadIdx.x;
+  unsigned long long F = st.frontier_size, mf = st.mf, vrev = st.visited_rev, vis = st.visited;
+  uint32_t d = st.d, L = 0;
+  while (L < kTinyLevels) {
+    if (F == 0 || (max_depth >= 0 && (int64_t)d >= max_depth)) break;
+    const unsigned long long m_u = rr_nnz - vrev;
+    const bool pull = pulling && F * 24 > n && (m_u < mf || vis == n);
+    if (pull || mf > kTinyEntries || F > kTinyNodes || mf * kSmallPushRatio >= n) break;
+    if (tid == 0) qcount = 0;
+    {
+      // Up to four frontier nodes a thread, their rows' lengths scanned.
+      const uint32_t per = (uint32_t)((F + 1023) / 1024), i0 = tid * per;
+      uint32_t len[4], un[4], my = 0;
+      uint64_t ra[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        len[k] = 0;
+        un[k] = 0;
+        ra[k] = 0;
+        if (k < per && i0 + k < F) {
+          un[k] = qa[i0 + k];
+          ra[k] = off[un[k]];
+          len[k] = (uint32_t)(off[un[k] + 1] - ra[k]);
+          my += len[k];
+        }
+      }
+      uint32_t total;
+      uint32_t base = scan_1024(my, wsum, &total);
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (k < per && i0 + k < F) {
+          nod[i0 + k] = un[k];
+          rowa[i0 + k] = ra[k];
+          pre[i0 + k] = base;
+          base += len[k];
+        }
+      if (tid == 0) pre[F] = total;
+    }
+    __syncthreads();
+    const uint32_t ne = pre[F];
+    // The level's entries, four a thread in flight: entry e lies in the row
+    // of the last frontier node whose offset is at most e.
+    for (uint32_t e0 = tid; e0 < ne; e0 += 4 * 1024) {
+      uint32_t v[4], u[4], word[4];
+      bool ok[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t e = e0 + k * 1024;
+        ok[k] = e < ne;
+        v[k] = u[k] = 0;
+        if (ok[k]) {
+          uint32_t lo = 0, hi = (uint32_t)F;  // pre[lo] <= e < pre[hi]
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= e) lo = mid;
+            else hi = mid;
+          }
+          u[k] = nod[lo];
+          v[k] = tgt[rowa[lo] + (e - pre[lo])];
+        }
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        word[k] = ok[k] ? __hip_atomic_load(&visited[v[k] >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0u;
+      bool go[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) go[k] = ok[k] && !((word[k] >> (v[k] & 31)) & 1u);
+      if (filter) {
+        uint8_t f[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) f[k] = go[k] ? filter[v[k]] : 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) go[k] = go[k] && f[k];
+      }
+      // The four entries' atomics go out together; the claim's old bit
+      // decides which thread queues the node.
+      uint32_t old[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (go[k]) atomicMin(&via[v[k]], u[k]);
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) old[k] = go[k] ? atomicOr(&fb[v[k] >> 5], 1u << (v[k] & 31)) : ~0u;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (!((old[k] >> (v[k] & 31)) & 1u)) qb[atomicAdd(&qcount, 1u)] = v[k];
+    }
+    __syncthreads();
+    const uint32_t Fn = qcount;
+    unsigned long long ent = 0, vr = 0;
+    // The claimed nodes, four a thread in flight.
+    for (uint32_t i0 = tid; i0 < Fn; i0 += 4 * 1024) {
+      uint32_t v[4];
+      uint64_t a0[4], a1[4], r0[4], r1[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) v[k] = i0 + k * 1024 < Fn ? qb[i0 + k * 1024] : ~0u;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (v[k] != ~0u) {
+          depth[v[k]] = d + 1;
+          atomicOr(&visited[v[k] >> 5], 1u << (v[k] & 31));
+          a0[k] = off[v[k]];
+          a1[k] = off[v[k] + 1];
+          r0[k] = roff[v[k]];
+          r1[k] = roff[v[k] + 1];
+        }
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (v[k] != ~0u) {
+          ent += a1[k] - a0[k];
+          vr += r1[k] - r0[k];
+        }
+    }
+    for (uint64_t i = tid; i < F; i += 1024) {
+      const uint32_t u = qa[i];
+      atomicAnd(&fa[u >> 5], ~(1u << (u & 31)));
+    }
+    ent = block_reduce_sum(ent, red);
+    vr = block_reduce_sum(vr, red);
+    if (tid == 0) {
+      recs[3 * L] = Fn;
+      recs[3 * L + 1] = ent;
+      recs[3 * L + 2] = vr;
+      tot[0] = ent;
+      tot[1] = vr;
+    }
+    __syncthreads();
+    F = Fn;
+    mf = tot[0];
+    vrev += tot[1];
+    vis += Fn;
+    ++d;
+    ++L;
+    uint32_t* t = qa;
+    qa = qb;
+    qb = t;
+    t = fa;
+    fa = fb;
+    fb = t;
+  }
+  if (tid == 0) {
+    __hip_atomic_store(&mail[1], (unsigned long long)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (uint32_t i = 0; i < 3 * L; ++i)
+      __hip_atomic_store(&mail[2 + i], recs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&mail[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -682,7 +899,7 @@ static void workspace_sizes(const wg_graph* g, const Traversal& t, size_t sizes[
     pieces = std::max<uint64_t>(pieces, c->npartials);
     parts += edge_parts(*c);
   }
-  const size_t s[kWsSlots] = {3 * words * 4, 2 * n * 4, pieces * 4, n, std::max<uint64_t>(nbc, 1) * 3 * 8,
+  const size_t s[kWsSlots] = {3 * words * 4, 2 * n * 4, pieces * 4, n, std::max<uint64_t>(nbc, 1) * 4 * 8,
                               std::max<uint64_t>(nbc, 1) * 8, std::max<uint64_t>(parts, 1) * 8};
   for (int k = 0; k < kWsSlots; ++k) sizes[k] = s[k];
 }
@@ -776,6 +993,10 @@ struct BfsSearch {
   uint32_t d = 0;
   uint32_t *frontier = nullptr, *claim = nullptr;
   bool queue_ready = true;  // q_a holds the frontier as an ascending list
+  // Small push levels may run several to a launch (bfs_tiny_levels), which
+  // leaves the queue in claim order: for searches whose levels are not
+  // listed (bfs_run), over one CSR each way.
+  bool tiny_levels = false;
   BfsRun run;
 
   int init(wg_context* c, wg_graph* gr, const Traversal& tr, const std::vector<uint32_t>& starts,
@@ -852,6 +1073,9 @@ struct BfsSearch {
     // resolves the candidates by the pull kernel (each scanning its reverse
     // row up to the first frontier node).
     const bool small_push = !pull && (multi || mf * kSmallPushRatio < n);
+    if (small_push && tiny_levels && !multi && t.rrows.size() == 1 && mf <= kTinyEntries &&
+        frontier_size <= kTinyNodes)
+      return tiny_step();
     if (!pull) {
       if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
       const unsigned qblocks = (unsigned)std::min<uint64_t>((frontier_size + 3) / 4, 256 * 64);
@@ -869,7 +1093,7 @@ struct BfsSearch {
     const unsigned long long seq = ++ctx->mail_seq;
     if (small_push) {
       bfs_count<true><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, w.cand, depth,
-                                           d + 1);
+                                           d + 1, nullptr, 0);
     } else {
       // One pass per reversed CSR (one after a push: its candidates).
       for (size_t j = 0; j < t.rrows.size(); ++j) {
@@ -888,9 +1112,9 @@ struct BfsSearch {
         if (!pull) break;
       }
       bfs_count<false><<<nbc, 256, 0, st>>>(n, words, claim, w.visited, frontier, ra, rr, w.part, nullptr,
-                                            nullptr, 0);
+                                            nullptr, 0, w.part_edges, pull ? nedges : 0);
     }
-    bfs_scan<<<1, 256, 0, st>>>(nbc, w.part, w.offsets, w.part_edges, pull ? nedges : 0, ctx->d_mail, seq);
+    bfs_scan<<<1, 256, 0, st>>>(nbc, w.part, w.offsets, ctx->d_mail, seq);
     // After a push the next level is most often pushed too: its queue is
     // listed now, while the host waits for the counters.
     if (!pull) bfs_write<<<nbc, 256, 0, st>>>(words, claim, w.offsets, w.q_a);
@@ -915,6 +1139,42 @@ struct BfsSearch {
     return WG_OK;
   }
 
+  // Small push levels, as many as bfs_tiny_levels takes in one launch
+  // (at least this one, whose test step() has made).
+  int tiny_step() {
+    hipStream_t st = ctx->stream;
+    if (!queue_ready) bfs_write<<<nbc, 256, 0, st>>>(words, frontier, w.offsets, w.q_a);
+    const unsigned long long seq = ++ctx->mail_seq;
+    const TinyState ts{frontier_size, mf, visited_rev, run.visited, d};
+    bfs_tiny_levels<<<1, 1024, 0, st>>>(t.rows[0]->offsets, t.rows[0]->targets, t.rrows[0]->offsets, n,
+                                        rr_nnz, pulling ? 1 : 0, max_depth, w.visited, frontier, claim, w.q_a,
+                                        w.q_b, filter, depth, via, ts, ctx->d_mail, seq);
+    WG_HIP(ctx, hipGetLastError());
+    unsigned long long rec[1 + 3 * kTinyLevels];
+    int s = wait_mail(ctx, seq, rec, 1 + 3 * kTinyLevels);
+    if (s) return s;
+    const uint32_t L = (uint32_t)rec[0];
+    if (L == 0 || L > kTinyLevels) return fail(ctx, WG_DEVICE_ERROR, "small BFS levels: no level ran");
+    for (uint32_t l = 0; l < L; ++l) {
+      // A pushed level scans every entry of every frontier row.
+      run.edges += mf;
+      run.push_edges += mf;
+      frontier_size = rec[1 + 3 * l];
+      mf = rec[2 + 3 * l];
+      visited_rev += rec[3 + 3 * l];
+      run.visited += frontier_size;
+      run.level_pulled.push_back(0);
+      run.level_size.push_back(frontier_size);
+      ++d;
+      // Each level swaps the queues and the frontier / claim bitmaps.
+      std::swap(frontier, claim);
+      std::swap(w.q_a, w.q_b);
+    }
+    run.levels = d;
+    queue_ready = true;  // in claim order, which only a push reads
+    return WG_OK;
+  }
+
   // The current level's nodes, ascending, into q_a (the next push's queue).
   int list_frontier() {
     if (!queue_ready && frontier_size)
@@ -936,6 +1196,7 @@ int bfs_run(wg_context* ctx, wg_graph* g, const Traversal& t, const std::vector<
   if (s) return s;
   BfsSearch S;
   if ((s = S.init(ctx, g, t, starts, filter_d, max_depth, pulling, depth, via, w))) return s;
+  S.tiny_levels = true;
   while (!S.exhausted())
     if ((s = S.step())) return s;
   run = S.run;
