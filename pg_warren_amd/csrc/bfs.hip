@@ -97,9 +97,10 @@ __global__ __launch_bounds__(256) void bfs_seed(const uint32_t* __restrict__ sta
 }
 
 // A search's set-up in one launch for at most kInitStarts starts: depth and
-// via unvisited, the three bitmaps (visited, frontier, claims) clear, and
-// the starts at depth 0 in the visited and frontier bitmaps. Each element
-// is written by one thread, its start bits included.
+// via unvisited, the three bitmaps (visited, frontier, claims) clear, the
+// starts at depth 0 in the visited and frontier bitmaps, and the starts, in
+// order, as the first queue. Each element is written by one thread, its
+// start bits included.
 constexpr uint32_t kInitStarts = 8;
 struct InitStarts {
   uint32_t v[kInitStarts];
@@ -108,8 +109,9 @@ struct InitStarts {
 
 __global__ __launch_bounds__(256) void bfs_init(uint64_t n, uint64_t words, uint32_t* __restrict__ depth,
                                                 uint32_t* __restrict__ via, uint32_t* __restrict__ bitmaps,
-                                                InitStarts st) {
+                                                uint32_t* __restrict__ queue, InitStarts st) {
   const uint64_t total = std::max<uint64_t>(n, 3 * words);
+  if (blockIdx.x == 0 && threadIdx.x < st.count) queue[threadIdx.x] = st.v[threadIdx.x];
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
     if (i < n) {
       uint32_t dv = kUnvisited;
@@ -1027,9 +1029,7 @@ struct BfsSearch {
       for (size_t k = 0; k < starts.size(); ++k) is.v[k] = starts[k];
       const uint64_t total = std::max<uint64_t>(n, 3 * words);
       bfs_init<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 8192)), 256, 0, st>>>(
-          n, words, depth, via, w.visited, is);
-      if (!starts.empty())
-        WG_HIP(ctx, hipMemcpyAsync(w.q_a, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, st));
+          n, words, depth, via, w.visited, w.q_a, is);
     } else {
       WG_HIP(ctx, hipMemsetAsync(depth, 0xff, std::max<uint64_t>(n, 1) * 4, st));
       WG_HIP(ctx, hipMemsetAsync(via, 0xff, std::max<uint64_t>(n, 1) * 4, st));
