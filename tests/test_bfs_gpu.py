@@ -145,3 +145,41 @@ def test_hub_rows_pushed_in_pieces(ctx, oracle):
         d1, v1 = d1.numpy(), v1.numpy()
         d2, v2, _ = W.bfs_levels(ctx, g, np.atleast_1d(s))
         assert np.array_equal(d1, d2.numpy()) and np.array_equal(v1, v2.numpy())
+
+
+def test_small_levels_with_filters_and_depth_limits(ctx, oracle):
+    """Searches whose push levels fit one workgroup's level loop (R-MAT scale
+    16: frontier rows under n / 4 and 16,384 entries, run several levels to
+    a launch), with node filters, depth limits and several starts: depths
+    and vias equal the oracle's push, pulled or not, and a pushed-only
+    search counts its visits and scanned entries as the oracle's push."""
+    off, tgt = W.rmat(16, 16, seed=7)
+    ref = oracle.Csr(len(off) - 1, off, tgt)
+    n = ref.n
+    g = up(ctx, ref)
+    rng = np.random.default_rng(21)
+    deg = np.diff(off.astype(np.int64))
+    live = np.flatnonzero(deg > 0)
+    flt = (rng.random(n) < 0.85).astype(np.uint8)
+    cases = [[int(s)] for s in rng.choice(live, 3, replace=False)]
+    cases.append([int(s) for s in rng.choice(live, 5, replace=False)])
+    for starts in cases:
+        flt[starts] = 1
+        for f in (None, flt):
+            for md in (None, 1, 3):
+                print(f"small levels: starts {starts} filter={f is not None} max_depth={md}", flush=True)
+                push, wst = oracle.bfs(ref, starts, node_filter=f, max_depth=md)
+                wd = np.full(n, 2**32 - 1, np.uint32)
+                wv = np.full(n, 2**32 - 1, np.uint32)
+                for v, dep, u in push:
+                    wd[v] = dep
+                    if u is not None:
+                        wv[v] = u
+                for pulling in (False, True):
+                    d, via, st = W.bfs_levels(ctx, g, starts, node_filter=f, max_depth=md, pulling=pulling)
+                    assert np.array_equal(d.numpy(), wd) and np.array_equal(via.numpy(), wv), pulling
+                    if not pulling:
+                        assert st["nodes_visited"] == wst["nodes_visited"]
+                        assert st["edges_traversed"] == wst["edges_traversed"]
+                    d.close()
+                    via.close()
