@@ -129,3 +129,27 @@ def test_graph_replays_equal_eager_launches(ctx, weighted):
     finally:
         ctx.set_graph_launch(True)
     g.close()
+
+
+@pytest.mark.parametrize("max_iter", [1, 2, 60])
+def test_first_iterations_tombstones_and_a_partial_tile(ctx, oracle, max_iter):
+    """40 seed sets (three tiles of 16, the last partial) over a weighted
+    graph with tombstones, stopped after the first iteration (every column
+    from tile 0's gathered sums), the second (zero rows' ranks left alone) or
+    run to convergence: every column against the oracle's PageRank with
+    that seed vector."""
+    ref = graph(oracle, 4000, 20000, 40, True)
+    g = W.Graph(ctx, ref.n, ref.offsets, ref.targets, ref.weights)
+    rng = np.random.default_rng(41)
+    sets = seed_sets(rng, ref.n, 40)
+    tomb = (rng.random(ref.n) < 0.03).astype(np.uint8)
+    for s in sets:
+        tomb[s] = 0
+    ranks, its = W.pagerank_batch(ctx, g, sets, tol=1e-10, max_iter=max_iter, tombstones=tomb)
+    got = ranks.numpy().reshape(ref.n, 40)
+    for c in range(0, 40, 3):
+        want, wit = oracle.pagerank(ref, tol=1e-10, max_iter=max_iter, tombstones=tomb,
+                                    seed=dense(ref.n, sets[c]))
+        assert its[c] == wit, (c, its[c], wit)
+        assert np.abs(got[:, c] - want).sum() < L1_TOL, c
+        assert np.all(got[tomb == 1, c] == 0.0)
