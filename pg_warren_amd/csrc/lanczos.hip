@@ -502,6 +502,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eigen_dev(double* __restrict_
   __shared__ double sh[kEigThreads / 64];
   __shared__ double qs[LDSQ ? kEigLdsM * kEigLdsM : 1];
   __shared__ int ctl[4];  // l, h, done, failed
+  __shared__ int lmax[kEigThreads / 64];
   const int tid = threadIdx.x;
   double* const q_ = LDSQ ? qs : qt;
   auto A = [&](int i, int j) -> double& { return a[(size_t)i * m + j]; };
@@ -597,35 +598,59 @@ __global__ __launch_bounds__(kEigThreads) void sym_eigen_dev(double* __restrict_
       } else if (++steps > 60 * m) {
         ctl[2] = 1;
         ctl[3] = 1;
-      } else {
-        int l = h - 1;
-        while (l > 0 && fabs(e[l - 1]) > eps * (fabs(d[l - 1]) + fabs(d[l]))) --l;
+      }
+    }
+    __syncthreads();
+    if (ctl[2]) break;
+    {
+      // The unreduced block ending at h starts at l, the largest i < h whose
+      // e[i - 1] is negligible beside its diagonal neighbours (0 if none):
+      // every thread tests its i, a block maximum picks l.
+      const int h = ctl[1];
+      int cand = 0;
+      for (int i = tid + 1; i < h; i += kEigThreads)
+        if (!(fabs(e[i - 1]) > eps * (fabs(d[i - 1]) + fabs(d[i])))) cand = i;
+      for (int o = 32; o > 0; o >>= 1) cand = max(cand, __shfl_xor(cand, o, 64));
+      if ((tid & 63) == 0) lmax[tid >> 6] = cand;
+      __syncthreads();
+      if (tid == 0) {
+        int l = 0;
+        for (int w = 0; w < kEigThreads / 64; ++w) l = max(l, lmax[w]);
         if (l > 0) e[l - 1] = 0.0;
         ctl[0] = l;
         const double delta = (d[h - 1] - d[h]) / 2.0;
         const double eh = e[h - 1];
         const double mu = d[h] - eh * eh / (delta + (delta >= 0.0 ? 1.0 : -1.0) * hypot(delta, eh));
         double x = d[l] - mu, z = e[l];
+        // The chain's running d[k] and e[k] stay in registers (each step's
+        // d[k + 1] and scaled e[k + 1] are read back by the next step);
+        // every other element is read from LDS before it is written.
+        double dk = d[l], ek = e[l];
         for (int k = l; k < h; ++k) {
+          const double dk1 = d[k + 1];
+          const double ek1 = k + 1 < h ? e[k + 1] : 0.0;
           const double r = hypot(x, z);
           const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
           if (k > l) e[k - 1] = r;
-          const double dk = d[k], ek = e[k], dk1 = d[k + 1];
-          d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
-          d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
-          e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
-          if (k + 1 < h) {
-            z = s * e[k + 1];
-            e[k + 1] = c * e[k + 1];
-            x = e[k];
-          }
+          const double ndk = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
+          const double ndk1 = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
+          const double nek = c * s * (dk1 - dk) + (c * c - s * s) * ek;
+          d[k] = ndk;
+          e[k] = nek;
           cs[k] = c;
           sn[k] = s;
+          if (k + 1 < h) {
+            z = s * ek1;
+            ek = c * ek1;
+            x = nek;
+          } else {
+            d[k + 1] = ndk1;
+          }
+          dk = ndk1;
         }
       }
     }
     __syncthreads();
-    if (ctl[2]) break;
     // The sweep's rotations, a column of Q^T per thread, in order.
     const int l = ctl[0], h = ctl[1];
     for (int r = tid; r < m; r += kEigThreads) {
