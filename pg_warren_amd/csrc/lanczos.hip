@@ -83,6 +83,22 @@ __device__ __forceinline__ E round_to(double v) {
 
 // out[c] = sum over b of part[b * ncols + c], in a fixed order; a workgroup
 // a column.
+// A step's coefficients (2 kMaxBasis) and scalars (8) into coherent
+// page-locked host memory, then the context mailbox's sequence number with
+// release: the host polls one word instead of copying each value and
+// synchronising the stream. One wavefront: every lane's stores are fenced
+// at system scope before lane 0 posts.
+constexpr uint32_t kPostWords = 2 * kMaxBasis + 8;
+__global__ __launch_bounds__(64) void lz_post(const double* __restrict__ coef, const double* __restrict__ scal,
+                                              double* post, unsigned long long* mail, unsigned long long seq) {
+  for (uint32_t i = threadIdx.x; i < kPostWords; i += 64) {
+    const double v = i < 2 * kMaxBasis ? coef[i] : scal[i - 2 * kMaxBasis];
+    __hip_atomic_store(&post[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  __threadfence_system();
+  if (threadIdx.x == 0) __hip_atomic_store(&mail[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(256) void reduce_cols(const double* __restrict__ part, uint32_t nblocks,
                                                    uint32_t ncols, double* __restrict__ out) {
   __shared__ double sh[4];
@@ -837,6 +853,8 @@ struct Eigen {
   double *input = nullptr, *scaled = nullptr, *output = nullptr, *scratch = nullptr, *deg = nullptr, *scale = nullptr;
   uint8_t* active = nullptr;
   double *part = nullptr, *coef = nullptr, *scal = nullptr, *topv = nullptr;
+  // Coherent page-locked host words a step's results are posted to (lz_post).
+  double *h_post = nullptr, *d_post = nullptr;
   uint64_t* topi = nullptr;
   uint32_t nblocks = 0;
   std::vector<uint8_t> h_active;
@@ -859,6 +877,7 @@ struct Eigen {
   ~Eigen() {
     (void)hipStreamSynchronize(st);
     for (auto& h : held) dfree(ctx, h.first, h.second);
+    if (h_post) (void)hipHostFree(h_post);
   }
   template <class T>
   int get(T** p, size_t count) {
@@ -912,6 +931,11 @@ struct Eigen {
         (s = get(&part, (size_t)nblocks * kMaxBasis)) || (s = get(&coef, 2 * kMaxBasis)) || (s = get(&scal, 8)) ||
         (s = get(&topv, nblocks)) || (s = get(&topi, nblocks)))
       return s;
+    if (hipHostMalloc((void**)&h_post, kPostWords * 8, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&d_post, h_post, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ctx, WG_OUT_OF_MEMORY, "page-locked host words for the step's results");
+    }
     weighted = (o.weighted < 0 ? g->weighted : o.weighted != 0) && g->fwd.weights;
     normalized = o.matrix == WG_NORMALIZED_LAPLACIAN || o.matrix == WG_RANDOM_WALK;
     transpose = o.symmetrize && !g->rev_is_fwd;
@@ -1288,16 +1312,17 @@ struct Solver {
     }
     if (deflated) op.deflate_enqueue(w, dd);
     WG_HIP(ctx, hipGetLastError());
-    std::vector<double> hf(len), hc(len);
-    double ha = 0.0, hn = 0.0, hd = 0.0;
-    if (len) {
-      if (!recurrence) WG_HIP(ctx, hipMemcpyAsync(hf.data(), first, len * 8, hipMemcpyDeviceToHost, st));
-      WG_HIP(ctx, hipMemcpyAsync(hc.data(), corr, len * 8, hipMemcpyDeviceToHost, st));
-    }
-    if (recurrence) WG_HIP(ctx, hipMemcpyAsync(&ha, alpha, 8, hipMemcpyDeviceToHost, st));
-    WG_HIP(ctx, hipMemcpyAsync(&hn, nd, 8, hipMemcpyDeviceToHost, st));
-    if (deflated) WG_HIP(ctx, hipMemcpyAsync(&hd, dd, 8, hipMemcpyDeviceToHost, st));
-    WG_HIP(ctx, hipStreamSynchronize(st));
+    // first, corr and the scalars, posted to host memory in one round trip.
+    const unsigned long long seq = ++ctx->mail_seq;
+    lz_post<<<1, 64, 0, st>>>(op.coef, op.scal, op.d_post, ctx->d_mail, seq);
+    WG_HIP(ctx, hipGetLastError());
+    int s = wait_mail(ctx, seq, nullptr, 0);
+    if (s) return s;
+    const double* hp = op.h_post;
+    std::vector<double> hf(hp + (first - op.coef), hp + (first - op.coef) + len);
+    std::vector<double> hc(hp + (corr - op.coef), hp + (corr - op.coef) + len);
+    const double ha = hp[2 * kMaxBasis + (alpha - op.scal)], hn = hp[2 * kMaxBasis + (nd - op.scal)];
+    const double hd = hp[2 * kMaxBasis + (dd - op.scal)];
     if (deflated) *deflated = std::sqrt(hd);
     if (recurrence) {
       coefs[len - 2] = prev;
