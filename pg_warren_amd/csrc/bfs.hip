@@ -24,7 +24,10 @@
 // The next frontier's queue and counters come from the claim bitmap by a
 // count / scan / write compaction with per-block partials; the scan posts
 // the counters to the context's mailbox (coherent host memory), which the
-// host polls, so a level costs no copy or stream synchronisation. The hot
+// host polls, so a level costs no copy or stream synchronisation.
+// Small push levels of a search that lists no levels run in one workgroup,
+// several to a launch (bfs_tiny_levels), without the compaction's passes
+// over the whole bitmap or a return to the host between them. The hot
 // path has no device-scope atomics on a shared address (on gfx950 they
 // resolve past the per-XCD L2 and serialise: an arrival counter bumped by
 // each of the count pass's 2048 blocks cost more than the scan's launch).

@@ -25,6 +25,12 @@
 // long rows go to a buffer that ppr_long sums in piece order. Per-column L1
 // changes and dangling ranks go to per-workgroup partial slots that
 // ppr_finish sums in a fixed order. No atomics and no cross-workgroup fences.
+//
+// The first iteration starts every column at x = 1 / live, so every column's
+// messages, and each row's sum, are the same: tile 0's pass records the sums
+// and ppr_first_rest advances the other tiles from them. From the second
+// iteration on, zero rows (no in-edges and no seed share, or dead), which
+// then hold 0 in every column, are left alone (PprRowIn).
 #include <algorithm>
 #include <functional>
 #include <cmath>
