@@ -146,10 +146,37 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
   // A set halt word (an iterative caller's stop flag) makes the launch a no-op.
   if (halt && __hip_atomic_load(halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   constexpr bool RX = MUL != FIRST && MUL != PAIR;  // the product reads x
-  __shared__ E hot[HOTN];
+  __shared__ __attribute__((aligned(16))) E hot[HOTN];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
   if constexpr (RX) {
-    for (uint32_t i = tid; i < HOTN; i += 1024) hot[i] = i < nhot ? xhot[i] : E();
+    // The hot run staged in 16-byte units, a thread's loads all in flight
+    // before its LDS stores (one load, wait and store at a time left the
+    // workgroup waiting some 38 L2 round trips before its first tile).
+    // Units wholly inside the run are vector loads from xhot (an allocation
+    // of at least nhot elements, 256-byte aligned; a thread whose unit lies
+    // past them loads the last one again and stores zeros); the unit the
+    // run ends inside is then completed element by element by the thread
+    // that stored it; slots past the run hold E().
+    constexpr uint32_t PER = 16 / sizeof(E);
+    constexpr uint32_t UNITS = HOTN / PER;
+    static_assert(HOTN % PER == 0, "the hot run is a whole number of 16-byte units");
+    constexpr uint32_t STEPS = (UNITS + 1023) / 1024;
+    const uint32_t full = min(nhot, HOTN) / PER;
+    const v4 zero = {0u, 0u, 0u, 0u};
+    v4 buf[STEPS];
+    if (full) {
+#pragma unroll
+      for (uint32_t s = 0; s < STEPS; ++s)
+        buf[s] = *(const v4*)(xhot + (uint64_t)min(tid + s * 1024, full - 1) * PER);
+    }
+#pragma unroll
+    for (uint32_t s = 0; s < STEPS; ++s) {
+      const uint32_t u = tid + s * 1024;
+      if (u < UNITS) *(v4*)(hot + (uint64_t)u * PER) = u < full ? buf[s] : zero;
+    }
+    if (full < UNITS && tid == full % 1024)
+      for (uint32_t i = full * PER; i < nhot && i < (full + 1) * PER; ++i) hot[i] = xhot[i];
     __syncthreads();
   }
   const uint32_t gw = blockIdx.x * kTileWaves + wave;
@@ -177,7 +204,6 @@ __global__ __launch_bounds__(1024) void spmv_tiles(
   // that their latencies overlap (a wave with fewer tiles left loads its
   // last tile again and drops the copy).
   constexpr uint32_t U = sizeof(E) <= 4 ? 2 : 1;
-  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
   for (uint32_t t = t0; t < t1; t += U) {
     uint32_t TQ[U][8];
     E WQ[U][8];
