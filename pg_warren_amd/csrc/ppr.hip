@@ -296,9 +296,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HW ? 3 : 4)
         }
       }
     }
-    for (uint32_t i = tid; i < nent; i += 256) {
-      ss[i] = ld_nt(a.src + b0 + i);
-      if (HW) sw[i] = ld_nt(a.w + b0 + i);
+    // The item's entries, a thread's eight loads in flight before its LDS
+    // stores (addresses clamped to the item's last entry, stores guarded).
+    if (nent) {
+      constexpr uint32_t kS = kItemNnz / 256;
+      uint32_t sv[kS];
+      double wv[kS];
+#pragma unroll
+      for (uint32_t k = 0; k < kS; ++k) {
+        const uint64_t e = b0 + min(tid + k * 256, nent - 1);
+        sv[k] = ld_nt(a.src + e);
+        if (HW) wv[k] = ld_nt(a.w + e);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kS; ++k) {
+        const uint32_t i = tid + k * 256;
+        if (i < nent) {
+          ss[i] = sv[k];
+          if (HW) sw[i] = wv[k];
+        }
+      }
     }
     __syncthreads();
     if (stop) break;
